@@ -1,0 +1,14 @@
+"""adtomo_hip: MI355X (gfx950) HIP implementation of ADTomo.jl's Eikonal ops.
+
+Public API (mirrors src/eikonal_op.jl and the registered gradient ops):
+    eikonal, eikonal_grad, eikonal3d, eikonal3d_grad,
+    eikonal3d_batched, EikonalGraph, eikonal3d_plain, eikonal3d_grad_plain.
+"""
+from .ops import (EikonalGraph, eikonal, eikonal3d, eikonal3d_batched, eikonal3d_grad,
+                  eikonal3d_grad_plain, eikonal3d_plain, eikonal_grad, last_info,
+                  selftest_arith)
+from ._lib import LIB_PATH, AdtError, lib
+
+__all__ = ["eikonal", "eikonal_grad", "eikonal3d", "eikonal3d_grad", "eikonal3d_batched",
+           "EikonalGraph", "eikonal3d_plain", "eikonal3d_grad_plain", "selftest_arith",
+           "last_info", "lib", "LIB_PATH", "AdtError"]

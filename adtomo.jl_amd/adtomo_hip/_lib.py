@@ -1,0 +1,124 @@
+"""ctypes binding of libadtomo_hip.so (the C ABI in include/adtomo_hip.h).
+
+The product path has no CPU fallback: if the HIP library is missing this
+module raises at load time.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "libadtomo_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_PKG_ROOT), "include", "adtomo_hip.h")
+
+ADT_OK = 0
+ADT_HOST_POINTERS = 0x1
+ADT_FORWARD = 0x2
+ADT_ADJOINT = 0x4
+MAX_LOGGED_ITERS = 100
+
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_f = ctypes.c_float
+_p = ctypes.c_void_p
+
+
+class AdtInfo(ctypes.Structure):
+    _fields_ = [
+        ("iterations", _i32),
+        ("converged", _i32),
+        ("err", _d),
+        ("ambiguous", _i32),
+        ("adj_sweeps", _i32),
+        ("adj_converged", _i32),
+        ("reserved", _i32),
+        ("n_singular", _i64),
+        ("err_log", _d * MAX_LOGGED_ITERS),
+    ]
+
+    def as_dict(self):
+        k = max(0, min(self.iterations, MAX_LOGGED_ITERS))
+        return {
+            "iterations": self.iterations,
+            "converged": bool(self.converged),
+            "err": self.err,
+            "ambiguous": bool(self.ambiguous),
+            "adj_sweeps": self.adj_sweeps,
+            "adj_converged": bool(self.adj_converged),
+            "n_singular": self.n_singular,
+            "err_log": list(self.err_log[:k]),
+        }
+
+
+class AdtEvent3d(ctypes.Structure):
+    _fields_ = [
+        ("m", _i64), ("n", _i64), ("l", _i64),
+        ("h", _d), ("tol", _d),
+        ("u0", _p), ("f", _p), ("grad_u", _p),
+        ("u", _p), ("grad_u0", _p), ("grad_f", _p),
+    ]
+
+
+class AdtError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> (restype, argtypes); REAL is filled per dtype suffix.
+_SIGS = {
+    "adt_eikonal2d_fwd": [_p, _i64, _i64, _i64, _i64, "R", _p, _p, _i32, _p],
+    "adt_eikonal2d_bwd": [_p, _p, _p, _i64, _i64, _i64, _i64, "R", _p, _p, _i32, _p],
+    "adt_eikonal3d_fwd": [_p, _p, "R", _i64, _i64, _i64, "R", _i32, _p, _p, _i32, _p],
+    "adt_eikonal3d_bwd": [_p, _p, _p, _p, "R", _i64, _i64, _i64, _p, _p, _p, _i32, _p],
+    "adt_eikonal3d_batched": [_p, _i32, _i32, _p, _p],
+    "adt_selftest_arith": [_i32, _p, _p, _p, _i64, _p],
+}
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AdtError(f"libadtomo_hip.so not found at {LIB_PATH}; build it with "
+                       f"`make -C adtomo.jl_amd` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    for base, args in _SIGS.items():
+        for sfx, real in (("_f64", _d), ("_f32", _f)):
+            fn = getattr(L, base + sfx)
+            fn.argtypes = [real if a == "R" else a for a in args]
+            fn.restype = _i32
+    L.adt_eikonal3d_fwd_plain_f64.argtypes = [_p, _p, _d, _i64, _i64, _i64, _d, _p, _p, _p]
+    L.adt_eikonal3d_fwd_plain_f64.restype = _i32
+    L.adt_eikonal3d_bwd_plain_f64.argtypes = [_p, _p, _p, _p, _d, _i64, _i64, _i64, _p, _p, _p, _p]
+    L.adt_eikonal3d_bwd_plain_f64.restype = _i32
+    L.adt_graph_create.argtypes = [_p, _i32, _i32, _i32, ctypes.POINTER(_p), _p]
+    L.adt_graph_create.restype = _i32
+    L.adt_graph_launch.argtypes = [_p, _p]
+    L.adt_graph_launch.restype = _i32
+    L.adt_graph_infos.argtypes = [_p, _p, _p]
+    L.adt_graph_infos.restype = _i32
+    L.adt_graph_destroy.argtypes = [_p]
+    L.adt_graph_destroy.restype = _i32
+    L.adt_version.restype = ctypes.c_char_p
+    L.adt_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != ADT_OK:
+        msg = lib().adt_last_error().decode(errors="replace")
+        raise AdtError(f"{what} failed (rc={rc}): {msg}")
+
+
+def declared_symbols(header: str = HEADER_PATH):
+    """Function names declared in include/adtomo_hip.h."""
+    import re
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(adt_[a-z0-9_]+)\s*\(", txt)))

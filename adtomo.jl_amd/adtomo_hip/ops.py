@@ -1,0 +1,365 @@
+"""Python mirror of ADTomo.jl's Eikonal op API over the HIP C ABI.
+
+Same names, argument meaning and layouts as src/eikonal_op.jl:
+
+  eikonal(f, srcx, srcy, h)                     src/eikonal_op.jl:3-21
+  eikonal3d(u0, f, h, m, n, l, tol, verbose)    src/eikonal_op.jl:24-32
+
+and the gradient ops the reference registers (EikonalGrad,
+Eikonal.cpp:48-63; EikonalThreeDGrad, EikonalThreeD.cpp:47-65).
+
+Inputs may be torch tensors on a HIP device (device pointers are handed to
+the library, outputs are torch tensors, and autograd flows through the
+adjoint kernels like the reference's ADCME gradient) or numpy arrays (the
+library stages them through device memory; outputs are numpy arrays).
+There is no CPU fallback: every call runs the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+last_info: dict = {}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_torch(x) -> bool:
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _sfx(dtype) -> str:
+    name = str(dtype)
+    return "_f32" if name.endswith("float32") else "_f64"
+
+
+def _np_dtype(x):
+    return np.float32 if np.asarray(x).dtype == np.float32 else np.float64
+
+
+def _set_info(info: L.AdtInfo):
+    global last_info
+    last_info = info.as_dict()
+    return last_info
+
+
+# ------------------------------------------------------------------- 2D ---
+
+def _eik2d_fwd_raw(f, srcx: int, srcy: int, h: float):
+    n1, m1 = f.shape
+    m, n = m1 - 1, n1 - 1
+    info = L.AdtInfo()
+    if _is_torch(f):
+        f = f.contiguous()
+        u = _torch().empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal2d_fwd" + _sfx(f.dtype))(
+            f.data_ptr(), m, n, srcx, srcy, h, u.data_ptr(), ctypes.byref(info), 0, _stream())
+    else:
+        f = np.ascontiguousarray(f, dtype=_np_dtype(f))
+        u = np.empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal2d_fwd" + _sfx(f.dtype))(
+            f.ctypes.data, m, n, srcx, srcy, h, u.ctypes.data, ctypes.byref(info),
+            L.ADT_HOST_POINTERS, None)
+    L.check(rc, "eikonal")
+    _set_info(info)
+    return u
+
+
+def eikonal_grad(grad_u, u, f, srcx: int, srcy: int, h: float):
+    """EikonalGrad (Eikonal.cpp:141-221): returns grad_f, shape of f."""
+    n1, m1 = f.shape
+    m, n = m1 - 1, n1 - 1
+    info = L.AdtInfo()
+    if _is_torch(f):
+        g, u, f = grad_u.contiguous(), u.contiguous(), f.contiguous()
+        gf = _torch().empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal2d_bwd" + _sfx(f.dtype))(
+            g.data_ptr(), u.data_ptr(), f.data_ptr(), m, n, srcx, srcy, h, gf.data_ptr(),
+            ctypes.byref(info), 0, _stream())
+    else:
+        dt = _np_dtype(f)
+        g, u, f = (np.ascontiguousarray(a, dtype=dt) for a in (grad_u, u, f))
+        gf = np.empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal2d_bwd" + _sfx(dt))(
+            g.ctypes.data, u.ctypes.data, f.ctypes.data, m, n, srcx, srcy, h, gf.ctypes.data,
+            ctypes.byref(info), L.ADT_HOST_POINTERS, None)
+    L.check(rc, "eikonal_grad")
+    _set_info(info)
+    return gf
+
+
+def _make_eik2d_fn():
+    torch = _torch()
+
+    class Eikonal2dFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, f, srcx, srcy, h):
+            u = _eik2d_fwd_raw(f.detach(), srcx, srcy, h)
+            ctx.save_for_backward(u, f)
+            ctx.args = (srcx, srcy, h)
+            return u
+
+        @staticmethod
+        def backward(ctx, grad_u):
+            u, f = ctx.saved_tensors
+            srcx, srcy, h = ctx.args
+            gf = eikonal_grad(grad_u.contiguous(), u, f.detach(), srcx, srcy, h)
+            return gf, None, None, None
+
+    return Eikonal2dFn
+
+
+_Eik2dFn = None
+
+
+def eikonal(f, srcx: int, srcy: int, h: float):
+    """2D first-arrival travel times (src/eikonal_op.jl:3-21).
+
+    ``f`` is the slowness on an (n+1) x (m+1) grid (rows = depth), ``srcx``
+    and ``srcy`` the 1-based source column and row, ``h`` the spacing.
+    Returns u with the shape of ``f``."""
+    global _Eik2dFn
+    if _is_torch(f):
+        if _Eik2dFn is None:
+            _Eik2dFn = _make_eik2d_fn()
+        return _Eik2dFn.apply(f, int(srcx), int(srcy), float(h))
+    return _eik2d_fwd_raw(np.asarray(f), int(srcx), int(srcy), float(h))
+
+
+# ------------------------------------------------------------------- 3D ---
+
+def _eik3d_fwd_raw(u0, f, h, m, n, l, tol, verbose):
+    info = L.AdtInfo()
+    if _is_torch(f):
+        u0 = u0.reshape(m, n, l).contiguous()
+        f = f.reshape(m, n, l).contiguous()
+        u = _torch().empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal3d_fwd" + _sfx(f.dtype))(
+            u0.data_ptr(), f.data_ptr(), h, m, n, l, tol, int(verbose), u.data_ptr(),
+            ctypes.byref(info), 0, _stream())
+    else:
+        dt = _np_dtype(f)
+        u0 = np.ascontiguousarray(np.reshape(u0, (m, n, l)), dtype=dt)
+        f = np.ascontiguousarray(np.reshape(f, (m, n, l)), dtype=dt)
+        u = np.empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal3d_fwd" + _sfx(dt))(
+            u0.ctypes.data, f.ctypes.data, h, m, n, l, tol, int(verbose), u.ctypes.data,
+            ctypes.byref(info), L.ADT_HOST_POINTERS, None)
+    L.check(rc, "eikonal3d")
+    _set_info(info)
+    return u
+
+
+def eikonal3d_grad(grad_u, u, u0, f, h, m, n, l, tol=None, verbose=False):
+    """EikonalThreeDGrad (EikonalThreeD.cpp:154-250): returns (grad_u0, grad_f)."""
+    info = L.AdtInfo()
+    if _is_torch(f):
+        g, u, u0, f = (a.reshape(m, n, l).contiguous() for a in (grad_u, u, u0, f))
+        gu0 = _torch().empty_like(f)
+        gf = _torch().empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal3d_bwd" + _sfx(f.dtype))(
+            g.data_ptr(), u.data_ptr(), u0.data_ptr(), f.data_ptr(), h, m, n, l,
+            gu0.data_ptr(), gf.data_ptr(), ctypes.byref(info), 0, _stream())
+    else:
+        dt = _np_dtype(f)
+        g, u, u0, f = (np.ascontiguousarray(np.reshape(a, (m, n, l)), dtype=dt)
+                       for a in (grad_u, u, u0, f))
+        gu0 = np.empty_like(f)
+        gf = np.empty_like(f)
+        rc = getattr(L.lib(), "adt_eikonal3d_bwd" + _sfx(dt))(
+            g.ctypes.data, u.ctypes.data, u0.ctypes.data, f.ctypes.data, h, m, n, l,
+            gu0.ctypes.data, gf.ctypes.data, ctypes.byref(info), L.ADT_HOST_POINTERS, None)
+    L.check(rc, "eikonal3d_grad")
+    _set_info(info)
+    return gu0, gf
+
+
+def _make_eik3d_fn():
+    torch = _torch()
+
+    class Eikonal3dFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, u0, f, h, m, n, l, tol, verbose):
+            u = _eik3d_fwd_raw(u0.detach(), f.detach(), h, m, n, l, tol, verbose)
+            ctx.save_for_backward(u, u0, f)
+            ctx.args = (h, m, n, l, tol, verbose)
+            return u
+
+        @staticmethod
+        def backward(ctx, grad_u):
+            u, u0, f = ctx.saved_tensors
+            h, m, n, l, tol, verbose = ctx.args
+            gu0, gf = eikonal3d_grad(grad_u.contiguous(), u, u0.detach(), f.detach(), h, m, n, l)
+            # The reference returns gradients only for u0 and f
+            # (EikonalThreeD.cpp:154-250); h, dims, tol, verbose get none.
+            return (gu0.reshape(u0.shape), gf.reshape(f.shape), None, None, None, None, None,
+                    None)
+
+    return Eikonal3dFn
+
+
+_Eik3dFn = None
+
+
+def eikonal3d(u0, f, h, m, n, l, tol, verbose=False):
+    """3D travel times by fast sweeping (src/eikonal_op.jl:24-32).
+
+    ``u0`` (initial times, e.g. 1000 with the source times set) and ``f``
+    (slowness) hold m*n*l values, row-major (m, n, l).  Returns (m, n, l)."""
+    global _Eik3dFn
+    m, n, l = int(m), int(n), int(l)
+    if _is_torch(f):
+        if _Eik3dFn is None:
+            _Eik3dFn = _make_eik3d_fn()
+        return _Eik3dFn.apply(u0, f, float(h), m, n, l, float(tol), bool(verbose))
+    return _eik3d_fwd_raw(np.asarray(u0), np.asarray(f), float(h), m, n, l, float(tol), verbose)
+
+
+# -------------------------------------------------------------- batched ---
+
+def _event_table(u0s, fs, hs, tols, grad_us, us, grad_u0s, grad_fs):
+    nev = len(fs)
+    tab = (L.AdtEvent3d * max(nev, 1))()
+    dtype = None
+    for e in range(nev):
+        f = fs[e]
+        if not _is_torch(f) or not f.is_cuda or not f.is_contiguous():
+            raise ValueError("batched calls take contiguous torch tensors on the HIP device")
+        if dtype is None:
+            dtype = f.dtype
+        elif f.dtype != dtype:
+            raise ValueError("all events must share one dtype")
+        m, n, l = f.shape
+        ev = tab[e]
+        ev.m, ev.n, ev.l = m, n, l
+        ev.h = float(hs[e])
+        ev.tol = float(tols[e])
+        ev.f = f.data_ptr()
+        ev.u0 = u0s[e].data_ptr() if u0s is not None else None
+        ev.u = us[e].data_ptr()
+        ev.grad_u = grad_us[e].data_ptr() if grad_us is not None else None
+        ev.grad_u0 = grad_u0s[e].data_ptr() if grad_u0s is not None else None
+        ev.grad_f = grad_fs[e].data_ptr() if grad_fs is not None else None
+    return tab, dtype
+
+
+def eikonal3d_batched(u0s, fs, hs, tols, grad_us=None, forward=True, us=None,
+                      return_info=True):
+    """Many independent 3D events (boxes of any size) in one persistent launch.
+
+    Lists of contiguous (m, n, l) device tensors.  With ``grad_us`` the
+    adjoint runs too.  Returns dict(u, grad_u0, grad_f, info)."""
+    torch = _torch()
+    nev = len(fs)
+    if us is None:
+        us = [torch.empty_like(f) for f in fs]
+    flags = (L.ADT_FORWARD if forward else 0) | (L.ADT_ADJOINT if grad_us is not None else 0)
+    gu0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
+    gfs = [torch.empty_like(f) for f in fs] if grad_us is not None else None
+    tab, dtype = _event_table(u0s, fs, hs, tols, grad_us, us, gu0s, gfs)
+    infos = (L.AdtInfo * max(nev, 1))() if return_info else None
+    rc = getattr(L.lib(), "adt_eikonal3d_batched" + _sfx(dtype))(
+        ctypes.byref(tab), nev, flags, ctypes.byref(infos) if infos is not None else None,
+        _stream())
+    L.check(rc, "eikonal3d_batched")
+    return {"u": us, "grad_u0": gu0s, "grad_f": gfs,
+            "info": [infos[e].as_dict() for e in range(nev)] if infos is not None else None}
+
+
+class EikonalGraph:
+    """hipGraph of the batched forward (+ adjoint) over fixed buffers.
+
+    Build once, then ``launch()`` repeatedly; the event buffers (inputs and
+    outputs, all kept alive by this object) are read/written in place."""
+
+    def __init__(self, u0s, fs, hs, tols, grad_us=None, forward=True):
+        torch = _torch()
+        self.u0s, self.fs, self.grad_us = u0s, fs, grad_us
+        self.us = [torch.empty_like(f) for f in fs]
+        self.grad_u0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
+        self.grad_fs = [torch.empty_like(f) for f in fs] if grad_us is not None else None
+        self.nev = len(fs)
+        flags = (L.ADT_FORWARD if forward else 0) | (L.ADT_ADJOINT if grad_us is not None else 0)
+        tab, dtype = _event_table(u0s, fs, hs, tols, grad_us, self.us, self.grad_u0s, self.grad_fs)
+        self._g = ctypes.c_void_p()
+        rc = L.lib().adt_graph_create(ctypes.byref(tab), self.nev, flags,
+                                      1 if str(dtype).endswith("float32") else 0,
+                                      ctypes.byref(self._g), _stream())
+        L.check(rc, "adt_graph_create")
+
+    def launch(self):
+        L.check(L.lib().adt_graph_launch(self._g, _stream()), "adt_graph_launch")
+
+    def infos(self):
+        arr = (L.AdtInfo * max(self.nev, 1))()
+        L.check(L.lib().adt_graph_infos(self._g, ctypes.byref(arr), _stream()), "adt_graph_infos")
+        return [arr[e].as_dict() for e in range(self.nev)]
+
+    def close(self):
+        if self._g:
+            L.lib().adt_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------- plain baseline ---
+
+def eikonal3d_plain(u0, f, h, tol):
+    """Per-event launch loop (one launch per hyperplane, host-synchronised
+    stop test per iteration); FP64 torch device tensors."""
+    torch = _torch()
+    u = torch.empty_like(f)
+    m, n, l = f.shape
+    info = L.AdtInfo()
+    rc = L.lib().adt_eikonal3d_fwd_plain_f64(u0.data_ptr(), f.data_ptr(), float(h), m, n, l,
+                                             float(tol), u.data_ptr(), ctypes.byref(info),
+                                             _stream())
+    L.check(rc, "eikonal3d_plain")
+    _set_info(info)
+    return u
+
+
+def eikonal3d_grad_plain(grad_u, u, u0, f, h):
+    torch = _torch()
+    m, n, l = f.shape
+    gu0 = torch.empty_like(f)
+    gf = torch.empty_like(f)
+    info = L.AdtInfo()
+    rc = L.lib().adt_eikonal3d_bwd_plain_f64(grad_u.data_ptr(), u.data_ptr(), u0.data_ptr(),
+                                             f.data_ptr(), float(h), m, n, l, gu0.data_ptr(),
+                                             gf.data_ptr(), ctypes.byref(info), _stream())
+    L.check(rc, "eikonal3d_grad_plain")
+    _set_info(info)
+    return gu0, gf
+
+
+def selftest_arith(op: int, a, b=None):
+    """Runs one device arithmetic op of the path on torch device tensors."""
+    torch = _torch()
+    n = a.numel() // (3 if op == 4 else 2 if op == 5 else 1)
+    out = torch.empty(n, dtype=a.dtype, device=a.device)
+    fn = getattr(L.lib(), "adt_selftest_arith" + _sfx(a.dtype))
+    rc = fn(op, a.data_ptr(), b.data_ptr() if b is not None else None, out.data_ptr(), n,
+            _stream())
+    L.check(rc, "selftest_arith")
+    return out

@@ -1,0 +1,702 @@
+// C ABI of libadtomo_hip.so (declared in include/adtomo_hip.h).
+//
+// Host-side orchestration only: argument checks, workspace, descriptor
+// tables, launches, diagnostics.  All arithmetic runs in the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define ADT_HIP(call)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(e_ == hipErrorOutOfMemory ? ADT_ENOMEM : ADT_EHIP,                     \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                    \
+  } while (0)
+
+constexpr int kMaxIter3 = 20;         // Eikonal3D.cpp:74
+constexpr int kMaxIter2 = 100;        // Eikonal.h:73
+constexpr int kMaxAdjSweeps = 8 * 64;  // cap of the adjoint fixed-point sweeps
+constexpr int kWG = 1024;
+
+// Stream-ordered device allocation that is released when the scope ends.
+struct DevBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+  int alloc(size_t bytes, hipStream_t st) {
+    s = st;
+    hipError_t e = hipMallocAsync(&p, bytes ? bytes : 1, st);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(ADT_ENOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    }
+    return ADT_OK;
+  }
+};
+
+int check_dims3(int64_t m, int64_t n, int64_t l) {
+  if (m < 2 || n < 2 || l < 2)
+    return fail(ADT_EINVAL, "eikonal3d: every dimension must be >= 2 (m=" + std::to_string(m) +
+                                ", n=" + std::to_string(n) + ", l=" + std::to_string(l) + ")");
+  if (m > (1 << 30) || n > (1 << 30) || l > (1 << 30) || m * n * l > (int64_t(1) << 40))
+    return fail(ADT_EINVAL, "eikonal3d: dimensions too large");
+  return ADT_OK;
+}
+
+int check_dims2(int64_t m, int64_t n, int64_t srcx, int64_t srcy) {
+  if (m < 1 || n < 1) return fail(ADT_EINVAL, "eikonal2d: m and n must be >= 1");
+  if (m >= (1 << 30) || n >= (1 << 30)) return fail(ADT_EINVAL, "eikonal2d: dimensions too large");
+  if (srcx < 1 || srcx > m + 1 || srcy < 1 || srcy > n + 1)
+    return fail(ADT_EINVAL, "eikonal2d: source (" + std::to_string(srcx) + "," +
+                                std::to_string(srcy) + ") outside the 1-based grid");
+  return ADT_OK;
+}
+
+void print_verbose(const adt_info& inf) {
+  for (int i = 0; i < inf.iterations && i < ADT_MAX_LOGGED_ITERS; ++i)
+    printf("Iteration %d, Error = %0.6e\n", i, inf.err_log[i]);
+}
+
+// Copies host input arrays in, or aliases device pointers.
+struct Staged {
+  std::vector<DevBuf> bufs;
+  int in(const void* host, size_t bytes, bool host_ptrs, const void** dev, hipStream_t s) {
+    if (!host_ptrs) {
+      *dev = host;
+      return ADT_OK;
+    }
+    bufs.emplace_back();
+    int rc = bufs.back().alloc(bytes, s);
+    if (rc) return rc;
+    if (host) ADT_HIP(hipMemcpyAsync(bufs.back().p, host, bytes, hipMemcpyHostToDevice, s));
+    *dev = bufs.back().p;
+    return ADT_OK;
+  }
+  int out(void* host, size_t bytes, bool host_ptrs, void** dev, hipStream_t s) {
+    if (!host_ptrs || !host) {
+      *dev = host;
+      return ADT_OK;
+    }
+    bufs.emplace_back();
+    int rc = bufs.back().alloc(bytes, s);
+    if (rc) return rc;
+    *dev = bufs.back().p;
+    return ADT_OK;
+  }
+};
+
+// ------------------------------------------------------------ 3D forward ---
+
+template <typename T>
+int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T tol, int verbose,
+              T* u, adt_info* info, int flags, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!u0 || !f || !u) return fail(ADT_EINVAL, "eikonal3d: NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const bool hp = flags & ADT_HOST_POINTERS;
+  const int64_t N = m * n * l;
+  const size_t bytes = sizeof(T) * N;
+  Staged st;
+  const void *du0, *df;
+  void* du;
+  if ((rc = st.in(u0, bytes, hp, &du0, s)) || (rc = st.in(f, bytes, hp, &df, s)) ||
+      (rc = st.out(u, bytes, hp, &du, s)))
+    return rc;
+  DevBuf ws;
+  const size_t off_desc = ((bytes + 255) / 256) * 256;
+  const size_t off_info = off_desc + 256;
+  if ((rc = ws.alloc(off_info + sizeof(adt_info), s))) return rc;
+  char* base = (char*)ws.p;
+  adt::EventDesc d{};
+  d.m = (int)m;
+  d.n = (int)n;
+  d.l = (int)l;
+  d.h = (double)h;
+  d.tol = (double)tol;
+  d.u0 = du0;
+  d.f = df;
+  d.u = du;
+  d.uold = base;
+  d.info = (adt_info*)(base + off_info);
+  ADT_HIP(hipMemcpyAsync(base + off_desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::fwd3_event_kernel<T><<<1, kWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3);
+  ADT_HIP(hipGetLastError());
+  adt_info hinf{};
+  ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
+  if (hp) ADT_HIP(hipMemcpyAsync(u, du, bytes, hipMemcpyDeviceToHost, s));
+  ADT_HIP(hipStreamSynchronize(s));
+  if (verbose) print_verbose(hinf);
+  if (info) *info = hinf;
+  return ADT_OK;
+}
+
+// ------------------------------------------------------------ 3D adjoint ---
+
+template <typename T>
+int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, int64_t n,
+              int64_t l, T* grad_u0, T* grad_f, adt_info* info, int flags, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!g || !u || !u0 || !f || !grad_f) return fail(ADT_EINVAL, "eikonal3d grad: NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const bool hp = flags & ADT_HOST_POINTERS;
+  const int64_t N = m * n * l;
+  const size_t bytes = sizeof(T) * N;
+  Staged st;
+  const void *dg, *du, *du0, *df;
+  void *dgu0, *dgf;
+  if ((rc = st.in(g, bytes, hp, &dg, s)) || (rc = st.in(u, bytes, hp, &du, s)) ||
+      (rc = st.in(u0, bytes, hp, &du0, s)) || (rc = st.in(f, bytes, hp, &df, s)) ||
+      (rc = st.out(grad_u0, bytes, hp, &dgu0, s)) || (rc = st.out(grad_f, bytes, hp, &dgf, s)))
+    return rc;
+  DevBuf ws;
+  const size_t a = ((bytes + 255) / 256) * 256;
+  const size_t c = ((size_t(N) + 255) / 256) * 256;
+  if ((rc = ws.alloc(a + c + 256 + sizeof(adt_info), s))) return rc;
+  char* base = (char*)ws.p;
+  adt::EventDesc d{};
+  d.m = (int)m;
+  d.n = (int)n;
+  d.l = (int)l;
+  d.h = (double)h;
+  d.u0 = du0;
+  d.f = df;
+  d.grad_u = dg;
+  d.u = (void*)du;
+  d.lam = base;
+  d.code = (uint8_t*)(base + a);
+  d.grad_u0 = dgu0;
+  d.grad_f = dgf;
+  d.info = (adt_info*)(base + a + c + 256);
+  ADT_HIP(hipMemcpyAsync(base + a + c, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::adj3_event_kernel<T><<<1, kWG, 0, s>>>((const adt::EventDesc*)(base + a + c), kMaxAdjSweeps);
+  ADT_HIP(hipGetLastError());
+  adt_info hinf{};
+  ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
+  if (hp) {
+    if (grad_u0) ADT_HIP(hipMemcpyAsync(grad_u0, dgu0, bytes, hipMemcpyDeviceToHost, s));
+    ADT_HIP(hipMemcpyAsync(grad_f, dgf, bytes, hipMemcpyDeviceToHost, s));
+  }
+  ADT_HIP(hipStreamSynchronize(s));
+  if (info) {
+    hinf.iterations = 0;
+    *info = hinf;
+  }
+  if (!hinf.adj_converged) return fail(ADT_EINVAL, "eikonal3d grad: adjoint did not reach its fixed point");
+  return ADT_OK;
+}
+
+// ------------------------------------------------------------------- 2D ---
+
+template <typename T>
+int fwd2_impl(const T* f, int64_t m, int64_t n, int64_t srcx, int64_t srcy, T h, T* u,
+              adt_info* info, int flags, void* stream) {
+  int rc = check_dims2(m, n, srcx, srcy);
+  if (rc) return rc;
+  if (!f || !u) return fail(ADT_EINVAL, "eikonal2d: NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const bool hp = flags & ADT_HOST_POINTERS;
+  const int64_t N = (m + 1) * (n + 1);
+  const size_t bytes = sizeof(T) * N;
+  Staged st;
+  const void* df;
+  void* du;
+  if ((rc = st.in(f, bytes, hp, &df, s)) || (rc = st.out(u, bytes, hp, &du, s))) return rc;
+  DevBuf ws;
+  const size_t a = ((bytes + 255) / 256) * 256;
+  if ((rc = ws.alloc(a + 256 + sizeof(adt_info), s))) return rc;
+  char* base = (char*)ws.p;
+  adt::Event2Desc d{};
+  d.m = (int)m;
+  d.n = (int)n;
+  d.ix = (int)(srcx - 1);
+  d.jx = (int)(srcy - 1);
+  d.h = (double)h;
+  d.f = df;
+  d.u = du;
+  d.uold = base;
+  d.info = (adt_info*)(base + a + 256);
+  ADT_HIP(hipMemcpyAsync(base + a, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::fwd2_event_kernel<T><<<1, kWG, 0, s>>>((const adt::Event2Desc*)(base + a), kMaxIter2);
+  ADT_HIP(hipGetLastError());
+  adt_info hinf{};
+  ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
+  if (hp) ADT_HIP(hipMemcpyAsync(u, du, bytes, hipMemcpyDeviceToHost, s));
+  ADT_HIP(hipStreamSynchronize(s));
+  if (!hinf.converged) fprintf(stderr, "ERROR: Eikonal does not converge!\n");
+  if (info) *info = hinf;
+  return ADT_OK;
+}
+
+template <typename T>
+int bwd2_impl(const T* g, const T* u, const T* f, int64_t m, int64_t n, int64_t srcx,
+              int64_t srcy, T h, T* grad_f, adt_info* info, int flags, void* stream) {
+  int rc = check_dims2(m, n, srcx, srcy);
+  if (rc) return rc;
+  if (!g || !u || !f || !grad_f) return fail(ADT_EINVAL, "eikonal2d grad: NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const bool hp = flags & ADT_HOST_POINTERS;
+  const int64_t N = (m + 1) * (n + 1);
+  const size_t bytes = sizeof(T) * N;
+  Staged st;
+  const void *dg, *du, *df;
+  void* dgf;
+  if ((rc = st.in(g, bytes, hp, &dg, s)) || (rc = st.in(u, bytes, hp, &du, s)) ||
+      (rc = st.in(f, bytes, hp, &df, s)) || (rc = st.out(grad_f, bytes, hp, &dgf, s)))
+    return rc;
+  DevBuf ws;
+  const size_t a = ((bytes + 255) / 256) * 256;
+  const size_t c = ((size_t(N) + 255) / 256) * 256;
+  if ((rc = ws.alloc(a + c + 256 + sizeof(adt_info), s))) return rc;
+  char* base = (char*)ws.p;
+  adt::Event2Desc d{};
+  d.m = (int)m;
+  d.n = (int)n;
+  d.ix = (int)(srcx - 1);
+  d.jx = (int)(srcy - 1);
+  d.h = (double)h;
+  d.f = df;
+  d.grad_u = dg;
+  d.u = (void*)du;
+  d.lam = base;
+  d.code = (uint8_t*)(base + a);
+  d.grad_f = dgf;
+  d.info = (adt_info*)(base + a + c + 256);
+  ADT_HIP(hipMemcpyAsync(base + a + c, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::adj2_event_kernel<T><<<1, kWG, 0, s>>>((const adt::Event2Desc*)(base + a + c), kMaxAdjSweeps);
+  ADT_HIP(hipGetLastError());
+  adt_info hinf{};
+  ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
+  if (hp) ADT_HIP(hipMemcpyAsync(grad_f, dgf, bytes, hipMemcpyDeviceToHost, s));
+  ADT_HIP(hipStreamSynchronize(s));
+  if (info) {
+    hinf.iterations = 0;
+    *info = hinf;
+  }
+  if (!hinf.adj_converged) return fail(ADT_EINVAL, "eikonal2d grad: adjoint did not reach its fixed point");
+  return ADT_OK;
+}
+
+// -------------------------------------------------------------- batched ---
+
+// Builds the device descriptor table and workspace for `nev` events.
+template <typename T>
+struct BatchPlan {
+  std::vector<adt::EventDesc> descs;
+  size_t ws_bytes = 0;
+  size_t off_desc = 0, off_info = 0;
+
+  int build(const adt_event3d* ev, int nev, int flags) {
+    if (nev < 0) return fail(ADT_EINVAL, "batched: nev < 0");
+    if (nev > 0 && !ev) return fail(ADT_EINVAL, "batched: NULL events");
+    descs.resize(nev);
+    size_t off = 0;
+    auto take = [&](size_t b) {
+      size_t o = off;
+      off += ((b + 255) / 256) * 256;
+      return o;
+    };
+    std::vector<size_t> o_uold(nev), o_lam(nev), o_code(nev);
+    for (int e = 0; e < nev; ++e) {
+      const adt_event3d& x = ev[e];
+      int rc = check_dims3(x.m, x.n, x.l);
+      if (rc) return fail(rc, "event " + std::to_string(e) + ": " + g_err);
+      if (!x.f || !x.u || ((flags & ADT_FORWARD) && !x.u0))
+        return fail(ADT_EINVAL, "event " + std::to_string(e) + ": NULL buffer");
+      if ((flags & ADT_ADJOINT) && (!x.grad_u || !x.grad_f || !x.u0))
+        return fail(ADT_EINVAL, "event " + std::to_string(e) + ": NULL adjoint buffer");
+      const size_t N = size_t(x.m * x.n * x.l);
+      o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
+      o_lam[e] = (flags & ADT_ADJOINT) ? take(N * sizeof(T)) : 0;
+      o_code[e] = (flags & ADT_ADJOINT) ? take(N) : 0;
+    }
+    off_desc = take(sizeof(adt::EventDesc) * (nev ? nev : 1));
+    off_info = take(sizeof(adt_info) * (nev ? nev : 1));
+    ws_bytes = off;
+    for (int e = 0; e < nev; ++e) {
+      const adt_event3d& x = ev[e];
+      adt::EventDesc& d = descs[e];
+      d = adt::EventDesc{};
+      d.m = (int)x.m;
+      d.n = (int)x.n;
+      d.l = (int)x.l;
+      d.h = x.h;
+      d.tol = x.tol;
+      d.u0 = x.u0;
+      d.f = x.f;
+      d.grad_u = x.grad_u;
+      d.u = x.u;
+      d.grad_u0 = x.grad_u0;
+      d.grad_f = x.grad_f;
+      // workspace pointers are relative offsets until bind()
+      d.uold = (void*)o_uold[e];
+      d.lam = (void*)o_lam[e];
+      d.code = (uint8_t*)o_code[e];
+      d.info = (adt_info*)(off_info + sizeof(adt_info) * e);
+    }
+    return ADT_OK;
+  }
+  void bind(char* base) {
+    for (auto& d : descs) {
+      d.uold = base + (size_t)d.uold;
+      d.lam = base + (size_t)d.lam;
+      d.code = (uint8_t*)(base + (size_t)d.code);
+      d.info = (adt_info*)(base + (size_t)d.info);
+    }
+  }
+};
+
+template <typename T>
+int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, hipStream_t s) {
+  if (nev == 0) return ADT_OK;
+  if (flags & ADT_FORWARD) {
+    adt::fwd3_event_kernel<T><<<nev, kWG, 0, s>>>(ddesc, kMaxIter3);
+    ADT_HIP(hipGetLastError());
+  }
+  if (flags & ADT_ADJOINT) {
+    adt::adj3_event_kernel<T><<<nev, kWG, 0, s>>>(ddesc, kMaxAdjSweeps);
+    ADT_HIP(hipGetLastError());
+  }
+  return ADT_OK;
+}
+
+template <typename T>
+int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, void* stream) {
+  if (!(flags & (ADT_FORWARD | ADT_ADJOINT))) return fail(ADT_EINVAL, "batched: no ADT_FORWARD/ADT_ADJOINT flag");
+  if (flags & ADT_HOST_POINTERS) return fail(ADT_EINVAL, "batched: device pointers only");
+  hipStream_t s = (hipStream_t)stream;
+  BatchPlan<T> plan;
+  int rc = plan.build(ev, nev, flags);
+  if (rc) return rc;
+  if (nev == 0) return ADT_OK;
+  DevBuf ws;
+  if ((rc = ws.alloc(plan.ws_bytes, s))) return rc;
+  plan.bind((char*)ws.p);
+  char* base = (char*)ws.p;
+  ADT_HIP(hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
+                         hipMemcpyHostToDevice, s));
+  if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, s))) return rc;
+  if (infos) {
+    ADT_HIP(hipMemcpyAsync(infos, base + plan.off_info, sizeof(adt_info) * nev,
+                           hipMemcpyDeviceToHost, s));
+    ADT_HIP(hipStreamSynchronize(s));
+  }
+  return ADT_OK;
+}
+
+// ------------------------------------------------------- plain baseline ---
+
+template <typename T>
+int fwd3_plain_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T tol, T* u,
+                    adt_info* info, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!u0 || !f || !u) return fail(ADT_EINVAL, "eikonal3d plain: NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t N = m * n * l;
+  DevBuf ws;
+  const size_t a = ((sizeof(T) * N + 255) / 256) * 256;
+  if ((rc = ws.alloc(a + 256, s))) return rc;
+  T* uold = (T*)ws.p;
+  unsigned long long* dkey = (unsigned long long*)((char*)ws.p + a);
+  const int grid = 1024;
+  adt::copy_kernel<T><<<grid, 256, 0, s>>>(u, u0, N);
+  ADT_HIP(hipGetLastError());
+  adt_info hinf{};
+  const int S = int(m + n + l - 2);
+  const int dirs[8] = {0, 1, 3, 2, 6, 4, 5, 7};
+  int it;
+  T err = 0;
+  for (it = 0; it < kMaxIter3; ++it) {
+    adt::copy_kernel<T><<<grid, 256, 0, s>>>(uold, u, N);
+    for (int sw = 0; sw < 8; ++sw)
+      for (int sp = 0; sp < S; ++sp) {
+        int lo = sp - int(n - 1) - int(l - 1);
+        lo = lo > 0 ? lo : 0;
+        int hi = sp < int(m - 1) ? sp : int(m - 1);
+        int cnt = hi - lo + 1;
+        int64_t threads = int64_t(cnt) * n;
+        adt::fwd3_plane_kernel<T><<<int((threads + 255) / 256), 256, 0, s>>>(
+            u, f, h, (int)m, (int)n, (int)l, dirs[sw], sp, lo, cnt);
+      }
+    ADT_HIP(hipGetLastError());
+    ADT_HIP(hipMemsetAsync(dkey, 0, sizeof(unsigned long long), s));
+    adt::maxdiff_kernel<T><<<grid, 256, 0, s>>>(u, uold, N, dkey);
+    unsigned long long hk = 0;
+    ADT_HIP(hipMemcpyAsync(&hk, dkey, sizeof(hk), hipMemcpyDeviceToHost, s));
+    ADT_HIP(hipStreamSynchronize(s));
+    if (sizeof(T) == 8) {
+      double e;
+      memcpy(&e, &hk, 8);
+      err = (T)e;
+    } else {
+      uint32_t k32 = (uint32_t)hk;
+      float e;
+      memcpy(&e, &k32, 4);
+      err = (T)e;
+    }
+    if (it < ADT_MAX_LOGGED_ITERS) hinf.err_log[it] = (double)err;
+    if (err < tol) {
+      ++it;
+      break;
+    }
+  }
+  hinf.iterations = it > kMaxIter3 ? kMaxIter3 : it;
+  hinf.converged = err < tol;
+  hinf.err = (double)err;
+  if (info) *info = hinf;
+  return ADT_OK;
+}
+
+template <typename T>
+int bwd3_plain_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, int64_t n,
+                    int64_t l, T* grad_u0, T* grad_f, adt_info* info, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!g || !u || !u0 || !f || !grad_f) return fail(ADT_EINVAL, "eikonal3d plain grad: NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t N = m * n * l;
+  DevBuf ws;
+  const size_t a = ((sizeof(T) * N + 255) / 256) * 256;
+  const size_t c = ((size_t(N) + 255) / 256) * 256;
+  if ((rc = ws.alloc(a + c + 256, s))) return rc;
+  T* lam = (T*)ws.p;
+  uint8_t* code = (uint8_t*)ws.p + a;
+  int* dchg = (int*)((char*)ws.p + a + c);
+  adt::adj3_prep_kernel<T><<<1024, 256, 0, s>>>(u, u0, g, (int)m, (int)n, (int)l, code, lam, grad_u0);
+  ADT_HIP(hipGetLastError());
+  const int S = int(m + n + l - 2);
+  const int dirs[8] = {0, 1, 3, 2, 6, 4, 5, 7};
+  int sweeps;
+  bool conv = false;
+  for (sweeps = 0; sweeps < kMaxAdjSweeps;) {
+    const int dir = dirs[7 - (sweeps & 7)] ^ 7;
+    ADT_HIP(hipMemsetAsync(dchg, 0, sizeof(int), s));
+    for (int sp = 0; sp < S; ++sp) {
+      int lo = sp - int(n - 1) - int(l - 1);
+      lo = lo > 0 ? lo : 0;
+      int hi = sp < int(m - 1) ? sp : int(m - 1);
+      int cnt = hi - lo + 1;
+      int64_t threads = int64_t(cnt) * n;
+      adt::adj3_plane_kernel<T><<<int((threads + 255) / 256), 256, 0, s>>>(
+          u, g, code, lam, (int)m, (int)n, (int)l, dir, sp, lo, cnt, dchg);
+    }
+    ADT_HIP(hipGetLastError());
+    int hc = 0;
+    ADT_HIP(hipMemcpyAsync(&hc, dchg, sizeof(int), hipMemcpyDeviceToHost, s));
+    ADT_HIP(hipStreamSynchronize(s));
+    ++sweeps;
+    if (!hc) {
+      conv = true;
+      break;
+    }
+  }
+  adt::adj3_finish_kernel<T><<<1024, 256, 0, s>>>(lam, f, h, N, grad_f);
+  ADT_HIP(hipGetLastError());
+  ADT_HIP(hipStreamSynchronize(s));
+  adt_info hinf{};
+  hinf.adj_sweeps = sweeps;
+  hinf.adj_converged = conv;
+  if (info) *info = hinf;
+  if (!conv) return fail(ADT_EINVAL, "eikonal3d plain grad: adjoint did not reach its fixed point");
+  return ADT_OK;
+}
+
+// ----------------------------------------------------------------- graph ---
+
+struct GraphBase {
+  virtual ~GraphBase() {}
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* ws = nullptr;
+  int nev = 0;
+  size_t off_info = 0;
+};
+
+template <typename T>
+int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out, hipStream_t s) {
+  if (!(flags & (ADT_FORWARD | ADT_ADJOINT))) return fail(ADT_EINVAL, "graph: no ADT_FORWARD/ADT_ADJOINT flag");
+  BatchPlan<T> plan;
+  int rc = plan.build(ev, nev, flags);
+  if (rc) return rc;
+  GraphBase* g = new GraphBase();
+  g->nev = nev;
+  g->off_info = plan.off_info;
+  hipError_t e = hipMalloc(&g->ws, plan.ws_bytes ? plan.ws_bytes : 1);
+  if (e != hipSuccess) {
+    delete g;
+    return fail(ADT_ENOMEM, std::string("graph: hipMalloc: ") + hipGetErrorString(e));
+  }
+  plan.bind((char*)g->ws);
+  char* base = (char*)g->ws;
+  e = hipMemcpy(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
+                hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(g->ws);
+    delete g;
+    return fail(ADT_EHIP, std::string("graph: hipMemcpy: ") + hipGetErrorString(e));
+  }
+  hipStream_t cs;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+    if (rc == ADT_OK && e2 != hipSuccess) e = e2;
+  }
+  if (e == hipSuccess && rc == ADT_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess || rc != ADT_OK) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    (void)hipFree(g->ws);
+    delete g;
+    return rc ? rc : fail(ADT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  (void)s;
+  *out = g;
+  return ADT_OK;
+}
+
+}  // namespace
+
+struct adt_graph {
+  GraphBase* g;
+};
+
+extern "C" {
+
+const char* adt_version(void) { return "adtomo_hip 0.1.0 (gfx950)"; }
+const char* adt_last_error(void) { return g_err.c_str(); }
+
+int adt_eikonal2d_fwd_f64(const double* f, int64_t m, int64_t n, int64_t srcx, int64_t srcy,
+                          double h, double* u, adt_info* info, int32_t flags, void* stream) {
+  return fwd2_impl<double>(f, m, n, srcx, srcy, h, u, info, flags, stream);
+}
+int adt_eikonal2d_fwd_f32(const float* f, int64_t m, int64_t n, int64_t srcx, int64_t srcy,
+                          float h, float* u, adt_info* info, int32_t flags, void* stream) {
+  return fwd2_impl<float>(f, m, n, srcx, srcy, h, u, info, flags, stream);
+}
+int adt_eikonal2d_bwd_f64(const double* grad_u, const double* u, const double* f, int64_t m,
+                          int64_t n, int64_t srcx, int64_t srcy, double h, double* grad_f,
+                          adt_info* info, int32_t flags, void* stream) {
+  return bwd2_impl<double>(grad_u, u, f, m, n, srcx, srcy, h, grad_f, info, flags, stream);
+}
+int adt_eikonal2d_bwd_f32(const float* grad_u, const float* u, const float* f, int64_t m,
+                          int64_t n, int64_t srcx, int64_t srcy, float h, float* grad_f,
+                          adt_info* info, int32_t flags, void* stream) {
+  return bwd2_impl<float>(grad_u, u, f, m, n, srcx, srcy, h, grad_f, info, flags, stream);
+}
+int adt_eikonal3d_fwd_f64(const double* u0, const double* f, double h, int64_t m, int64_t n,
+                          int64_t l, double tol, int32_t verbose, double* u, adt_info* info,
+                          int32_t flags, void* stream) {
+  return fwd3_impl<double>(u0, f, h, m, n, l, tol, verbose, u, info, flags, stream);
+}
+int adt_eikonal3d_fwd_f32(const float* u0, const float* f, float h, int64_t m, int64_t n,
+                          int64_t l, float tol, int32_t verbose, float* u, adt_info* info,
+                          int32_t flags, void* stream) {
+  return fwd3_impl<float>(u0, f, h, m, n, l, tol, verbose, u, info, flags, stream);
+}
+int adt_eikonal3d_bwd_f64(const double* grad_u, const double* u, const double* u0,
+                          const double* f, double h, int64_t m, int64_t n, int64_t l,
+                          double* grad_u0, double* grad_f, adt_info* info, int32_t flags,
+                          void* stream) {
+  return bwd3_impl<double>(grad_u, u, u0, f, h, m, n, l, grad_u0, grad_f, info, flags, stream);
+}
+int adt_eikonal3d_bwd_f32(const float* grad_u, const float* u, const float* u0, const float* f,
+                          float h, int64_t m, int64_t n, int64_t l, float* grad_u0,
+                          float* grad_f, adt_info* info, int32_t flags, void* stream) {
+  return bwd3_impl<float>(grad_u, u, u0, f, h, m, n, l, grad_u0, grad_f, info, flags, stream);
+}
+int adt_eikonal3d_batched_f64(const adt_event3d* events, int32_t nev, int32_t flags,
+                              adt_info* infos, void* stream) {
+  return batched_impl<double>(events, nev, flags, infos, stream);
+}
+int adt_eikonal3d_batched_f32(const adt_event3d* events, int32_t nev, int32_t flags,
+                              adt_info* infos, void* stream) {
+  return batched_impl<float>(events, nev, flags, infos, stream);
+}
+int adt_eikonal3d_fwd_plain_f64(const double* u0, const double* f, double h, int64_t m,
+                                int64_t n, int64_t l, double tol, double* u, adt_info* info,
+                                void* stream) {
+  return fwd3_plain_impl<double>(u0, f, h, m, n, l, tol, u, info, stream);
+}
+int adt_eikonal3d_bwd_plain_f64(const double* grad_u, const double* u, const double* u0,
+                                const double* f, double h, int64_t m, int64_t n, int64_t l,
+                                double* grad_u0, double* grad_f, adt_info* info, void* stream) {
+  return bwd3_plain_impl<double>(grad_u, u, u0, f, h, m, n, l, grad_u0, grad_f, info, stream);
+}
+
+int adt_graph_create(const adt_event3d* events, int32_t nev, int32_t flags, int32_t dtype,
+                     adt_graph** out, void* stream) {
+  if (!out) return fail(ADT_EINVAL, "graph: NULL out");
+  if (flags & ADT_HOST_POINTERS) return fail(ADT_EINVAL, "graph: device pointers only");
+  GraphBase* g = nullptr;
+  int rc = dtype == 0 ? graph_create_impl<double>(events, nev, flags, &g, (hipStream_t)stream)
+                      : graph_create_impl<float>(events, nev, flags, &g, (hipStream_t)stream);
+  if (rc) return rc;
+  *out = new adt_graph{g};
+  return ADT_OK;
+}
+int adt_graph_launch(adt_graph* g, void* stream) {
+  if (!g) return fail(ADT_EINVAL, "graph: NULL");
+  ADT_HIP(hipGraphLaunch(g->g->exec, (hipStream_t)stream));
+  return ADT_OK;
+}
+int adt_graph_infos(adt_graph* g, adt_info* infos, void* stream) {
+  if (!g || !infos) return fail(ADT_EINVAL, "graph: NULL");
+  hipStream_t s = (hipStream_t)stream;
+  ADT_HIP(hipMemcpyAsync(infos, (char*)g->g->ws + g->g->off_info, sizeof(adt_info) * g->g->nev,
+                         hipMemcpyDeviceToHost, s));
+  ADT_HIP(hipStreamSynchronize(s));
+  return ADT_OK;
+}
+int adt_graph_destroy(adt_graph* g) {
+  if (!g) return ADT_OK;
+  if (g->g->exec) (void)hipGraphExecDestroy(g->g->exec);
+  if (g->g->graph) (void)hipGraphDestroy(g->g->graph);
+  if (g->g->ws) (void)hipFree(g->g->ws);
+  delete g->g;
+  delete g;
+  return ADT_OK;
+}
+
+int adt_selftest_arith_f64(int32_t op, const double* a, const double* b, double* out, int64_t n,
+                           void* stream) {
+  if (n < 0 || !a || !out) return fail(ADT_EINVAL, "selftest: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  adt::arith_kernel<double><<<256, 256, 0, s>>>(op, a, b, out, n);
+  ADT_HIP(hipGetLastError());
+  ADT_HIP(hipStreamSynchronize(s));
+  return ADT_OK;
+}
+int adt_selftest_arith_f32(int32_t op, const float* a, const float* b, float* out, int64_t n,
+                           void* stream) {
+  if (n < 0 || !a || !out) return fail(ADT_EINVAL, "selftest: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  adt::arith_kernel<float><<<256, 256, 0, s>>>(op, a, b, out, n);
+  ADT_HIP(hipGetLastError());
+  ADT_HIP(hipStreamSynchronize(s));
+  return ADT_OK;
+}
+
+}  // extern "C"

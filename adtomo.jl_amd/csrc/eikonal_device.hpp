@@ -1,0 +1,80 @@
+// Device-side numerics of the Eikonal forward/adjoint (gfx950, CDNA4).
+//
+// Every expression keeps the operand order of the reference op so that, built
+// with -ffp-contract=off (no FMA contraction) and IEEE-correct sqrt/divide,
+// FP64 results are the reference CPU op's results bit for bit.
+//   std::min(x, y) in the reference is `(y < x) ? y : x`  -> smin(x, y)
+//   std::max(x, y) in the reference is `(x < y) ? y : x`  -> smax(x, y)
+// The argument order is kept at each call site: it decides which operand
+// survives a NaN.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace adt {
+
+template <typename T> struct Real;
+template <> struct Real<double> {
+  static __device__ __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }
+  static __device__ __forceinline__ double fabs_(double x) { return __builtin_fabs(x); }
+};
+template <> struct Real<float> {
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ float fabs_(float x) { return __builtin_fabsf(x); }
+};
+
+template <typename T> __device__ __forceinline__ T smin(T x, T y) { return (y < x) ? y : x; }
+template <typename T> __device__ __forceinline__ T smax(T x, T y) { return (x < y) ? y : x; }
+
+// deps/CustomOps/Eikonal/Eikonal.h:14-20 (`solution`).
+template <typename T>
+__device__ __forceinline__ T solution2d(T a, T b, T f, T h) {
+  T d = Real<T>::fabs_(a - b);
+  if (d >= f * h) return smin(a, b) + f * h;
+  return (a + b + Real<T>::sqrt_(T(2.0) * f * f * h * h - (a - b) * (a - b))) / T(2.0);
+}
+
+// deps/CustomOps/Eikonal3D/Eikonal3D.cpp:11-28 (`calculate_unique_solution`).
+template <typename T>
+__device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h) {
+  T temp;
+  if (a1 > a2) { temp = a1; a1 = a2; a2 = temp; }
+  if (a1 > a3) { temp = a1; a1 = a3; a3 = temp; }
+  if (a2 > a3) { temp = a2; a2 = a3; a3 = temp; }
+  T x = a1 + f * h;
+  if (x <= a2) return x;
+  T B = -(a1 + a2);
+  T C = (a1 * a1 + a2 * a2 - f * f * h * h) / T(2.0);
+  x = (-B + Real<T>::sqrt_(B * B - T(4.0) * C)) / T(2.0);
+  if (x <= a3) return x;
+  B = T(-2.0) * (a1 + a2 + a3) / T(3.0);
+  C = (a1 * a1 + a2 * a2 + a3 * a3 - f * f * h * h) / T(3.0);
+  x = (-B + Real<T>::sqrt_(B * B - T(4.0) * C)) / T(2.0);
+  return x;
+}
+
+// Bit pattern of a value (for exact "changed?" tests that also hold for NaN).
+__device__ __forceinline__ uint64_t bits_of(double x) { return __double_as_longlong(x); }
+__device__ __forceinline__ uint64_t bits_of(float x) { return (uint32_t)__float_as_uint(x); }
+
+// Order-preserving key of a travel-time difference for max-reductions with
+// integer atomics.  Non-negative IEEE values order like their bit patterns;
+// NaN maps to 0 so that it never wins, matching the reference's
+// `err = std::max(fabs(u - u_old), err)` which ignores NaN (Eikonal3D.cpp:79-81).
+__device__ __forceinline__ uint64_t maxkey(double x) {
+  return (x == x) ? (uint64_t)__double_as_longlong(x) : 0ull;
+}
+__device__ __forceinline__ uint64_t maxkey(float x) {
+  return (x == x) ? (uint64_t)__float_as_uint(x) : 0ull;
+}
+__device__ __forceinline__ double key_to_real(uint64_t k, double) { return __longlong_as_double((long long)k); }
+__device__ __forceinline__ float key_to_real(uint64_t k, float) { return __uint_as_float((uint32_t)k); }
+
+// The 8 sweep orderings of Eikonal3D.cpp:59-68, in order.  Packed as bits
+// (bit 0: dI<0, bit 1: dJ<0, bit 2: dK<0).
+__device__ __constant__ static const int kDir3[8] = {0, 1, 3, 2, 6, 4, 5, 7};
+// The 4 orderings of Eikonal.h:74-77: (I,J), (iI,J), (iI,iJ), (I,iJ).
+__device__ __constant__ static const int kDir2[4] = {0, 1, 3, 2};
+
+}  // namespace adt

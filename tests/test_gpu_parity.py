@@ -1,0 +1,304 @@
+"""GPU parity of the HIP path against the CPU oracle and the golden fixtures.
+
+Bar: FP64 travel times bit-exact (and the same iteration count K); FP64
+gradients within 1e-10 relative (max-norm) of the oracle; FP32 bit-exact
+against the oracle's binary32 twin, with its error against FP64 reported.
+All calls go through the C ABI (adtomo_hip -> libadtomo_hip.so).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle  # noqa: E402
+from adtomo_hip import synth  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GRAD_RTOL = 1e-10
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def dev(a, dtype=None):
+    torch = _torch()
+    a = np.asarray(a)
+    t = torch.from_numpy(np.ascontiguousarray(a if dtype is None else a.astype(dtype)))
+    return t.to("cuda")
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def bits_equal(a, b):
+    """Bit-for-bit equality; any NaN equals any NaN (payloads differ between
+    x86 and CDNA and carry no meaning for the op)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype
+    w = np.uint64 if a.dtype == np.float64 else np.uint32
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(a[~na].view(w), b[~nb].view(w))
+
+
+# ---------------------------------------------------------- arithmetic ---
+
+def test_device_arith_is_ieee():
+    import adtomo_hip as A
+    r = synth.rng(1)
+    x = np.concatenate([
+        r.uniform(0, 1e4, 200000), r.uniform(0, 1e-6, 20000), np.exp(r.uniform(-700, 700, 20000)),
+        np.array([0.0, 1.0, 2.0, 3.0, 4.0, 1e-310, 5e-324, 1e308, np.inf]),
+        np.nextafter(np.arange(1, 2000, dtype=np.float64) ** 2, 0),
+        np.nextafter(np.arange(1, 2000, dtype=np.float64) ** 2, np.inf)])
+    x = np.concatenate([x, -x[:1000]])
+    for op, ref in ((0, np.sqrt), (1, lambda v: v / 3.0), (2, lambda v: v / 2.0)):
+        with np.errstate(invalid="ignore"):
+            want = ref(x)
+        got = host(A.selftest_arith(op, dev(x)))
+        assert bits_equal(got, want), op
+    y = r.uniform(0.1, 100, x.size)
+    got = host(A.selftest_arith(3, dev(x), dev(y)))
+    assert bits_equal(got, x / y)
+    x32 = x.astype(np.float32)
+    with np.errstate(invalid="ignore", over="ignore"):
+        assert bits_equal(host(A.selftest_arith(0, dev(x32))), np.sqrt(x32))
+        assert bits_equal(host(A.selftest_arith(1, dev(x32))), x32 / np.float32(3))
+
+
+def test_device_local_solvers_match_numpy():
+    import adtomo_hip as A
+    import _hyperplane as hp
+    r = synth.rng(2)
+    n = 100000
+    a = r.uniform(0, 10, (n, 3))
+    a[: n // 4, 1] = a[: n // 4, 0]  # ties
+    a[n // 4: n // 2, 2] = a[n // 4: n // 2, 0] + r.uniform(0, 1e-3, n // 4)
+    fh = np.stack([r.uniform(0.1, 2, n), r.uniform(0.1, 3, n)], 1)
+    got = host(A.selftest_arith(4, dev(a.reshape(-1)), dev(fh.reshape(-1))))
+    want = hp.solution3d(a[:, 0], a[:, 1], a[:, 2], fh[:, 0], fh[:, 1])
+    assert bits_equal(got, want)
+    got2 = host(A.selftest_arith(5, dev(a[:, :2].reshape(-1)), dev(fh.reshape(-1))))
+    want2 = hp.solution2d(a[:, 0], a[:, 1], fh[:, 0], fh[:, 1])
+    assert bits_equal(got2, want2)
+
+
+# ------------------------------------------------------------ golden ---
+
+@pytest.mark.parametrize("name", ["gradtest3d", "small3d"])
+def test_golden_3d(name):
+    import adtomo_hip as A
+    d = dict(np.load(os.path.join(GOLD, name + ".npz")))
+    m, n, l = d["u0"].shape
+    u = A.eikonal3d(dev(d["u0"]), dev(d["f"]), float(d["h"]), m, n, l, float(d["tol"]), False)
+    assert A.ops.last_info["iterations"] == int(d["iters"])
+    assert bits_equal(host(u), d["u"])
+    gu0, gf = A.eikonal3d_grad(dev(d["grad_u"]), u, dev(d["u0"]), dev(d["f"]), float(d["h"]), m, n, l)
+    assert bits_equal(host(gu0), d["grad_u0"])
+    assert rel(host(gf), d["grad_f"]) <= GRAD_RTOL
+
+
+@pytest.mark.parametrize("name", ["gradtest2d", "config1_2d"])
+def test_golden_2d(name):
+    import adtomo_hip as A
+    d = dict(np.load(os.path.join(GOLD, name + ".npz")))
+    u = A.eikonal(dev(d["f"]), int(d["srcx"]), int(d["srcy"]), float(d["h"]))
+    assert A.ops.last_info["iterations"] == int(d["iters"])
+    assert not A.ops.last_info["ambiguous"]
+    assert bits_equal(host(u), d["u"])
+    gf = A.eikonal_grad(dev(d["grad_u"]), u, dev(d["f"]), int(d["srcx"]), int(d["srcy"]), float(d["h"]))
+    assert rel(host(gf), d["grad_f"]) <= GRAD_RTOL
+
+
+# ------------------------------------------------------- 3D vs oracle ---
+
+@pytest.mark.parametrize("shape,tol,seed", [
+    ((2, 2, 2), 1e-3, 0), ((2, 5, 3), 1e-6, 1), ((7, 6, 5), 1e-3, 2), ((17, 9, 33), 1e-6, 3),
+    ((31, 40, 12), 1e-3, 4)])
+def test_forward3d_bit_exact_small(shape, tol, seed):
+    import adtomo_hip as A
+    r = synth.rng(seed)
+    f = r.uniform(0.3, 3.0, size=shape)
+    u0 = synth.point_source_u0(shape, tuple(int(r.integers(0, s)) for s in shape))
+    u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, tol)
+    u = A.eikonal3d(dev(u0), dev(f), 0.5, *shape, tol, False)
+    assert A.ops.last_info["iterations"] == it
+    assert A.ops.last_info["err"] == err
+    assert bits_equal(host(u), u_ref)
+
+
+@pytest.mark.parametrize("tol", [1e-3, 1e-6])
+def test_config2_forward_adjoint(tol):
+    # 64x64x32, h=1, 4->8 km/s layered x (1 +/- 5%) checkerboard (period 16),
+    # seeded source, 50 top-surface receivers.
+    import adtomo_hip as A
+    shape = (64, 64, 32)
+    f = 1.0 / synth.checkerboard_model(shape)
+    src = synth.interior_source(shape, seed=22)
+    u0 = synth.point_source_u0(shape, src)
+    u_ref, it, err = oracle.eikonal3d(u0, f, 1.0, tol)
+    u = A.eikonal3d(dev(u0), dev(f), 1.0, *shape, tol, False)
+    print("config2 tol", tol, "K", it, "gpu K", A.ops.last_info["iterations"], flush=True)
+    assert A.ops.last_info["iterations"] == it
+    assert bits_equal(host(u), u_ref)
+    g = synth.sparse_adjoint_source(shape, 50, seed=23, scale=1.0)
+    gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(g, u_ref, u0, f, 1.0)
+    gu0, gf = A.eikonal3d_grad(dev(g), u, dev(u0), dev(f), 1.0, *shape)
+    assert bits_equal(host(gu0), gu0_ref)
+    e = rel(host(gf), gf_ref)
+    print("config2 grad rel err", e, "adjoint sweeps", A.ops.last_info["adj_sweeps"], flush=True)
+    assert e <= GRAD_RTOL
+
+
+def test_headline_single_event_bit_exact():
+    import adtomo_hip as A
+    f, u0s, gs, srcs = synth.headline_event_set(1)
+    u_ref, it, err = oracle.eikonal3d(u0s[0], f, 0.5, 1e-3)
+    u = A.eikonal3d(dev(u0s[0]), dev(f), 0.5, 128, 128, 64, 1e-3, False)
+    print("headline K", it, flush=True)
+    assert A.ops.last_info["iterations"] == it
+    assert bits_equal(host(u), u_ref)
+
+
+def test_plain_path_matches_oracle():
+    import adtomo_hip as A
+    shape = (24, 20, 16)
+    r = synth.rng(7)
+    f = r.uniform(0.5, 2, size=shape)
+    u0 = synth.point_source_u0(shape, (5, 11, 8))
+    u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, 1e-6)
+    u = A.eikonal3d_plain(dev(u0), dev(f), 0.5, 1e-6)
+    assert A.ops.last_info["iterations"] == it
+    assert bits_equal(host(u), u_ref)
+    g = r.normal(size=shape)
+    gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(g, u_ref, u0, f, 0.5)
+    gu0, gf = A.eikonal3d_grad_plain(dev(g), u, dev(u0), dev(f), 0.5)
+    assert bits_equal(host(gu0), gu0_ref)
+    assert rel(host(gf), gf_ref) <= GRAD_RTOL
+
+
+def test_autograd_rrule_twin():
+    import adtomo_hip as A
+    torch = _torch()
+    shape = (16, 12, 10)
+    r = synth.rng(8)
+    fnp = r.uniform(0.5, 1.5, size=shape)
+    u0np = synth.point_source_u0(shape, (8, 6, 5))
+    g = r.normal(size=shape)
+    f = dev(fnp).requires_grad_(True)
+    u0 = dev(u0np).requires_grad_(True)
+    u = A.eikonal3d(u0, f, 0.5, *shape, 1e-6, False)
+    (u * dev(g)).sum().backward()
+    u_ref, *_ = oracle.eikonal3d(u0np, fnp, 0.5, 1e-6)
+    gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(g, u_ref, u0np, fnp, 0.5)
+    assert rel(host(f.grad), gf_ref) <= GRAD_RTOL
+    assert bits_equal(host(u0.grad), gu0_ref)
+    # 2D twin
+    d = dict(np.load(os.path.join(GOLD, "gradtest2d.npz")))
+    f2 = dev(d["f"]).requires_grad_(True)
+    u2 = A.eikonal(f2, int(d["srcx"]), int(d["srcy"]), float(d["h"]))
+    (u2 ** 2).sum().backward()
+    assert rel(host(f2.grad), d["grad_f"]) <= GRAD_RTOL
+    torch.cuda.synchronize()
+
+
+def test_host_pointer_mode_numpy():
+    import adtomo_hip as A
+    d = dict(np.load(os.path.join(GOLD, "small3d.npz")))
+    m, n, l = d["u0"].shape
+    u = A.eikonal3d(d["u0"], d["f"], float(d["h"]), m, n, l, float(d["tol"]), False)
+    assert isinstance(u, np.ndarray) and bits_equal(u, d["u"])
+    gu0, gf = A.eikonal3d_grad(d["grad_u"], u, d["u0"], d["f"], float(d["h"]), m, n, l)
+    assert rel(gf, d["grad_f"]) <= GRAD_RTOL
+
+
+# --------------------------------------------------------------- 2D ---
+
+@pytest.mark.parametrize("m,n,seed", [(1, 1, 0), (12, 9, 1), (60, 30, 2), (100, 100, 3)])
+def test_2d_vs_oracle(m, n, seed):
+    import adtomo_hip as A
+    r = synth.rng(seed)
+    f = r.uniform(0.5, 2.0, size=(n + 1, m + 1))
+    srcx, srcy = int(r.integers(1, m + 2)), int(r.integers(1, n + 2))
+    u_ref, it, err, conv = oracle.eikonal2d(f, srcx, srcy, 0.3)
+    u = A.eikonal(dev(f), srcx, srcy, 0.3)
+    assert A.ops.last_info["iterations"] == it and A.ops.last_info["converged"] == conv
+    assert bits_equal(host(u), u_ref)
+    g = r.normal(size=f.shape)
+    gf_ref, _, ns = oracle.eikonal2d_grad(g, u_ref, f, srcx, srcy, 0.3)
+    gf = A.eikonal_grad(dev(g), u, dev(f), srcx, srcy, 0.3)
+    assert A.ops.last_info["n_singular"] == ns
+    assert rel(host(gf), gf_ref) <= GRAD_RTOL
+
+
+# ---------------------------------------------------------- batched ---
+
+def _uneven(nev, seed):
+    fs, u0s, gs, srcs = synth.uneven_box_set(nev, seed=seed, lo=(8, 8, 4), hi=(40, 36, 24))
+    return fs, u0s, gs
+
+
+def test_batched_uneven_fp64_bit_exact():
+    import adtomo_hip as A
+    fs, u0s, gs = _uneven(12, 5)
+    res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * 12,
+                              [1e-3] * 12, grad_us=[dev(a) for a in gs])
+    for e in range(12):
+        u_ref, it, err = oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-3)
+        assert res["info"][e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), u_ref), e
+        gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(gs[e], u_ref, u0s[e], fs[e], 0.5)
+        assert bits_equal(host(res["grad_u0"][e]), gu0_ref), e
+        assert rel(host(res["grad_f"][e]), gf_ref) <= GRAD_RTOL, e
+
+
+def test_batched_fp32_vs_oracle_f32_and_fp64_error():
+    import adtomo_hip as A
+    fs, u0s, gs = _uneven(6, 6)
+    res = A.eikonal3d_batched([dev(a, np.float32) for a in u0s], [dev(a, np.float32) for a in fs],
+                              [0.5] * 6, [1e-3] * 6, grad_us=[dev(a, np.float32) for a in gs])
+    worst_u, worst_g = 0.0, 0.0
+    for e in range(6):
+        u32, it32, _ = oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-3, dtype=np.float32)
+        assert res["info"][e]["iterations"] == it32
+        assert bits_equal(host(res["u"][e]), u32), e
+        u64, it, _ = oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-3)
+        _, gf64, _ = oracle.eikonal3d_grad(gs[e], u64, u0s[e], fs[e], 0.5)
+        worst_u = max(worst_u, rel(host(res["u"][e]).astype(np.float64), u64))
+        worst_g = max(worst_g, rel(host(res["grad_f"][e]).astype(np.float64), gf64))
+    print(f"FP32 vs FP64: max rel travel-time err {worst_u:.3e}, grad err {worst_g:.3e}", flush=True)
+    assert worst_u < 1e-5
+
+
+def test_graph_replays_match_eager():
+    import adtomo_hip as A
+    torch = _torch()
+    fs, u0s, gs = _uneven(5, 7)
+    dfs = [dev(a) for a in fs]
+    du0 = [dev(a) for a in u0s]
+    dgs = [dev(a) for a in gs]
+    eager = A.eikonal3d_batched(du0, dfs, [0.5] * 5, [1e-3] * 5, grad_us=dgs)
+    G = A.EikonalGraph(du0, dfs, [0.5] * 5, [1e-3] * 5, grad_us=dgs)
+    for rep in range(4):
+        for t in G.us + G.grad_fs:
+            t.fill_(-1.0)
+        G.launch()
+        torch.cuda.synchronize()
+        x = torch.ones(1000, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        assert float(x) == 1000.0
+        for e in range(5):
+            assert torch.equal(G.us[e], eager["u"][e]), (rep, e)
+            assert torch.equal(G.grad_fs[e], eager["grad_f"][e]), (rep, e)
+        infos = G.infos()
+        assert [i["iterations"] for i in infos] == [i["iterations"] for i in eager["info"]]
+    G.close()

@@ -6,6 +6,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -38,6 +39,9 @@ constexpr int kWG = 1024;
 struct DevBuf {
   void* p = nullptr;
   hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
     if (p) (void)hipFreeAsync(p, s);
   }
@@ -75,9 +79,10 @@ void print_verbose(const adt_info& inf) {
     printf("Iteration %d, Error = %0.6e\n", i, inf.err_log[i]);
 }
 
-// Copies host input arrays in, or aliases device pointers.
+// Copies host input arrays in, or aliases device pointers.  std::list keeps
+// the buffers in place (they free themselves on destruction).
 struct Staged {
-  std::vector<DevBuf> bufs;
+  std::list<DevBuf> bufs;
   int in(const void* host, size_t bytes, bool host_ptrs, const void** dev, hipStream_t s) {
     if (!host_ptrs) {
       *dev = host;

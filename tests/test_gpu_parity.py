@@ -276,7 +276,8 @@ def test_batched_fp32_vs_oracle_f32_and_fp64_error():
         worst_u = max(worst_u, rel(host(res["u"][e]).astype(np.float64), u64))
         worst_g = max(worst_g, rel(host(res["grad_f"][e]).astype(np.float64), gf64))
     print(f"FP32 vs FP64: max rel travel-time err {worst_u:.3e}, grad err {worst_g:.3e}", flush=True)
-    assert worst_u < 1e-5
+    # binary32 accumulates ~1e-7 per update along O(100)-node paths
+    assert worst_u < 1e-4 and worst_g < 1e-2
 
 
 def test_graph_replays_match_eager():

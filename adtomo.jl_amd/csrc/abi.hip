@@ -176,7 +176,8 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   DevBuf ws;
   const size_t a = ((bytes + 255) / 256) * 256;
   const size_t c = ((size_t(N) + 255) / 256) * 256;
-  if ((rc = ws.alloc(a + c + 256 + sizeof(adt_info), s))) return rc;
+  const size_t q = ((size_t(N) * 4 + 255) / 256) * 256;
+  if ((rc = ws.alloc(a + 2 * c + q + 256 + sizeof(adt_info), s))) return rc;
   char* base = (char*)ws.p;
   adt::EventDesc d{};
   d.m = (int)m;
@@ -189,11 +190,14 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   d.u = (void*)du;
   d.lam = base;
   d.code = (uint8_t*)(base + a);
+  d.indeg = (uint8_t*)(base + a + c);
+  d.queue = (int32_t*)(base + a + 2 * c);
   d.grad_u0 = dgu0;
   d.grad_f = dgf;
-  d.info = (adt_info*)(base + a + c + 256);
-  ADT_HIP(hipMemcpyAsync(base + a + c, &d, sizeof(d), hipMemcpyHostToDevice, s));
-  adt::adj3_event_kernel<T><<<1, kWG, 0, s>>>((const adt::EventDesc*)(base + a + c), kMaxAdjSweeps);
+  const size_t od = a + 2 * c + q;
+  d.info = (adt_info*)(base + od + 256);
+  ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::adj3_kahn_kernel<T><<<1, kWG, 0, s>>>((const adt::EventDesc*)(base + od));
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -271,7 +275,8 @@ int bwd2_impl(const T* g, const T* u, const T* f, int64_t m, int64_t n, int64_t 
   DevBuf ws;
   const size_t a = ((bytes + 255) / 256) * 256;
   const size_t c = ((size_t(N) + 255) / 256) * 256;
-  if ((rc = ws.alloc(a + c + 256 + sizeof(adt_info), s))) return rc;
+  const size_t q = ((size_t(N) * 4 + 255) / 256) * 256;
+  if ((rc = ws.alloc(a + 2 * c + q + 256 + sizeof(adt_info), s))) return rc;
   char* base = (char*)ws.p;
   adt::Event2Desc d{};
   d.m = (int)m;
@@ -284,10 +289,13 @@ int bwd2_impl(const T* g, const T* u, const T* f, int64_t m, int64_t n, int64_t 
   d.u = (void*)du;
   d.lam = base;
   d.code = (uint8_t*)(base + a);
+  d.indeg = (uint8_t*)(base + a + c);
+  d.queue = (int32_t*)(base + a + 2 * c);
   d.grad_f = dgf;
-  d.info = (adt_info*)(base + a + c + 256);
-  ADT_HIP(hipMemcpyAsync(base + a + c, &d, sizeof(d), hipMemcpyHostToDevice, s));
-  adt::adj2_event_kernel<T><<<1, kWG, 0, s>>>((const adt::Event2Desc*)(base + a + c), kMaxAdjSweeps);
+  const size_t od = a + 2 * c + q;
+  d.info = (adt_info*)(base + od + 256);
+  ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::adj2_kahn_kernel<T><<<1, kWG, 0, s>>>((const adt::Event2Desc*)(base + od));
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -320,7 +328,7 @@ struct BatchPlan {
       off += ((b + 255) / 256) * 256;
       return o;
     };
-    std::vector<size_t> o_uold(nev), o_lam(nev), o_code(nev);
+    std::vector<size_t> o_uold(nev), o_lam(nev), o_code(nev), o_indeg(nev), o_queue(nev);
     for (int e = 0; e < nev; ++e) {
       const adt_event3d& x = ev[e];
       int rc = check_dims3(x.m, x.n, x.l);
@@ -333,6 +341,8 @@ struct BatchPlan {
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
       o_lam[e] = (flags & ADT_ADJOINT) ? take(N * sizeof(T)) : 0;
       o_code[e] = (flags & ADT_ADJOINT) ? take(N) : 0;
+      o_indeg[e] = (flags & ADT_ADJOINT) ? take(N) : 0;
+      o_queue[e] = (flags & ADT_ADJOINT) ? take(4 * N) : 0;
     }
     off_desc = take(sizeof(adt::EventDesc) * (nev ? nev : 1));
     off_info = take(sizeof(adt_info) * (nev ? nev : 1));
@@ -356,6 +366,8 @@ struct BatchPlan {
       d.uold = (void*)o_uold[e];
       d.lam = (void*)o_lam[e];
       d.code = (uint8_t*)o_code[e];
+      d.indeg = (uint8_t*)o_indeg[e];
+      d.queue = (int32_t*)o_queue[e];
       d.info = (adt_info*)(off_info + sizeof(adt_info) * e);
     }
     return ADT_OK;
@@ -365,6 +377,8 @@ struct BatchPlan {
       d.uold = base + (size_t)d.uold;
       d.lam = base + (size_t)d.lam;
       d.code = (uint8_t*)(base + (size_t)d.code);
+      d.indeg = (uint8_t*)(base + (size_t)d.indeg);
+      d.queue = (int32_t*)(base + (size_t)d.queue);
       d.info = (adt_info*)(base + (size_t)d.info);
     }
   }
@@ -378,7 +392,7 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, hipStream_t s)
     ADT_HIP(hipGetLastError());
   }
   if (flags & ADT_ADJOINT) {
-    adt::adj3_event_kernel<T><<<nev, kWG, 0, s>>>(ddesc, kMaxAdjSweeps);
+    adt::adj3_kahn_kernel<T><<<nev, kWG, 0, s>>>(ddesc);
     ADT_HIP(hipGetLastError());
   }
   return ADT_OK;

@@ -21,6 +21,8 @@ struct EventDesc {
   void* uold;      // workspace: iteration snapshot [N]
   void* lam;       // workspace: adjoint multipliers [N]
   uint8_t* code;   // workspace: upwind codes [N]
+  uint8_t* indeg;  // workspace: unresolved incoming count per node [N] (word-aligned)
+  int32_t* queue;  // workspace: topological order of the adjoint [N]
   void* grad_u0;
   void* grad_f;
   adt_info* info;  // device
@@ -36,6 +38,8 @@ struct Event2Desc {
   void* uold;
   void* lam;
   uint8_t* code;
+  uint8_t* indeg;
+  int32_t* queue;
   void* grad_f;
   adt_info* info;
 };
@@ -58,14 +62,11 @@ __global__ void adj3_plane_kernel(const T* u, const T* g, const uint8_t* code, T
 template <typename T>
 __global__ void adj3_finish_kernel(const T* lam, const T* f, T h, int64_t N, T* grad_f);
 template <typename T>
-__global__ void adj3_event_kernel(const EventDesc* evs, int max_sweeps);
-template <typename T>
 __global__ void fwd2_event_kernel(const Event2Desc* evs, int max_iter);
 template <typename T>
-__global__ void adj2_prep_kernel(const T* u, int m, int n, int ix, int jx, uint8_t* code, T* lam,
-                                 unsigned long long* n_singular);
+__global__ void adj3_kahn_kernel(const EventDesc* evs);
 template <typename T>
-__global__ void adj2_event_kernel(const Event2Desc* evs, int max_sweeps);
+__global__ void adj2_kahn_kernel(const Event2Desc* evs);
 template <typename T>
 __global__ void arith_kernel(int op, const T* a, const T* b, T* out, int64_t N);
 

@@ -10,6 +10,7 @@
 // Nodes of one hyperplane are independent and are updated in parallel.
 #include "eikonal_device.hpp"
 #include "kernels.hpp"
+#include "adjoint_device.hpp"
 
 namespace adt {
 
@@ -151,48 +152,8 @@ __global__ void maxdiff_kernel(const T* __restrict__ a, const T* __restrict__ b,
 }
 
 // ------------------------------------------------------------- 3D adjoint ---
-//
-// Per-node upwind code (one byte): bit 6 = Z (u == u0, or no active
-// dimension); bits 2d..2d+1 for dimension d (x, y, z): 0 inactive,
-// 1 argmin neighbour at -1, 2 argmin neighbour at +1 (Eikonal3D.cpp:118-171).
-
-template <typename T>
-__device__ __forceinline__ void adj3_prep_node(const T* __restrict__ u, const T* __restrict__ u0,
-                                               const T* __restrict__ g, int m, int n, int l,
-                                               int64_t p, uint8_t* __restrict__ code,
-                                               T* __restrict__ lam, T* __restrict__ grad_u0) {
-  const int64_t nl = (int64_t)n * l;
-  const int i = (int)(p / nl), j = (int)((p / l) % n), k = (int)(p % l);
-  const T up = u[p];
-  lam[p] = T(0);
-  if (grad_u0) grad_u0[p] = (up == u0[p]) ? g[p] : T(0);
-  if (up == u0[p]) {
-    code[p] = 0x40;
-    return;
-  }
-  uint8_t cd = 0;
-  const int co[3] = {i, j, k};
-  const int ex[3] = {m, n, l};
-  const int64_t st[3] = {nl, l, 1};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    T umin;
-    int side;
-    if (co[d] == 0) {
-      umin = u[p + st[d]];
-      side = 2;
-    } else if (co[d] == ex[d] - 1) {
-      umin = u[p - st[d]];
-      side = 1;
-    } else {
-      T vp = u[p + st[d]], vm = u[p - st[d]];
-      umin = smin(vp, vm);
-      side = (vp > vm) ? 1 : 2;
-    }
-    if (up > umin) cd |= (uint8_t)(side << (2 * d));
-  }
-  code[p] = cd ? cd : (uint8_t)0x40;
-}
+// Gauss-Seidel sweeps of the transposed upwind system, one launch per
+// hyperplane (the plain path); codes as described in adjoint_device.hpp.
 
 template <typename T>
 __global__ void adj3_prep_kernel(const T* __restrict__ u, const T* __restrict__ u0,
@@ -278,69 +239,6 @@ __global__ void adj3_finish_kernel(const T* __restrict__ lam, const T* __restric
        p += (int64_t)gridDim.x * blockDim.x) {
     const T rhs = T(-2.0) * f[p] * h * h;
     grad_f[p] = -lam[p] * rhs;
-  }
-}
-
-// Persistent adjoint of one event per workgroup: Gauss-Seidel sweeps of the
-// transposed upwind system, cycling through the 8 directions in reverse
-// order of the forward, until one whole sweep leaves every lambda unchanged
-// (then every equation holds exactly: the triangular solve's fixed point).
-template <typename T>
-__global__ void __launch_bounds__(1024) adj3_event_kernel(const EventDesc* __restrict__ evs,
-                                                          int max_sweeps) {
-  __shared__ int changed_sh;
-  const EventDesc& ev = evs[blockIdx.x];
-  const int m = ev.m, n = ev.n, l = ev.l;
-  const int64_t N = (int64_t)m * n * l;
-  const T* u = (const T*)ev.u;
-  const T* g = (const T*)ev.grad_u;
-  const uint8_t* code = ev.code;
-  T* lam = (T*)ev.lam;
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x)
-    adj3_prep_node(u, (const T*)ev.u0, g, m, n, l, p, ev.code, lam, (T*)ev.grad_u0);
-  __syncthreads();
-  const int S = m + n + l - 2;
-  int sweeps = 0;
-  bool conv = false;
-  for (sweeps = 0; sweeps < max_sweeps;) {
-    const int dir = kDir3[7 - (sweeps & 7)] ^ 7;
-    if (threadIdx.x == 0) changed_sh = 0;
-    __syncthreads();
-    bool ch = false;
-    for (int s = 0; s < S; ++s) {
-      int a_lo, a_hi;
-      plane_a_range(s, m, n, l, &a_lo, &a_hi);
-      const int cnt = (a_hi - a_lo + 1) * n;
-      for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-        int a = a_lo + t / n, b = t % n, c = s - a - b;
-        if (c < 0 || c >= l) continue;
-        int i = (dir & 1) ? m - 1 - a : a;
-        int j = (dir & 2) ? n - 1 - b : b;
-        int k = (dir & 4) ? l - 1 - c : c;
-        ch |= adj3_node(u, g, code, lam, m, n, l, i, j, k);
-      }
-      __syncthreads();
-    }
-    if (ch) changed_sh = 1;
-    __syncthreads();
-    ++sweeps;
-    const int any = changed_sh;
-    __syncthreads();
-    if (!any) {
-      conv = true;
-      break;
-    }
-  }
-  const T h = (T)ev.h;
-  const T* f = (const T*)ev.f;
-  T* gf = (T*)ev.grad_f;
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x) {
-    const T rhs = T(-2.0) * f[p] * h * h;
-    gf[p] = -lam[p] * rhs;
-  }
-  if (threadIdx.x == 0) {
-    ev.info->adj_sweeps = sweeps;
-    ev.info->adj_converged = conv ? 1 : 0;
   }
 }
 
@@ -455,167 +353,6 @@ __global__ void __launch_bounds__(1024) fwd2_event_kernel(const Event2Desc* __re
 
 // 2D upwind codes: bits 0-1 x side, bits 2-3 y side (1: -1, 2: +1), bit 6
 // singular (non-source node without an active dimension), bit 7 source.
-template <typename T>
-__device__ __forceinline__ void adj2_prep_node(const T* __restrict__ u, int m, int n, int ix,
-                                               int jx, int64_t p, uint8_t* __restrict__ code,
-                                               T* __restrict__ lam,
-                                               unsigned long long* n_singular) {
-  const int64_t W = (int64_t)m + 1;
-  const int i = (int)(p % W), j = (int)(p / W);
-  lam[p] = T(0);
-  if (i == ix && j == jx) {
-    code[p] = 0x80;
-    return;
-  }
-  uint8_t cd = 0;
-  const T up = u[p];
-  T a;
-  int sa;
-  if (i == 0) {
-    a = u[j * W + 1];
-    sa = 2;
-  } else if (i == m) {
-    a = u[j * W + m - 1];
-    sa = 1;
-  } else {
-    const bool left = u[j * W + i + 1] > u[j * W + i - 1];
-    a = left ? u[j * W + i - 1] : u[j * W + i + 1];
-    sa = left ? 1 : 2;
-  }
-  if (up > a) cd |= (uint8_t)sa;
-  T b;
-  int sb;
-  if (j == 0) {
-    b = u[W + i];
-    sb = 2;
-  } else if (j == n) {
-    b = u[(int64_t)(n - 1) * W + i];
-    sb = 1;
-  } else {
-    const bool upn = u[(int64_t)(j + 1) * W + i] > u[(int64_t)(j - 1) * W + i];
-    b = upn ? u[(int64_t)(j - 1) * W + i] : u[(int64_t)(j + 1) * W + i];
-    sb = upn ? 1 : 2;
-  }
-  if (up > b) cd |= (uint8_t)(sb << 2);
-  if (!cd) {
-    cd = 0x40;
-    atomicAdd(n_singular, 1ull);
-  }
-  code[p] = cd;
-}
-
-template <typename T>
-__global__ void adj2_prep_kernel(const T* __restrict__ u, int m, int n, int ix, int jx,
-                                 uint8_t* __restrict__ code, T* __restrict__ lam,
-                                 unsigned long long* n_singular) {
-  const int64_t N = ((int64_t)m + 1) * (n + 1);
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
-       p += (int64_t)gridDim.x * blockDim.x)
-    adj2_prep_node(u, m, n, ix, jx, p, code, lam, n_singular);
-}
-
-template <typename T>
-__device__ __forceinline__ bool adj2_node(const T* __restrict__ u, const T* __restrict__ g,
-                                          const uint8_t* __restrict__ code, T* __restrict__ lam,
-                                          int m, int n, int i, int j) {
-  const int64_t W = (int64_t)m + 1;
-  const int64_t p = (int64_t)j * W + i;
-  const uint8_t cr = code[p];
-  if (cr & 0x40) return false;
-  const T ur = u[p];
-  T num = g[p];
-  const int co[2] = {i, j};
-  const int ex[2] = {m, n};
-  const int64_t st[2] = {1, W};
-  T D = T(0);
-  bool have = false;
-#pragma unroll
-  for (int d = 0; d < 2; ++d) {
-    if (co[d] > 0) {
-      const int64_t c = p - st[d];
-      const uint8_t cc = code[c];
-      if (!(cc & 0xC0) && ((cc >> (2 * d)) & 3) == 2) num = num + (T(2.0) * (u[c] - ur)) * lam[c];
-    }
-    if (co[d] < ex[d]) {
-      const int64_t c = p + st[d];
-      const uint8_t cc = code[c];
-      if (!(cc & 0xC0) && ((cc >> (2 * d)) & 3) == 1) num = num + (T(2.0) * (u[c] - ur)) * lam[c];
-    }
-    const int sd = (cr >> (2 * d)) & 3;
-    if (sd) {
-      const T t = T(2.0) * (ur - u[sd == 1 ? p - st[d] : p + st[d]]);
-      D = have ? D + t : t;
-      have = true;
-    }
-  }
-  if (cr & 0x80) D = T(1.0);
-  const T v = num / D;
-  if (bits_of(v) != bits_of(lam[p])) {
-    lam[p] = v;
-    return true;
-  }
-  return false;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(1024) adj2_event_kernel(const Event2Desc* __restrict__ evs,
-                                                          int max_sweeps) {
-  __shared__ int changed_sh;
-  const Event2Desc& ev = evs[blockIdx.x];
-  const int m = ev.m, n = ev.n;
-  const int64_t W = (int64_t)m + 1, N = W * (n + 1);
-  const T* u = (const T*)ev.u;
-  const T* g = (const T*)ev.grad_u;
-  const uint8_t* code = ev.code;
-  T* lam = (T*)ev.lam;
-  if (threadIdx.x == 0) ev.info->n_singular = 0;
-  __syncthreads();
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x)
-    adj2_prep_node(u, m, n, ev.ix, ev.jx, p, ev.code, lam,
-                   (unsigned long long*)&ev.info->n_singular);
-  __syncthreads();
-  int sweeps;
-  bool conv = false;
-  for (sweeps = 0; sweeps < max_sweeps;) {
-    const int dir = kDir2[3 - (sweeps & 3)] ^ 3;
-    if (threadIdx.x == 0) changed_sh = 0;
-    __syncthreads();
-    bool ch = false;
-    for (int s = 0; s <= m + n; ++s) {
-      const int a_lo = s - n > 0 ? s - n : 0, a_hi = s < m ? s : m;
-      for (int a = a_lo + threadIdx.x; a <= a_hi; a += blockDim.x) {
-        const int b = s - a;
-        const int i = (dir & 1) ? m - a : a;
-        const int j = (dir & 2) ? n - b : b;
-        ch |= adj2_node(u, g, code, lam, m, n, i, j);
-      }
-      __syncthreads();
-    }
-    if (ch) changed_sh = 1;
-    __syncthreads();
-    ++sweeps;
-    const int any = changed_sh;
-    __syncthreads();
-    if (!any) {
-      conv = true;
-      break;
-    }
-  }
-  const T h = (T)ev.h;
-  const T* f = (const T*)ev.f;
-  T* gf = (T*)ev.grad_f;
-  const int64_t src = (int64_t)ev.jx * W + ev.ix;
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x) {
-    T dFdf = T(-2.0) * f[p] * h * h;
-    if (p == src) dFdf = T(0.0);
-    gf[p] = -lam[p] * dFdf;
-  }
-  if (threadIdx.x == 0) {
-    ev.info->adj_sweeps = sweeps;
-    ev.info->adj_converged = conv ? 1 : 0;
-  }
-}
-
 // ------------------------------------------------------------ self-test ---
 // Device arithmetic used on the path, for checks against host IEEE results.
 template <typename T>
@@ -649,11 +386,7 @@ __global__ void arith_kernel(int op, const T* __restrict__ a, const T* __restric
   template __global__ void adj3_plane_kernel<T>(const T*, const T*, const uint8_t*, T*, int,    \
                                                 int, int, int, int, int, int, int*);            \
   template __global__ void adj3_finish_kernel<T>(const T*, const T*, T, int64_t, T*);           \
-  template __global__ void adj3_event_kernel<T>(const EventDesc*, int);                         \
   template __global__ void fwd2_event_kernel<T>(const Event2Desc*, int);                        \
-  template __global__ void adj2_prep_kernel<T>(const T*, int, int, int, int, uint8_t*, T*,      \
-                                               unsigned long long*);                            \
-  template __global__ void adj2_event_kernel<T>(const Event2Desc*, int);                        \
   template __global__ void arith_kernel<T>(int, const T*, const T*, T*, int64_t);
 
 ADT_INSTANTIATE(double)

@@ -77,7 +77,7 @@ def main():
     Ks = np.array([i["iterations"] for i in infos])
     sweeps = np.array([i["adj_sweeps"] for i in infos])
     assert all(i["converged"] for i in infos) and all(i["adj_converged"] for i in infos)
-    log(f"K: mean {Ks.mean():.2f} min {Ks.min()} max {Ks.max()}; adjoint sweeps mean {sweeps.mean():.1f}")
+    log(f"K: mean {Ks.mean():.2f} min {Ks.min()} max {Ks.max()}; adjoint levels mean {sweeps.mean():.1f}")
 
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

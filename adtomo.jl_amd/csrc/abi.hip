@@ -34,6 +34,7 @@ constexpr int kMaxIter3 = 20;         // Eikonal3D.cpp:74
 constexpr int kMaxIter2 = 100;        // Eikonal.h:73
 constexpr int kMaxAdjSweeps = 8 * 64;  // cap of the adjoint fixed-point sweeps
 constexpr int kWG = 1024;
+constexpr int kStreamWG = 512;  // fwd3_stream_kernel: 8 waves
 
 // Stream-ordered device allocation that is released when the scope ends.
 struct DevBuf {
@@ -143,7 +144,7 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   d.uold = base;
   d.info = (adt_info*)(base + off_info);
   ADT_HIP(hipMemcpyAsync(base + off_desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
-  adt::fwd3_event_kernel<T><<<1, kWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3);
+  adt::fwd3_stream_kernel<T><<<1, kStreamWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -388,7 +389,7 @@ template <typename T>
 int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, hipStream_t s) {
   if (nev == 0) return ADT_OK;
   if (flags & ADT_FORWARD) {
-    adt::fwd3_event_kernel<T><<<nev, kWG, 0, s>>>(ddesc, kMaxIter3);
+    adt::fwd3_stream_kernel<T><<<nev, kStreamWG, 0, s>>>(ddesc, kMaxIter3);
     ADT_HIP(hipGetLastError());
   }
   if (flags & ADT_ADJOINT) {

@@ -50,6 +50,8 @@ __global__ void fwd3_plane_kernel(T* u, const T* f, T h, int m, int n, int l, in
 template <typename T>
 __global__ void fwd3_event_kernel(const EventDesc* evs, int max_iter);
 template <typename T>
+__global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter);
+template <typename T>
 __global__ void copy_kernel(T* dst, const T* src, int64_t N);
 template <typename T>
 __global__ void maxdiff_kernel(const T* a, const T* b, int64_t N, unsigned long long* out);

@@ -1,0 +1,405 @@
+// 3D forward: k-streaming fast sweeping, one workgroup per event.
+//
+// Schedule.  A sweep (Eikonal3D.cpp:35-57) is equivalent to processing the
+// hyperplanes a+b+c = s in increasing s, where (a, b, c) are the indices
+// counted along the sweep direction (c along k, the contiguous axis).  Each
+// lane owns a column (a, b) and streams it along c, one node per step,
+// keeping the column in registers (16-node windows loaded as whole cache
+// lines).  The columns of one wave lie on one "virtual anti-diagonal" v, so
+// all lanes of a wave are at the same c at every step; wave v runs one step
+// behind wave v-1, which is the hyperplane order.  Neighbour values cross
+// waves through LDS once per step (one workgroup barrier per step).
+//
+// Column mapping.  Let b run along the longer of the two non-k axes (N_b
+// nodes) and a along the other (M_a nodes).  Lanes p < H (H <= 64) of a
+// wave cover rows a = H*beta + p of band beta; lane p of virtual diagonal v
+// owns column b = (v - p) mod N_b of band beta = (v - p) div N_b.  Its
+// upwind neighbours (a-1, b) and (a, b-1) then sit on diagonal v-1 (lane p-1
+// and p) or, across a band edge, on diagonal v - (N_b - H + 1); its downwind
+// neighbours on v+1 or v + (N_b - H + 1).  Waves 0..W-1 of the workgroup take
+// diagonals v = g*W .. g*W+W-1 for g = 0, 1, ... (a new group starts when a
+// wave finishes its column), so within a group neighbours are one step apart
+// (LDS), and across groups or bands they are at least Lp - W + 1 steps apart
+// and go through global memory, where they have been stored already.
+//
+// Iteration (Eikonal3D.cpp:71-88): u = u0 is fused into the first sweep
+// (it reads u0 and writes u); the snapshot u_old is written during the first
+// sweep of each later iteration; err = max|u - u_old| is a coalesced pass
+// after the eighth sweep against the snapshot (u0 for the first iteration).
+#include "eikonal_device.hpp"
+#include "kernels.hpp"
+
+namespace adt {
+
+namespace {
+
+// Explicit address spaces: generic (flat) accesses would make every LDS
+// wait also wait for all outstanding global loads.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+using lptr = __attribute__((address_space(3))) T*;
+
+constexpr int kW = 8;       // waves per workgroup
+constexpr int kC = 16;      // window length (nodes along k)
+constexpr int kPF = 2;      // prefetch distance (steps) of the global neighbour reads
+
+struct Geom {
+  int Ma, Nb, L, H, Lp, G, delta;
+  int64_t sA, sB;  // memory strides of the a and b axes
+  bool a_is_i;
+};
+
+__device__ inline Geom make_geom(int m, int n, int l) {
+  Geom g;
+  g.a_is_i = (m <= n);
+  g.Ma = g.a_is_i ? m : n;
+  g.Nb = g.a_is_i ? n : m;
+  g.sA = g.a_is_i ? (int64_t)n * l : (int64_t)l;
+  g.sB = g.a_is_i ? (int64_t)l : (int64_t)n * l;
+  g.L = l;
+  if (g.Nb <= 64) {
+    g.H = g.Nb;  // band edge neighbours on the adjacent diagonal
+  } else if (g.Nb >= 64 + kW - 1) {
+    g.H = 64;
+  } else {
+    g.H = g.Nb - kW + 1;  // keep band-edge neighbours >= W diagonals apart
+  }
+  g.delta = g.Nb - g.H + 1;
+  int Lp = ((l + kC - 1) / kC) * kC;
+  const int minLp = ((kW + kC + kPF + 2 + kC - 1) / kC) * kC;  // stores visible across groups
+  g.Lp = Lp > minLp ? Lp : minLp;
+  const int bands = (g.Ma + g.H - 1) / g.H;
+  const int V = (g.H - 1) + (g.Nb - 1) + (bands - 1) * g.Nb + 1;  // virtual diagonals
+  g.G = (V + kW - 1) / kW;
+  return g;
+}
+
+// Column of lane p on virtual diagonal v, with the canonical-origin address.
+struct Col {
+  int a, b;
+  bool valid;
+  int64_t addr;  // address of canonical (a, b, c = 0)
+};
+
+__device__ __forceinline__ Col col_of(const Geom& g, int v, int p, int64_t base0, int64_t tA,
+                                      int64_t tB) {
+  Col c;
+  const int q = v - p;
+  c.valid = (p < g.H) && (q >= 0);
+  const int qq = q > 0 ? q : 0;
+  const int beta = qq / g.Nb;
+  c.b = qq - beta * g.Nb;
+  c.a = beta * g.H + p;
+  c.valid = c.valid && (c.a < g.Ma);
+  c.addr = base0 + (int64_t)c.a * tA + (int64_t)c.b * tB;
+  return c;
+}
+
+template <typename T>
+struct Sweep {
+  // geometry and direction
+  Geom g;
+  int64_t tA, tB, tC, base0;
+  bool flipA, flipB, flipC;
+  T h;
+  gptr<const T> src;           // old values (u0 in the very first sweep)
+  gptr<T> dst;                 // u
+  gptr<const T> f;
+  gptr<T> snap_w;              // snapshot written at chunk start (first sweep of it >= 1)
+  gptr<const T> snap_r;        // snapshot the iteration's error is measured against
+};
+
+// Combines the canonical upwind (new) / downwind (old) values of one axis
+// into the reference's neighbour minimum along the real axis.
+template <typename T>
+__device__ __forceinline__ T axis_min(T up, T dn, bool flip, int a, int ext) {
+  const T plus = flip ? up : dn;
+  const T minus = flip ? dn : up;
+  const int r = flip ? ext - 1 - a : a;
+  return r == 0 ? plus : (r == ext - 1 ? minus : smin(plus, minus));
+}
+
+template <typename T>
+__device__ __forceinline__ void load_chunk(T (&buf)[kC], gptr<const T> base, const Col& col,
+                                           int c0, int L, int64_t tC) {
+#pragma unroll
+  for (int e = 0; e < kC; ++e)
+    if (col.valid && c0 + e < L) buf[e] = base[col.addr + (int64_t)(c0 + e) * tC];
+}
+
+template <typename T>
+__device__ __forceinline__ void load_half(T (&buf)[kC / 2], gptr<const T> base,
+                                          const Col& col, int c0, int L, int64_t tC) {
+#pragma unroll
+  for (int e = 0; e < kC / 2; ++e)
+    if (col.valid && c0 + e < L) buf[e] = base[col.addr + (int64_t)(c0 + e) * tC];
+}
+
+template <typename T>
+__device__ __forceinline__ void store_chunk(const T (&buf)[kC], gptr<T> base, const Col& col,
+                                            int c0, int L, int64_t tC) {
+#pragma unroll
+  for (int e = 0; e < kC; ++e)
+    if (col.valid && c0 + e < L) base[col.addr + (int64_t)(c0 + e) * tC] = buf[e];
+}
+
+}  // namespace
+
+// Per-wave state of one sweep.
+template <typename T>
+struct Lane {
+  int w, p;
+  bool up_glob_a, up_glob_b, dn_glob_a, dn_glob_b;
+  int sp, dp;              // LDS lanes of the upwind-a / downwind-a neighbours
+  int wu, wd;              // LDS waves of the upwind / downwind diagonal (clamped)
+  int step;                // global step index (parity of the LDS buffers)
+  Col pcol;                // column of the stream position being prefetched
+  int pgroup;
+  T pUA[kPF], pUB[kPF], pDA[kPF], pDB[kPF];
+};
+
+// Loads, for stream position s, the neighbour values this lane takes from
+// global memory (upwind = new values in dst, downwind = old values in src).
+template <typename T>
+__device__ __forceinline__ void glob_fetch(const Sweep<T>& S, Lane<T>& ln, int s, int slot) {
+  const Geom& g = S.g;
+  const int gs = s / g.Lp;
+  const int c = s - gs * g.Lp;
+  if (gs != ln.pgroup) {
+    ln.pgroup = gs;
+    ln.pcol = col_of(g, gs * kW + ln.w, ln.p, S.base0, S.tA, S.tB);
+  }
+  if (gs >= g.G || !ln.pcol.valid || c >= g.L) return;
+  const int64_t node = ln.pcol.addr + (int64_t)c * S.tC;
+  if (ln.up_glob_a && ln.pcol.a > 0) ln.pUA[slot] = S.dst[node - S.tA];
+  if (ln.up_glob_b && ln.pcol.b > 0) ln.pUB[slot] = S.dst[node - S.tB];
+  if (ln.dn_glob_a && ln.pcol.a + 1 < g.Ma) ln.pDA[slot] = S.src[node + S.tA];
+  if (ln.dn_glob_b && ln.pcol.b + 1 < g.Nb) ln.pDB[slot] = S.src[node + S.tB];
+}
+
+// Processes one 16-node chunk of the lane's stream held in A (old values in,
+// new values out); B holds the next chunk (old values).  The slowness comes
+// in two 8-node halves: F0 (elements 0-7, loaded beforehand) and F1
+// (elements 8-15, loaded at element 0); at element 8, F0 is refilled with
+// the first half of the next chunk (column ncol, offset nc0, if nvalid).
+template <typename T>
+__device__ __forceinline__ void process_chunk(const Sweep<T>& S, Lane<T>& ln, const Col& col,
+                                              int s0, int c0, T (&A)[kC], const T (&B)[kC],
+                                              T (&F0)[kC / 2], T (&F1)[kC / 2], const Col& ncol,
+                                              int nc0, bool nvalid, T& prev_new,
+                                              lptr<T> lds_new, lptr<T> lds_old) {
+  const Geom& g = S.g;
+  const int w = ln.w, p = ln.p;
+#pragma unroll
+  for (int e = 0; e < kC; ++e) {
+    const int c = c0 + e;
+    const int slot = e % kPF;
+    if (e == 0) load_half(F1, S.f, col, c0 + kC / 2, g.L, S.tC);
+    if (e == kC / 2 && nvalid) load_half(F0, S.f, ncol, nc0, g.L, S.tC);
+    const T gUA = ln.pUA[slot], gUB = ln.pUB[slot], gDA = ln.pDA[slot], gDB = ln.pDB[slot];
+    glob_fetch(S, ln, s0 + e + kPF, slot);
+    const int rp = (ln.step & 1) ^ 1;  // buffers written in the previous step
+    const int wp = ln.step & 1;
+    const T own = A[e];
+    const T od_c = (e + 1 < kC) ? A[(e + 1) % kC] : B[0];
+    const T od_c2 = (e + 2 < kC) ? A[(e + 2) % kC] : B[(e + 2) % kC];
+    const T ua = ln.up_glob_a ? gUA : lds_new[(rp * kW + ln.wu) * 64 + ln.sp];
+    const T ub = ln.up_glob_b ? gUB : lds_new[(rp * kW + ln.wu) * 64 + p];
+    const T da = ln.dn_glob_a ? gDA : lds_old[(rp * kW + ln.wd) * 64 + ln.dp];
+    const T db = ln.dn_glob_b ? gDB : lds_old[(rp * kW + ln.wd) * 64 + p];
+    T un = own;
+    if (col.valid && c < g.L) {
+      const T ma = axis_min(ua, da, S.flipA, col.a, g.Ma);
+      const T mb = axis_min(ub, db, S.flipB, col.b, g.Nb);
+      const T mc = axis_min(prev_new, od_c, S.flipC, c, g.L);
+      const T ux = g.a_is_i ? ma : mb;
+      const T uy = g.a_is_i ? mb : ma;
+      const T fe = e < kC / 2 ? F0[e % (kC / 2)] : F1[e % (kC / 2)];
+      const T cand = solution3d(ux, uy, mc, fe, S.h);
+      un = smin(cand, own);
+    }
+    A[e] = un;
+    prev_new = un;
+    lds_new[(wp * kW + w) * 64 + p] = un;
+    lds_old[(wp * kW + w) * 64 + p] = od_c2;
+    ++ln.step;
+    __syncthreads();
+  }
+}
+
+// One sweep of one event by the whole workgroup.
+template <typename T, bool SNAP>
+__device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_old) {
+  // Local copy: keeps the fields in registers instead of re-reading them
+  // through a generic pointer after every store.
+  const Sweep<T> S = S_in;
+  const Geom& g = S.g;
+  Lane<T> ln;
+  ln.w = threadIdx.x >> 6;
+  ln.p = threadIdx.x & 63;
+  const int w = ln.w, p = ln.p;
+  ln.up_glob_b = (w == 0);
+  ln.dn_glob_b = (w == kW - 1);
+  ln.up_glob_a = (w == 0) || (p == 0 && g.delta > 1);
+  ln.dn_glob_a = (w == kW - 1) || (p == g.H - 1 && g.delta > 1);
+  ln.sp = (p == 0) ? g.H - 1 : p - 1;
+  ln.dp = (p == g.H - 1) ? 0 : p + 1;
+  ln.wu = w > 0 ? w - 1 : 0;
+  ln.wd = w < kW - 1 ? w + 1 : kW - 1;
+  ln.pgroup = -1;
+#pragma unroll
+  for (int k = 0; k < kPF; ++k) ln.pUA[k] = ln.pUB[k] = ln.pDA[k] = ln.pDB[k] = T(0);
+  const int L = g.L, Lp = g.Lp;
+  const int chunks = g.G * (Lp / kC);
+
+  T X[kC], Y[kC], F0[kC / 2], F1[kC / 2];
+  Col cur = col_of(g, w, p, S.base0, S.tA, S.tB);        // group 0
+  Col nxt = col_of(g, kW + w, p, S.base0, S.tA, S.tB);   // group 1
+  // chunks 0 and 1 of the stream
+  load_chunk(X, S.src, cur, 0, L, S.tC);
+  load_half(F0, S.f, cur, 0, L, S.tC);
+  {
+    const bool same = Lp > kC;
+    load_chunk(Y, S.src, same ? cur : nxt, same ? kC : 0, L, S.tC);
+  }
+#pragma unroll
+  for (int k = 0; k < kPF; ++k) glob_fetch(S, ln, k, k);
+
+  // Wave w processes stream position s at global step T = w + 1 + s and, at
+  // step T, publishes old(s + 2) for the upwind wave's next step.  Before its
+  // first step it publishes old(0) (step w - 1) and old(1) (step w).
+  ln.step = 0;
+  for (; ln.step < w - 1; ++ln.step) __syncthreads();
+  if (w >= 1) {
+    lds_old[((ln.step & 1) * kW + w) * 64 + p] = X[0];
+    ++ln.step;
+    __syncthreads();
+  }
+  lds_old[((ln.step & 1) * kW + w) * 64 + p] = X[1];
+  ++ln.step;
+  __syncthreads();
+
+  T prev_new = T(0);
+  int gcur = 0;
+  // One chunk: A holds it (old values), B the finished previous chunk (new
+  // values, stored here) and then the following chunk (old values).
+  auto do_chunk = [&](int jj, T(&A)[kC], T(&B)[kC]) {
+    const int s0 = jj * kC;
+    const int gs = s0 / Lp;
+    const int c0 = s0 - gs * Lp;
+    const Col prev = cur;
+    if (gs != gcur) {  // this chunk starts a new column
+      gcur = gs;
+      cur = nxt;
+      nxt = col_of(g, (gs + 1) * kW + w, p, S.base0, S.tA, S.tB);
+    }
+    const int sn = s0 + kC;
+    const int gn = sn / Lp;
+    const bool nvalid = gn < g.G;
+    const Col& cn = (gn == gs) ? cur : nxt;
+    const int cn0 = sn - gn * Lp;
+    if (jj > 0) {
+      const int sp0 = s0 - kC;
+      const int gp = sp0 / Lp;
+      store_chunk(B, S.dst, gp == gs ? cur : prev, sp0 - gp * Lp, L, S.tC);
+      if (nvalid) load_chunk(B, S.src, cn, cn0, L, S.tC);
+    }
+    if (SNAP) store_chunk(A, S.snap_w, cur, c0, L, S.tC);
+    process_chunk<T>(S, ln, cur, s0, c0, A, B, F0, F1, cn, cn0, nvalid, prev_new, lds_new,
+                     lds_old);
+  };
+  for (int j = 0; j < chunks; j += 2) {
+    do_chunk(j, X, Y);
+    if (j + 1 < chunks) do_chunk(j + 1, Y, X);
+  }
+  {  // store the last chunk
+    const int jl = chunks - 1;
+    const int sl = jl * kC;
+    const int gl = sl / Lp;
+    if (jl & 1)
+      store_chunk(Y, S.dst, cur, sl - gl * Lp, L, S.tC);
+    else
+      store_chunk(X, S.dst, cur, sl - gl * Lp, L, S.tC);
+  }
+  // trailing idle steps (every wave passes G*Lp + W barriers), then one more
+  // so that the final stores are visible to the next sweep
+  for (int t = w; t < kW - 1; ++t) __syncthreads();
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* __restrict__ evs,
+                                                              int max_iter) {
+  __shared__ T lds_new[2 * kW * 64];
+  __shared__ T lds_old[2 * kW * 64];
+  __shared__ uint64_t red[kW];
+  const EventDesc& ev = evs[blockIdx.x];
+  const int m = ev.m, n = ev.n, l = ev.l;
+  Sweep<T> S;
+  S.g = make_geom(m, n, l);
+  S.h = (T)ev.h;
+  S.f = (gptr<const T>)ev.f;
+  S.dst = (gptr<T>)ev.u;
+  const T tol = (T)ev.tol;
+  int it = 0;
+  T err = T(0);
+  for (it = 0; it < max_iter; ++it) {
+    uint64_t key = 0;
+    for (int sw = 0; sw < 8; ++sw) {
+      const int dir = kDir3[sw];
+      const bool fi = dir & 1, fj = dir & 2, fk = dir & 4;
+      S.flipA = S.g.a_is_i ? fi : fj;
+      S.flipB = S.g.a_is_i ? fj : fi;
+      S.flipC = fk;
+      S.tA = S.flipA ? -S.g.sA : S.g.sA;
+      S.tB = S.flipB ? -S.g.sB : S.g.sB;
+      S.tC = S.flipC ? -1 : 1;
+      S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
+                (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
+      S.src = (gptr<const T>)((it == 0 && sw == 0) ? ev.u0 : ev.u);
+      S.snap_w = (gptr<T>)ev.uold;
+      S.snap_r = (gptr<const T>)((it == 0) ? ev.u0 : ev.uold);
+      if (sw == 0 && it > 0)
+        stream_sweep<T, true>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
+      else
+        stream_sweep<T, false>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
+    }
+    // err = max |u - u_old| over the event (coalesced pass after the 8 sweeps)
+    {
+      gptr<const T> ua = (gptr<const T>)ev.u;
+      gptr<const T> sr = S.snap_r;
+      const int64_t N = (int64_t)m * n * l;
+      for (int64_t q = threadIdx.x; q < N; q += blockDim.x) {
+        const uint64_t kq = maxkey(Real<T>::fabs_(ua[q] - sr[q]));
+        key = kq > key ? kq : key;
+      }
+    }
+    // workgroup max of err
+    for (int off = 32; off > 0; off >>= 1) {
+      uint64_t o = __shfl_xor(key, off);
+      key = o > key ? o : key;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
+    __syncthreads();
+    uint64_t r = 0;
+    for (int q = 0; q < kW; ++q) r = red[q] > r ? red[q] : r;
+    __syncthreads();
+    err = key_to_real(r, T(0));
+    if (threadIdx.x == 0 && it < kMaxLoggedIters) ev.info->err_log[it] = (double)err;
+    if (err < tol) {
+      ++it;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) {
+    ev.info->iterations = it > max_iter ? max_iter : it;
+    ev.info->converged = (err < tol) ? 1 : 0;
+    ev.info->err = (double)err;
+  }
+}
+
+template __global__ void fwd3_stream_kernel<double>(const EventDesc*, int);
+template __global__ void fwd3_stream_kernel<float>(const EventDesc*, int);
+
+}  // namespace adt

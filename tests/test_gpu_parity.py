@@ -136,6 +136,23 @@ def test_forward3d_bit_exact_small(shape, tol, seed):
     assert bits_equal(host(u), u_ref)
 
 
+@pytest.mark.parametrize("shape", [
+    (64, 64, 16), (48, 65, 23), (70, 40, 9), (20, 71, 33), (100, 72, 5), (130, 66, 17),
+    (3, 129, 40), (129, 3, 2)])
+def test_forward3d_column_mappings(shape):
+    # exercises the band mapping of the k-streaming forward: the longer
+    # non-k axis <= 64, 65..70, >= 71, either axis longer, short/odd k
+    import adtomo_hip as A
+    r = synth.rng(sum(shape))
+    f = 1.0 / synth.layered_grf_model(shape, 0.5, seed=sum(shape))
+    u0 = synth.point_source_u0(shape, tuple(int(r.integers(0, s)) for s in shape))
+    u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, 1e-4)
+    u = A.eikonal3d(dev(u0), dev(f), 0.5, *shape, 1e-4, False)
+    print(shape, "K", it, flush=True)
+    assert A.ops.last_info["iterations"] == it
+    assert bits_equal(host(u), u_ref)
+
+
 @pytest.mark.parametrize("tol", [1e-3, 1e-6])
 def test_config2_forward_adjoint(tol):
     # 64x64x32, h=1, 4->8 km/s layered x (1 +/- 5%) checkerboard (period 16),

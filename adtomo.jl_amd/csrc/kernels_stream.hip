@@ -4,11 +4,11 @@
 // hyperplanes a+b+c = s in increasing s, where (a, b, c) are the indices
 // counted along the sweep direction (c along k, the contiguous axis).  Each
 // lane owns a column (a, b) and streams it along c, one node per step,
-// keeping the column in registers (16-node windows loaded as whole cache
-// lines).  The columns of one wave lie on one "virtual anti-diagonal" v, so
-// all lanes of a wave are at the same c at every step; wave v runs one step
-// behind wave v-1, which is the hyperplane order.  Neighbour values cross
-// waves through LDS once per step (one workgroup barrier per step).
+// keeping the column in registers (16-node windows).  The columns of one
+// wave lie on one "virtual anti-diagonal" v, so all lanes of a wave are at
+// the same c at every step; wave v runs one step behind wave v-1, which is
+// the hyperplane order.  Neighbour values cross waves through LDS once per
+// step (one workgroup barrier per step).
 //
 // Column mapping.  Let b run along the longer of the two non-k axes (N_b
 // nodes) and a along the other (M_a nodes).  Lanes p < H (H <= 64) of a
@@ -22,6 +22,12 @@
 // (LDS), and across groups or bands they are at least Lp - W + 1 steps apart
 // and go through global memory, where they have been stored already.
 //
+// Memory instructions are never skipped: loads and stores of lanes or nodes
+// outside the event use an out-of-range buffer offset, which the hardware
+// turns into a zero load / dropped store.  Unconditional memory operations
+// keep the compiler's wait counts exact, so a wait for a prefetched value
+// does not also wait for younger loads.
+//
 // Iteration (Eikonal3D.cpp:71-88): u = u0 is fused into the first sweep
 // (it reads u0 and writes u); the snapshot u_old is written during the first
 // sweep of each later iteration; err = max|u - u_old| is a coalesced pass
@@ -33,16 +39,35 @@ namespace adt {
 
 namespace {
 
-// Explicit address spaces: generic (flat) accesses would make every LDS
-// wait also wait for all outstanding global loads.
-template <typename T>
-using gptr = __attribute__((address_space(1))) T*;
 template <typename T>
 using lptr = __attribute__((address_space(3))) T*;
 
 constexpr int kW = 8;       // waves per workgroup
 constexpr int kC = 16;      // window length (nodes along k)
 constexpr int kPF = 2;      // prefetch distance (steps) of the global neighbour reads
+constexpr uint32_t kOOB = 0xFFFFFFF0u;  // out-of-range buffer offset
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint64_t bytes) {
+  const uint32_t nb = bytes >= 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, nb, 0x00020000);
+}
+
+__device__ __forceinline__ double bload(rsrc_t r, uint32_t off, double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float bload(rsrc_t r, uint32_t off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, double v) {
+  using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, float v) {
+  using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, v), r, off, 0, 0);
+}
 
 struct Geom {
   int Ma, Nb, L, H, Lp, G, delta;
@@ -75,11 +100,11 @@ __device__ inline Geom make_geom(int m, int n, int l) {
   return g;
 }
 
-// Column of lane p on virtual diagonal v, with the canonical-origin address.
+// Column of lane p on virtual diagonal v, with the canonical-origin node index.
 struct Col {
   int a, b;
   bool valid;
-  int64_t addr;  // address of canonical (a, b, c = 0)
+  int64_t node0;  // flat index of canonical (a, b, c = 0)
 };
 
 __device__ __forceinline__ Col col_of(const Geom& g, int v, int p, int64_t base0, int64_t tA,
@@ -92,127 +117,136 @@ __device__ __forceinline__ Col col_of(const Geom& g, int v, int p, int64_t base0
   c.b = qq - beta * g.Nb;
   c.a = beta * g.H + p;
   c.valid = c.valid && (c.a < g.Ma);
-  c.addr = base0 + (int64_t)c.a * tA + (int64_t)c.b * tB;
+  c.node0 = base0 + (int64_t)c.a * tA + (int64_t)c.b * tB;
   return c;
 }
 
 template <typename T>
 struct Sweep {
-  // geometry and direction
   Geom g;
   int64_t tA, tB, tC, base0;
   bool flipA, flipB, flipC;
   T h;
-  gptr<const T> src;           // old values (u0 in the very first sweep)
-  gptr<T> dst;                 // u
-  gptr<const T> f;
-  gptr<T> snap_w;              // snapshot written at chunk start (first sweep of it >= 1)
-  gptr<const T> snap_r;        // snapshot the iteration's error is measured against
+  rsrc_t src;     // old values (u0 in the very first sweep)
+  rsrc_t dst;     // u
+  rsrc_t f;
+  rsrc_t snap_w;  // snapshot written at chunk start (first sweep of it >= 1)
 };
 
 // Combines the canonical upwind (new) / downwind (old) values of one axis
-// into the reference's neighbour minimum along the real axis.
+// into the reference's neighbour minimum along the real axis
+// (std::min(u(+1), u(-1)), one-sided at the edges).
 template <typename T>
-__device__ __forceinline__ T axis_min(T up, T dn, bool flip, int a, int ext) {
+__device__ __forceinline__ T axis_min(T up, T dn, bool flip, bool first, bool last) {
   const T plus = flip ? up : dn;
   const T minus = flip ? dn : up;
-  const int r = flip ? ext - 1 - a : a;
-  return r == 0 ? plus : (r == ext - 1 ? minus : smin(plus, minus));
+  return first ? plus : (last ? minus : smin(plus, minus));
 }
 
 template <typename T>
-__device__ __forceinline__ void load_chunk(T (&buf)[kC], gptr<const T> base, const Col& col,
-                                           int c0, int L, int64_t tC) {
+__device__ __forceinline__ uint32_t boff(bool ok, int64_t node) {
+  return ok ? (uint32_t)(node * (int64_t)sizeof(T)) : kOOB;
+}
+
+template <typename T, int NE>
+__device__ __forceinline__ void load_n(T (&buf)[NE], rsrc_t r, const Col& col, int c0, int L,
+                                       int64_t tC) {
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+    buf[e] = bload(r, boff<T>(col.valid && c0 + e < L, col.node0 + (int64_t)(c0 + e) * tC), T(0));
+}
+
+template <typename T>
+__device__ __forceinline__ void store_chunk(const T (&buf)[kC], rsrc_t r, const Col& col, int c0,
+                                            int L, int64_t tC) {
 #pragma unroll
   for (int e = 0; e < kC; ++e)
-    if (col.valid && c0 + e < L) buf[e] = base[col.addr + (int64_t)(c0 + e) * tC];
+    bstore(r, boff<T>(col.valid && c0 + e < L, col.node0 + (int64_t)(c0 + e) * tC), buf[e]);
 }
-
-template <typename T>
-__device__ __forceinline__ void load_half(T (&buf)[kC / 2], gptr<const T> base,
-                                          const Col& col, int c0, int L, int64_t tC) {
-#pragma unroll
-  for (int e = 0; e < kC / 2; ++e)
-    if (col.valid && c0 + e < L) buf[e] = base[col.addr + (int64_t)(c0 + e) * tC];
-}
-
-template <typename T>
-__device__ __forceinline__ void store_chunk(const T (&buf)[kC], gptr<T> base, const Col& col,
-                                            int c0, int L, int64_t tC) {
-#pragma unroll
-  for (int e = 0; e < kC; ++e)
-    if (col.valid && c0 + e < L) base[col.addr + (int64_t)(c0 + e) * tC] = buf[e];
-}
-
-}  // namespace
 
 // Per-wave state of one sweep.
 template <typename T>
 struct Lane {
   int w, p;
   bool up_glob_a, up_glob_b, dn_glob_a, dn_glob_b;
-  int sp, dp;              // LDS lanes of the upwind-a / downwind-a neighbours
-  int wu, wd;              // LDS waves of the upwind / downwind diagonal (clamped)
-  int step;                // global step index (parity of the LDS buffers)
-  Col pcol;                // column of the stream position being prefetched
-  int pgroup;
+  int sp, dp;   // LDS lanes of the upwind-a / downwind-a neighbours
+  int wu, wd;   // LDS waves of the upwind / downwind diagonal (clamped)
+  int step;     // global step index (parity of the LDS buffers)
+  // prefetch cursor (kPF steps ahead of the stream)
+  Col pcol;
+  int pgroup, pc;
+  bool nUA, nUB, nDA, nDB;  // which neighbours of pcol this lane reads globally
   T pUA[kPF], pUB[kPF], pDA[kPF], pDB[kPF];
 };
 
-// Loads, for stream position s, the neighbour values this lane takes from
-// global memory (upwind = new values in dst, downwind = old values in src).
 template <typename T>
-__device__ __forceinline__ void glob_fetch(const Sweep<T>& S, Lane<T>& ln, int s, int slot) {
+__device__ __forceinline__ void set_prefetch_col(const Sweep<T>& S, Lane<T>& ln, int group) {
   const Geom& g = S.g;
-  const int gs = s / g.Lp;
-  const int c = s - gs * g.Lp;
-  if (gs != ln.pgroup) {
-    ln.pgroup = gs;
-    ln.pcol = col_of(g, gs * kW + ln.w, ln.p, S.base0, S.tA, S.tB);
-  }
-  if (gs >= g.G || !ln.pcol.valid || c >= g.L) return;
-  const int64_t node = ln.pcol.addr + (int64_t)c * S.tC;
-  if (ln.up_glob_a && ln.pcol.a > 0) ln.pUA[slot] = S.dst[node - S.tA];
-  if (ln.up_glob_b && ln.pcol.b > 0) ln.pUB[slot] = S.dst[node - S.tB];
-  if (ln.dn_glob_a && ln.pcol.a + 1 < g.Ma) ln.pDA[slot] = S.src[node + S.tA];
-  if (ln.dn_glob_b && ln.pcol.b + 1 < g.Nb) ln.pDB[slot] = S.src[node + S.tB];
+  ln.pgroup = group;
+  ln.pc = 0;
+  ln.pcol = col_of(g, group * kW + ln.w, ln.p, S.base0, S.tA, S.tB);
+  const bool v = ln.pcol.valid && group < g.G;
+  ln.nUA = v && ln.up_glob_a && ln.pcol.a > 0;
+  ln.nUB = v && ln.up_glob_b && ln.pcol.b > 0;
+  ln.nDA = v && ln.dn_glob_a && ln.pcol.a + 1 < g.Ma;
+  ln.nDB = v && ln.dn_glob_b && ln.pcol.b + 1 < g.Nb;
+}
+
+// Loads the globally-read neighbours of the cursor node into `slot`, then
+// advances the cursor by one stream position.
+template <typename T>
+__device__ __forceinline__ void glob_fetch(const Sweep<T>& S, Lane<T>& ln, int slot) {
+  const Geom& g = S.g;
+  const bool inc = ln.pc < g.L;
+  const int64_t node = ln.pcol.node0 + (int64_t)ln.pc * S.tC;
+  ln.pUA[slot] = bload(S.dst, boff<T>(inc && ln.nUA, node - S.tA), T(0));
+  ln.pUB[slot] = bload(S.dst, boff<T>(inc && ln.nUB, node - S.tB), T(0));
+  ln.pDA[slot] = bload(S.src, boff<T>(inc && ln.nDA, node + S.tA), T(0));
+  ln.pDB[slot] = bload(S.src, boff<T>(inc && ln.nDB, node + S.tB), T(0));
+  if (++ln.pc == g.Lp) set_prefetch_col(S, ln, ln.pgroup + 1);
 }
 
 // Processes one 16-node chunk of the lane's stream held in A (old values in,
 // new values out); B holds the next chunk (old values).  The slowness comes
 // in two 8-node halves: F0 (elements 0-7, loaded beforehand) and F1
 // (elements 8-15, loaded at element 0); at element 8, F0 is refilled with
-// the first half of the next chunk (column ncol, offset nc0, if nvalid).
+// the first half of the next chunk (column ncol, offset nc0).
 template <typename T>
 __device__ __forceinline__ void process_chunk(const Sweep<T>& S, Lane<T>& ln, const Col& col,
-                                              int s0, int c0, T (&A)[kC], const T (&B)[kC],
+                                              bool afirst, bool alast, bool bfirst, bool blast,
+                                              int c0, T (&A)[kC], const T (&B)[kC],
                                               T (&F0)[kC / 2], T (&F1)[kC / 2], const Col& ncol,
-                                              int nc0, bool nvalid, T& prev_new,
-                                              lptr<T> lds_new, lptr<T> lds_old) {
+                                              int nc0, T& prev_new, lptr<T> lds_new,
+                                              lptr<T> lds_old) {
   const Geom& g = S.g;
   const int w = ln.w, p = ln.p;
 #pragma unroll
   for (int e = 0; e < kC; ++e) {
     const int c = c0 + e;
     const int slot = e % kPF;
-    if (e == 0) load_half(F1, S.f, col, c0 + kC / 2, g.L, S.tC);
-    if (e == kC / 2 && nvalid) load_half(F0, S.f, ncol, nc0, g.L, S.tC);
+    if (e == 0) load_n(F1, S.f, col, c0 + kC / 2, g.L, S.tC);
+    if (e == kC / 2) load_n(F0, S.f, ncol, nc0, g.L, S.tC);
     const T gUA = ln.pUA[slot], gUB = ln.pUB[slot], gDA = ln.pDA[slot], gDB = ln.pDB[slot];
-    glob_fetch(S, ln, s0 + e + kPF, slot);
+    glob_fetch(S, ln, slot);
     const int rp = (ln.step & 1) ^ 1;  // buffers written in the previous step
     const int wp = ln.step & 1;
     const T own = A[e];
     const T od_c = (e + 1 < kC) ? A[(e + 1) % kC] : B[0];
     const T od_c2 = (e + 2 < kC) ? A[(e + 2) % kC] : B[(e + 2) % kC];
-    const T ua = ln.up_glob_a ? gUA : lds_new[(rp * kW + ln.wu) * 64 + ln.sp];
-    const T ub = ln.up_glob_b ? gUB : lds_new[(rp * kW + ln.wu) * 64 + p];
-    const T da = ln.dn_glob_a ? gDA : lds_old[(rp * kW + ln.wd) * 64 + ln.dp];
-    const T db = ln.dn_glob_b ? gDB : lds_old[(rp * kW + ln.wd) * 64 + p];
+    const T lua = lds_new[(rp * kW + ln.wu) * 64 + ln.sp];
+    const T lub = lds_new[(rp * kW + ln.wu) * 64 + p];
+    const T lda = lds_old[(rp * kW + ln.wd) * 64 + ln.dp];
+    const T ldb = lds_old[(rp * kW + ln.wd) * 64 + p];
+    const T ua = ln.up_glob_a ? gUA : lua;
+    const T ub = ln.up_glob_b ? gUB : lub;
+    const T da = ln.dn_glob_a ? gDA : lda;
+    const T db = ln.dn_glob_b ? gDB : ldb;
     T un = own;
     if (col.valid && c < g.L) {
-      const T ma = axis_min(ua, da, S.flipA, col.a, g.Ma);
-      const T mb = axis_min(ub, db, S.flipB, col.b, g.Nb);
-      const T mc = axis_min(prev_new, od_c, S.flipC, c, g.L);
+      const int rc = S.flipC ? g.L - 1 - c : c;
+      const T ma = axis_min(ua, da, S.flipA, afirst, alast);
+      const T mb = axis_min(ub, db, S.flipB, bfirst, blast);
+      const T mc = axis_min(prev_new, od_c, S.flipC, rc == 0, rc == g.L - 1);
       const T ux = g.a_is_i ? ma : mb;
       const T uy = g.a_is_i ? mb : ma;
       const T fe = e < kC / 2 ? F0[e % (kC / 2)] : F1[e % (kC / 2)];
@@ -227,6 +261,8 @@ __device__ __forceinline__ void process_chunk(const Sweep<T>& S, Lane<T>& ln, co
     __syncthreads();
   }
 }
+
+}  // namespace
 
 // One sweep of one event by the whole workgroup.
 template <typename T, bool SNAP>
@@ -247,24 +283,18 @@ __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_
   ln.dp = (p == g.H - 1) ? 0 : p + 1;
   ln.wu = w > 0 ? w - 1 : 0;
   ln.wd = w < kW - 1 ? w + 1 : kW - 1;
-  ln.pgroup = -1;
-#pragma unroll
-  for (int k = 0; k < kPF; ++k) ln.pUA[k] = ln.pUB[k] = ln.pDA[k] = ln.pDB[k] = T(0);
   const int L = g.L, Lp = g.Lp;
   const int chunks = g.G * (Lp / kC);
 
   T X[kC], Y[kC], F0[kC / 2], F1[kC / 2];
   Col cur = col_of(g, w, p, S.base0, S.tA, S.tB);        // group 0
   Col nxt = col_of(g, kW + w, p, S.base0, S.tA, S.tB);   // group 1
-  // chunks 0 and 1 of the stream
-  load_chunk(X, S.src, cur, 0, L, S.tC);
-  load_half(F0, S.f, cur, 0, L, S.tC);
-  {
-    const bool same = Lp > kC;
-    load_chunk(Y, S.src, same ? cur : nxt, same ? kC : 0, L, S.tC);
-  }
+  load_n(X, S.src, cur, 0, L, S.tC);
+  load_n(F0, S.f, cur, 0, L, S.tC);
+  load_n(Y, S.src, cur, kC, L, S.tC);  // Lp >= 2 * kC: chunk 1 is in the same column
+  set_prefetch_col(S, ln, 0);
 #pragma unroll
-  for (int k = 0; k < kPF; ++k) glob_fetch(S, ln, k, k);
+  for (int k = 0; k < kPF; ++k) glob_fetch(S, ln, k);
 
   // Wave w processes stream position s at global step T = w + 1 + s and, at
   // step T, publishes old(s + 2) for the upwind wave's next step.  Before its
@@ -282,6 +312,17 @@ __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_
 
   T prev_new = T(0);
   int gcur = 0;
+  // boundary flags of the current column along the real a and b axes
+  auto edges = [&](const Col& c, bool& af, bool& al, bool& bf, bool& bl) {
+    const int ra = S.flipA ? g.Ma - 1 - c.a : c.a;
+    const int rb = S.flipB ? g.Nb - 1 - c.b : c.b;
+    af = ra == 0;
+    al = ra == g.Ma - 1;
+    bf = rb == 0;
+    bl = rb == g.Nb - 1;
+  };
+  bool af, al, bf, bl;
+  edges(cur, af, al, bf, bl);
   // One chunk: A holds it (old values), B the finished previous chunk (new
   // values, stored here) and then the following chunk (old values).
   auto do_chunk = [&](int jj, T(&A)[kC], T(&B)[kC]) {
@@ -293,20 +334,20 @@ __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_
       gcur = gs;
       cur = nxt;
       nxt = col_of(g, (gs + 1) * kW + w, p, S.base0, S.tA, S.tB);
+      edges(cur, af, al, bf, bl);
     }
     const int sn = s0 + kC;
     const int gn = sn / Lp;
-    const bool nvalid = gn < g.G;
     const Col& cn = (gn == gs) ? cur : nxt;
     const int cn0 = sn - gn * Lp;
     if (jj > 0) {
       const int sp0 = s0 - kC;
       const int gp = sp0 / Lp;
       store_chunk(B, S.dst, gp == gs ? cur : prev, sp0 - gp * Lp, L, S.tC);
-      if (nvalid) load_chunk(B, S.src, cn, cn0, L, S.tC);
+      load_n(B, S.src, cn, cn0, L, S.tC);
     }
     if (SNAP) store_chunk(A, S.snap_w, cur, c0, L, S.tC);
-    process_chunk<T>(S, ln, cur, s0, c0, A, B, F0, F1, cn, cn0, nvalid, prev_new, lds_new,
+    process_chunk<T>(S, ln, cur, af, al, bf, bl, c0, A, B, F0, F1, cn, cn0, prev_new, lds_new,
                      lds_old);
   };
   for (int j = 0; j < chunks; j += 2) {
@@ -336,16 +377,18 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
   __shared__ uint64_t red[kW];
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
+  const int64_t N = (int64_t)m * n * l;
+  const uint64_t bytes = (uint64_t)N * sizeof(T);
   Sweep<T> S;
   S.g = make_geom(m, n, l);
   S.h = (T)ev.h;
-  S.f = (gptr<const T>)ev.f;
-  S.dst = (gptr<T>)ev.u;
+  S.f = make_rsrc(ev.f, bytes);
+  S.dst = make_rsrc(ev.u, bytes);
+  S.snap_w = make_rsrc(ev.uold, bytes);
   const T tol = (T)ev.tol;
   int it = 0;
   T err = T(0);
   for (it = 0; it < max_iter; ++it) {
-    uint64_t key = 0;
     for (int sw = 0; sw < 8; ++sw) {
       const int dir = kDir3[sw];
       const bool fi = dir & 1, fj = dir & 2, fk = dir & 4;
@@ -357,25 +400,22 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
       S.tC = S.flipC ? -1 : 1;
       S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
-      S.src = (gptr<const T>)((it == 0 && sw == 0) ? ev.u0 : ev.u);
-      S.snap_w = (gptr<T>)ev.uold;
-      S.snap_r = (gptr<const T>)((it == 0) ? ev.u0 : ev.uold);
+      S.src = make_rsrc((it == 0 && sw == 0) ? ev.u0 : ev.u, bytes);
       if (sw == 0 && it > 0)
         stream_sweep<T, true>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
       else
         stream_sweep<T, false>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
     }
     // err = max |u - u_old| over the event (coalesced pass after the 8 sweeps)
+    uint64_t key = 0;
     {
-      gptr<const T> ua = (gptr<const T>)ev.u;
-      gptr<const T> sr = S.snap_r;
-      const int64_t N = (int64_t)m * n * l;
+      const T* __restrict__ ua = (const T*)ev.u;
+      const T* __restrict__ sr = (const T*)((it == 0) ? ev.u0 : ev.uold);
       for (int64_t q = threadIdx.x; q < N; q += blockDim.x) {
         const uint64_t kq = maxkey(Real<T>::fabs_(ua[q] - sr[q]));
         key = kq > key ? kq : key;
       }
     }
-    // workgroup max of err
     for (int off = 32; off > 0; off >>= 1) {
       uint64_t o = __shfl_xor(key, off);
       key = o > key ? o : key;

@@ -175,8 +175,8 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
       (rc = st.out(grad_u0, bytes, hp, &dgu0, s)) || (rc = st.out(grad_f, bytes, hp, &dgf, s)))
     return rc;
   DevBuf ws;
-  const size_t a = ((bytes + 255) / 256) * 256;
-  const size_t c = ((size_t(N) + 255) / 256) * 256;
+  const size_t a = ((4 * bytes + 255) / 256) * 256;  // per-node records (4 values)
+  const size_t c = 0;
   const size_t q = ((size_t(N) * 4 + 255) / 256) * 256;
   if ((rc = ws.alloc(a + 2 * c + q + 256 + sizeof(adt_info), s))) return rc;
   char* base = (char*)ws.p;
@@ -190,8 +190,8 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   d.grad_u = dg;
   d.u = (void*)du;
   d.lam = base;
-  d.code = (uint8_t*)(base + a);
-  d.indeg = (uint8_t*)(base + a + c);
+  d.code = nullptr;
+  d.indeg = nullptr;
   d.queue = (int32_t*)(base + a + 2 * c);
   d.grad_u0 = dgu0;
   d.grad_f = dgf;
@@ -340,9 +340,9 @@ struct BatchPlan {
         return fail(ADT_EINVAL, "event " + std::to_string(e) + ": NULL adjoint buffer");
       const size_t N = size_t(x.m * x.n * x.l);
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
-      o_lam[e] = (flags & ADT_ADJOINT) ? take(N * sizeof(T)) : 0;
-      o_code[e] = (flags & ADT_ADJOINT) ? take(N) : 0;
-      o_indeg[e] = (flags & ADT_ADJOINT) ? take(N) : 0;
+      o_lam[e] = (flags & ADT_ADJOINT) ? take(4 * N * sizeof(T)) : 0;  // per-node records
+      o_code[e] = 0;
+      o_indeg[e] = 0;
       o_queue[e] = (flags & ADT_ADJOINT) ? take(4 * N) : 0;
     }
     off_desc = take(sizeof(adt::EventDesc) * (nev ? nev : 1));
@@ -377,8 +377,8 @@ struct BatchPlan {
     for (auto& d : descs) {
       d.uold = base + (size_t)d.uold;
       d.lam = base + (size_t)d.lam;
-      d.code = (uint8_t*)(base + (size_t)d.code);
-      d.indeg = (uint8_t*)(base + (size_t)d.indeg);
+      d.code = nullptr;
+      d.indeg = nullptr;
       d.queue = (int32_t*)(base + (size_t)d.queue);
       d.info = (adt_info*)(base + (size_t)d.info);
     }

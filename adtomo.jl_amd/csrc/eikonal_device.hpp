@@ -14,6 +14,12 @@
 
 namespace adt {
 
+// Pointers into device (global) memory.  Pointers read from descriptor
+// tables are generic; generic (flat) accesses share the wait counter with
+// LDS accesses, so kernels convert them to this type once.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
 template <typename T> struct Real;
 template <> struct Real<double> {
   static __device__ __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }

@@ -409,8 +409,8 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
     // err = max |u - u_old| over the event (coalesced pass after the 8 sweeps)
     uint64_t key = 0;
     {
-      const T* __restrict__ ua = (const T*)ev.u;
-      const T* __restrict__ sr = (const T*)((it == 0) ? ev.u0 : ev.uold);
+      const gptr<const T> ua = (gptr<const T>)ev.u;
+      const gptr<const T> sr = (gptr<const T>)((it == 0) ? ev.u0 : ev.uold);
       for (int64_t q = threadIdx.x; q < N; q += blockDim.x) {
         const uint64_t kq = maxkey(Real<T>::fabs_(ua[q] - sr[q]));
         key = kq > key ? kq : key;

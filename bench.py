@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--fp32", action="store_true", help="also time the FP32 variant")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -54,6 +56,13 @@ def main():
     from adtomo_hip import synth
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.calib_copy:
+        # 2^27 doubles = 1 GiB read + 1 GiB written by one copy kernel
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     shape = (128, 128, 64)
     m, n, l = shape
     N = m * n * l

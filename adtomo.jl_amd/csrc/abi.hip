@@ -35,6 +35,9 @@ constexpr int kMaxIter2 = 100;        // Eikonal.h:73
 constexpr int kMaxAdjSweeps = 8 * 64;  // cap of the adjoint fixed-point sweeps
 constexpr int kWG = 1024;
 constexpr int kStreamWG = 512;  // fwd3_stream_kernel: 8 waves
+constexpr int kDiagWG = 512;    // fwd3_diag_kernel: 8 waves
+// The forward kernels address each array with 32-bit buffer offsets.
+constexpr size_t kMaxArrayBytes = 0xFFFFFFF0ull;
 
 // Stream-ordered device allocation that is released when the scope ends.
 struct DevBuf {
@@ -121,6 +124,7 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   const bool hp = flags & ADT_HOST_POINTERS;
   const int64_t N = m * n * l;
   const size_t bytes = sizeof(T) * N;
+  if (bytes >= kMaxArrayBytes) return fail(ADT_EINVAL, "eikonal3d: arrays of 4 GiB or more are not supported");
   Staged st;
   const void *du0, *df;
   void* du;
@@ -144,7 +148,11 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   d.uold = base;
   d.info = (adt_info*)(base + off_info);
   ADT_HIP(hipMemcpyAsync(base + off_desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
-  adt::fwd3_stream_kernel<T><<<1, kStreamWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3);
+  if (l <= adt::kDiagMaxL)
+    adt::fwd3_diag_kernel<T><<<1, kDiagWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3);
+  else
+    adt::fwd3_stream_kernel<T><<<1, kStreamWG, 0, s>>>((const adt::EventDesc*)(base + off_desc),
+                                                      kMaxIter3, adt::kDiagMaxL);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -318,6 +326,7 @@ struct BatchPlan {
   std::vector<adt::EventDesc> descs;
   size_t ws_bytes = 0;
   size_t off_desc = 0, off_info = 0;
+  int max_l = 0;  // largest k extent of the batch
 
   int build(const adt_event3d* ev, int nev, int flags) {
     if (nev < 0) return fail(ADT_EINVAL, "batched: nev < 0");
@@ -339,6 +348,9 @@ struct BatchPlan {
       if ((flags & ADT_ADJOINT) && (!x.grad_u || !x.grad_f || !x.u0))
         return fail(ADT_EINVAL, "event " + std::to_string(e) + ": NULL adjoint buffer");
       const size_t N = size_t(x.m * x.n * x.l);
+      if (N * sizeof(T) >= kMaxArrayBytes)
+        return fail(ADT_EINVAL, "event " + std::to_string(e) + ": arrays of 4 GiB or more are not supported");
+      if (x.l > max_l) max_l = (int)x.l;
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
       o_lam[e] = (flags & ADT_ADJOINT) ? take(4 * N * sizeof(T)) : 0;  // per-node records
       o_code[e] = 0;
@@ -386,11 +398,16 @@ struct BatchPlan {
 };
 
 template <typename T>
-int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, hipStream_t s) {
+int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, hipStream_t s) {
   if (nev == 0) return ADT_OK;
   if (flags & ADT_FORWARD) {
-    adt::fwd3_stream_kernel<T><<<nev, kStreamWG, 0, s>>>(ddesc, kMaxIter3);
+    // events with l <= kDiagMaxL: diagonal streaming; longer ones: k-streaming
+    adt::fwd3_diag_kernel<T><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3);
     ADT_HIP(hipGetLastError());
+    if (max_l > adt::kDiagMaxL) {
+      adt::fwd3_stream_kernel<T><<<nev, kStreamWG, 0, s>>>(ddesc, kMaxIter3, adt::kDiagMaxL);
+      ADT_HIP(hipGetLastError());
+    }
   }
   if (flags & ADT_ADJOINT) {
     adt::adj3_kahn_kernel<T><<<nev, kWG, 0, s>>>(ddesc);
@@ -414,7 +431,9 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   char* base = (char*)ws.p;
   ADT_HIP(hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
                          hipMemcpyHostToDevice, s));
-  if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, s))) return rc;
+  if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags,
+                            plan.max_l, s)))
+    return rc;
   if (infos) {
     ADT_HIP(hipMemcpyAsync(infos, base + plan.off_info, sizeof(adt_info) * nev,
                            hipMemcpyDeviceToHost, s));
@@ -579,7 +598,7 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, cs);
+    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, cs);
     hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
     if (rc == ADT_OK && e2 != hipSuccess) e = e2;
   }

@@ -60,6 +60,16 @@ __device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h) {
   return x;
 }
 
+// Combines the canonical upwind (new) / downwind (old) values of one axis
+// into the reference's neighbour minimum along the real axis
+// (std::min(u(+1), u(-1)), one-sided at the edges).
+template <typename T>
+__device__ __forceinline__ T axis_min(T up, T dn, bool flip, bool first, bool last) {
+  const T plus = flip ? up : dn;
+  const T minus = flip ? dn : up;
+  return first ? plus : (last ? minus : smin(plus, minus));
+}
+
 // Bit pattern of a value (for exact "changed?" tests that also hold for NaN).
 __device__ __forceinline__ uint64_t bits_of(double x) { return __double_as_longlong(x); }
 __device__ __forceinline__ uint64_t bits_of(float x) { return (uint32_t)__float_as_uint(x); }

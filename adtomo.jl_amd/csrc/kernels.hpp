@@ -50,7 +50,11 @@ __global__ void fwd3_plane_kernel(T* u, const T* f, T h, int m, int n, int l, in
 template <typename T>
 __global__ void fwd3_event_kernel(const EventDesc* evs, int max_iter);
 template <typename T>
-__global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter);
+__global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter, int min_l);
+// Largest k extent handled by fwd3_diag_kernel (8 waves x 8 nodes).
+constexpr int kDiagMaxL = 64;
+template <typename T>
+__global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter);
 template <typename T>
 __global__ void copy_kernel(T* dst, const T* src, int64_t N);
 template <typename T>

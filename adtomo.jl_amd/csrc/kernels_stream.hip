@@ -32,6 +32,7 @@
 // (it reads u0 and writes u); the snapshot u_old is written during the first
 // sweep of each later iteration; err = max|u - u_old| is a coalesced pass
 // after the eighth sweep against the snapshot (u0 for the first iteration).
+#include "buffer_ops.hpp"
 #include "eikonal_device.hpp"
 #include "kernels.hpp"
 
@@ -44,30 +45,7 @@ using lptr = __attribute__((address_space(3))) T*;
 
 constexpr int kW = 8;       // waves per workgroup
 constexpr int kC = 16;      // window length (nodes along k)
-constexpr int kPF = 2;      // prefetch distance (steps) of the global neighbour reads
-constexpr uint32_t kOOB = 0xFFFFFFF0u;  // out-of-range buffer offset
-
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-
-__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint64_t bytes) {
-  const uint32_t nb = bytes >= 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)bytes;
-  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, nb, 0x00020000);
-}
-
-__device__ __forceinline__ double bload(rsrc_t r, uint32_t off, double) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ __forceinline__ float bload(rsrc_t r, uint32_t off, float) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-__device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, double v) {
-  using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, off, 0, 0);
-}
-__device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, float v) {
-  using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, v), r, off, 0, 0);
-}
+constexpr int kPF = 4;      // prefetch distance (steps) of the global neighbour reads
 
 struct Geom {
   int Ma, Nb, L, H, Lp, G, delta;
@@ -132,21 +110,6 @@ struct Sweep {
   rsrc_t f;
   rsrc_t snap_w;  // snapshot written at chunk start (first sweep of it >= 1)
 };
-
-// Combines the canonical upwind (new) / downwind (old) values of one axis
-// into the reference's neighbour minimum along the real axis
-// (std::min(u(+1), u(-1)), one-sided at the edges).
-template <typename T>
-__device__ __forceinline__ T axis_min(T up, T dn, bool flip, bool first, bool last) {
-  const T plus = flip ? up : dn;
-  const T minus = flip ? dn : up;
-  return first ? plus : (last ? minus : smin(plus, minus));
-}
-
-template <typename T>
-__device__ __forceinline__ uint32_t boff(bool ok, int64_t node) {
-  return ok ? (uint32_t)(node * (int64_t)sizeof(T)) : kOOB;
-}
 
 template <typename T, int NE>
 __device__ __forceinline__ void load_n(T (&buf)[NE], rsrc_t r, const Col& col, int c0, int L,
@@ -371,12 +334,13 @@ __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_
 
 template <typename T>
 __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* __restrict__ evs,
-                                                              int max_iter) {
+                                                              int max_iter, int min_l) {
   __shared__ T lds_new[2 * kW * 64];
   __shared__ T lds_old[2 * kW * 64];
   __shared__ uint64_t red[kW];
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
+  if (l <= min_l) return;  // handled by the diagonal-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
   Sweep<T> S;
@@ -439,7 +403,7 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
   }
 }
 
-template __global__ void fwd3_stream_kernel<double>(const EventDesc*, int);
-template __global__ void fwd3_stream_kernel<float>(const EventDesc*, int);
+template __global__ void fwd3_stream_kernel<double>(const EventDesc*, int, int);
+template __global__ void fwd3_stream_kernel<float>(const EventDesc*, int, int);
 
 }  // namespace adt

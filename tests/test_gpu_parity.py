@@ -138,10 +138,13 @@ def test_forward3d_bit_exact_small(shape, tol, seed):
 
 @pytest.mark.parametrize("shape", [
     (64, 64, 16), (48, 65, 23), (70, 40, 9), (20, 71, 33), (100, 72, 5), (130, 66, 17),
-    (3, 129, 40), (129, 3, 2)])
+    (3, 129, 40), (129, 3, 2), (65, 65, 57), (66, 140, 64), (128, 128, 3), (20, 24, 70),
+    (9, 66, 130)])
 def test_forward3d_column_mappings(shape):
-    # exercises the band mapping of the k-streaming forward: the longer
-    # non-k axis <= 64, 65..70, >= 71, either axis longer, short/odd k
+    # exercises the column mappings of the forward kernels: one band (shorter
+    # non-k axis <= 64) or several, either non-k axis longer, k segments of
+    # 1..8 nodes with idle waves (l <= 64, diagonal streaming) and l > 64
+    # (k-streaming)
     import adtomo_hip as A
     r = synth.rng(sum(shape))
     f = 1.0 / synth.layered_grf_model(shape, 0.5, seed=sum(shape))
@@ -276,6 +279,23 @@ def test_batched_uneven_fp64_bit_exact():
         gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(gs[e], u_ref, u0s[e], fs[e], 0.5)
         assert bits_equal(host(res["grad_u0"][e]), gu0_ref), e
         assert rel(host(res["grad_f"][e]), gf_ref) <= GRAD_RTOL, e
+
+
+def test_batched_mixed_k_extents_bit_exact():
+    # one batch with events for both forward kernels (l <= 64 and l > 64)
+    import adtomo_hip as A
+    shapes = [(10, 12, 70), (20, 9, 30), (8, 8, 100), (12, 14, 64), (6, 7, 65)]
+    fs, u0s = [], []
+    for e, shp in enumerate(shapes):
+        r = synth.rng(100 + e)
+        fs.append(r.uniform(0.3, 3.0, size=shp))
+        u0s.append(synth.point_source_u0(shp, tuple(int(r.integers(0, s)) for s in shp)))
+    res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(shapes),
+                              [1e-4] * len(shapes))
+    for e in range(len(shapes)):
+        u_ref, it, err = oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-4)
+        assert res["info"][e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), u_ref), e
 
 
 def test_batched_fp32_vs_oracle_f32_and_fp64_error():

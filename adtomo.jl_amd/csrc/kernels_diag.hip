@@ -18,11 +18,12 @@
 // The c axis is cut into kDW segments of Lw <= kLW nodes, one per wave.  At
 // step T wave w processes diagonal v = T - w over its segment, node by node
 // in increasing c.  Each node's neighbours then come from:
-//   (a-1, b), (a, b-1): the lane's and its left neighbour's new values of
-//       diagonal v-1 (registers of the previous step; DPP lane shift), or,
-//       across a band edge, memory (stored delta - 1 >= 32 steps before);
-//   (a+1, b), (a, b+1): old values of diagonal v+1, loaded one step ahead
-//       (DPP lane shift); across a band edge from memory (not yet updated);
+//   (a-1, b), (a, b-1): the new values of diagonal v-1 that lanes p-1 and p
+//       left in the wave's LDS plane n, or, across a band edge, memory
+//       (stored delta - 1 >= 32 steps before);
+//   (a+1, b), (a, b+1): old values of diagonal v+1 (lanes p+1 and p), loaded
+//       one step ahead into plane x; across a band edge from memory (not yet
+//       updated);
 //   (a, b, c-1): the lane's previous node, or for the first node of the
 //       segment wave w-1's last node of diagonal v (step T-1, through LDS);
 //   (a, b, c+1): the lane's next node, or one past the segment (loaded with
@@ -31,8 +32,10 @@
 //
 // Loads for diagonals v+1 (slowness, band-edge values) and v+2 (old travel
 // times) are issued at the start of step T and used at step T+1, so each
-// step hides one step of memory latency.  Memory instructions are never
-// skipped: masked lanes use an out-of-range buffer offset.
+// step hides one step of memory latency.  Segments move between memory and
+// the planes in 16-byte pieces, several lanes per contiguous run.  Memory
+// instructions are never skipped: masked lanes use an out-of-range buffer
+// offset.
 //
 // Iteration (Eikonal3D.cpp:71-88): u = u0 is a copy pass at the start; the
 // snapshot u_old is written during the first sweep of each later
@@ -51,22 +54,6 @@ using lptr = __attribute__((address_space(3))) T*;
 
 constexpr int kDW = 8;  // waves per workgroup = c segments
 constexpr int kLW = 8;  // maximum segment length (nodes per lane per step)
-
-// DPP wavefront shifts (gfx9 family): lane i reads lane i-1 / i+1.
-constexpr int kDppWaveShr1 = 0x138;
-constexpr int kDppWaveShl1 = 0x130;
-
-template <int CTRL>
-__device__ __forceinline__ double lane_shift(double x) {
-  const uint64_t b = (uint64_t)__double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf, false);
-  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
-}
-template <int CTRL>
-__device__ __forceinline__ float lane_shift(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
-}
 
 struct DGeom {
   int Ma, Nb, L, H, bands, V, Lw;
@@ -120,7 +107,8 @@ struct DSweep {
   rsrc_t snap;  // snapshot written during the first sweep of iteration >= 1
 };
 
-// cnt values of column `col` from canonical c0 on (element j <- c0 + j).
+// cnt values of column `col` from canonical c0 on (element j <- c0 + j),
+// one load per element.
 template <typename T, int NE>
 __device__ __forceinline__ void load_seg(T (&buf)[NE], rsrc_t r, const DCol& col, int64_t delta,
                                          bool ok, int c0, int cnt, int L, int64_t tC) {
@@ -131,13 +119,139 @@ __device__ __forceinline__ void load_seg(T (&buf)[NE], rsrc_t r, const DCol& col
   }
 }
 
+// ---- per-wave LDS planes and coalesced segment transfers ------------------
+//
+// Each wave keeps three planes in LDS, each holding one value per (node of
+// the segment, column of the wave): element m (memory order) of lane q's
+// segment sits at row m, position q + 1.  Position 0 stands for lane -1 and
+// position H + 1 for lane H: the band-edge neighbours of lanes 0 and H-1 are
+// written there, so every lane reads its a-neighbours at positions p and
+// p + 2 without exceptions.  Rows are padded so the piece writes spread over
+// the banks.
+//
+// Segments move between memory and a plane in 16-byte pieces: lane L of
+// transfer i handles piece L % LPC of the segment of lane i*CPI + L / LPC,
+// so the LPC lanes of one segment read or write one contiguous run.  A
+// segment whose length is not a multiple of the piece is finished by its
+// own lane, element by element.
+
+using u32x4 = __attribute__((ext_vector_type(4))) unsigned int;
+
 template <typename T>
-__device__ __forceinline__ void store_seg(const T (&buf)[kLW + 1], rsrc_t r, const DCol& col,
-                                          int c0, int cnt, int L, int64_t tC) {
+struct Pieces {
+  static constexpr int EP = 16 / (int)sizeof(T);      // elements per piece
+  static constexpr int LPC = kLW / EP;                // lanes per segment
+  static constexpr int CPI = 64 / LPC;                // segments per transfer
+  static constexpr int RS = 64 + 32 / (int)sizeof(T);  // plane row stride (elements)
+  static constexpr int PLANE = kLW * RS;              // plane size (elements)
+};
+
+__device__ __forceinline__ u32x4 bload4(rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+}
+__device__ __forceinline__ void bstore4(rsrc_t r, uint32_t off, u32x4 v) {
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ T piece_get(const u32x4& v, int e);
+template <>
+__device__ __forceinline__ double piece_get<double>(const u32x4& v, int e) {
+  const uint32_t lo = e == 0 ? v.x : v.z, hi = e == 0 ? v.y : v.w;
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <>
+__device__ __forceinline__ float piece_get<float>(const u32x4& v, int e) {
+  return __uint_as_float(e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w);
+}
+
+template <typename T>
+__device__ __forceinline__ u32x4 piece_make(const T (&x)[16 / sizeof(T)]);
+template <>
+__device__ __forceinline__ u32x4 piece_make<double>(const double (&x)[2]) {
+  const uint64_t a = (uint64_t)__double_as_longlong(x[0]), b = (uint64_t)__double_as_longlong(x[1]);
+  u32x4 v;
+  v.x = (uint32_t)a;
+  v.y = (uint32_t)(a >> 32);
+  v.z = (uint32_t)b;
+  v.w = (uint32_t)(b >> 32);
+  return v;
+}
+template <>
+__device__ __forceinline__ u32x4 piece_make<float>(const float (&x)[4]) {
+  u32x4 v;
+  v.x = __float_as_uint(x[0]);
+  v.y = __float_as_uint(x[1]);
+  v.z = __float_as_uint(x[2]);
+  v.w = __float_as_uint(x[3]);
+  return v;
+}
+
+// Loads in flight for one segment transfer.
+template <typename T>
+struct Flight {
+  u32x4 pc[Pieces<T>::LPC];
+  T tail[Pieces<T>::EP - 1];
+};
+
+// Issues the loads of every lane's segment: `seg` = memory index of the
+// lowest node of the lane's segment (-1: none), cnt = nodes per segment.
+template <typename T>
+__device__ __forceinline__ void seg_issue(Flight<T>& fl, rsrc_t r, int seg, int cnt, int p) {
+  using P = Pieces<T>;
 #pragma unroll
-  for (int j = 0; j < kLW; ++j) {
-    const int c = c0 + j;
-    bstore(r, boff<T>(col.valid && j < cnt && c < L, col.node0 + (int64_t)c * tC), buf[j]);
+  for (int i = 0; i < P::LPC; ++i) {
+    const int q = i * P::CPI + p / P::LPC;
+    const int m0 = (p % P::LPC) * P::EP;
+    const int sq = __shfl(seg, q);
+    const bool ok = sq >= 0 && m0 + P::EP <= cnt;
+    fl.pc[i] = bload4(r, ok ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB);
+  }
+  const int nfull = (cnt / P::EP) * P::EP;
+#pragma unroll
+  for (int t = 0; t < P::EP - 1; ++t) {
+    const int m = nfull + t;
+    fl.tail[t] = bload(r, boff<T>(seg >= 0 && m < cnt, (int64_t)seg + m), T(0));
+  }
+}
+
+// Writes arrived loads into a plane.
+template <typename T>
+__device__ __forceinline__ void seg_land(const Flight<T>& fl, lptr<T> pl, int cnt, int p) {
+  using P = Pieces<T>;
+#pragma unroll
+  for (int i = 0; i < P::LPC; ++i) {
+    const int q = i * P::CPI + p / P::LPC;
+    const int m0 = (p % P::LPC) * P::EP;
+#pragma unroll
+    for (int e = 0; e < P::EP; ++e) pl[(m0 + e) * P::RS + q + 1] = piece_get<T>(fl.pc[i], e);
+  }
+  const int nfull = (cnt / P::EP) * P::EP;
+#pragma unroll
+  for (int t = 0; t < P::EP - 1; ++t)
+    if (nfull + t < cnt) pl[(nfull + t) * P::RS + p + 1] = fl.tail[t];
+}
+
+// Stores every lane's segment from a plane.
+template <typename T>
+__device__ __forceinline__ void seg_store(lptr<const T> pl, rsrc_t r, int seg, int cnt, int p) {
+  using P = Pieces<T>;
+#pragma unroll
+  for (int i = 0; i < P::LPC; ++i) {
+    const int q = i * P::CPI + p / P::LPC;
+    const int m0 = (p % P::LPC) * P::EP;
+    const int sq = __shfl(seg, q);
+    const bool ok = sq >= 0 && m0 + P::EP <= cnt;
+    T x[P::EP];
+#pragma unroll
+    for (int e = 0; e < P::EP; ++e) x[e] = pl[(m0 + e) * P::RS + q + 1];
+    bstore4(r, ok ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB, piece_make<T>(x));
+  }
+  const int nfull = (cnt / P::EP) * P::EP;
+#pragma unroll
+  for (int t = 0; t < P::EP - 1; ++t) {
+    const int m = nfull + t;
+    if (m < cnt) bstore(r, boff<T>(seg >= 0, (int64_t)seg + m), pl[m * P::RS + p + 1]);
   }
 }
 
@@ -146,68 +260,97 @@ __device__ __forceinline__ void store_seg(const T (&buf)[kLW + 1], rsrc_t r, con
 // One sweep of one event by the whole workgroup.  `snap`: also store every
 // node's old value to the snapshot.
 template <typename T>
-__device__ __forceinline__ void diag_sweep(const DSweep<T>& S_in, bool snap, lptr<T> xch) {
+__device__ __forceinline__ void diag_sweep(const DSweep<T>& S_in, bool snap, lptr<T> xch,
+                                           lptr<T> planes) {
+  using P = Pieces<T>;
   // Local copy: keeps the fields in registers.
   const DSweep<T> S = S_in;
   const DGeom& g = S.g;
   const int w = threadIdx.x >> 6, p = threadIdx.x & 63;
   const int L = g.L, Lw = g.Lw;
   const int c0 = w * Lw;  // canonical start of this wave's segment
+  int cnt = L - c0;       // nodes in this wave's segment (uniform)
+  cnt = cnt < 0 ? 0 : (cnt > Lw ? Lw : cnt);
+  const bool fwd = !S.flipC;  // the segment runs forward in memory
   const bool lane_first = (p == 0);
   const bool lane_last = (p == g.H - 1);
+  const bool has_x = c0 + Lw < L;  // a node exists one past the segment
+  // planes of this wave: new values of v-1 (then v), old values of v+1,
+  // slowness of v
+  lptr<T> pn = planes + w * 3 * P::PLANE;
+  lptr<T> px = pn + P::PLANE;
+  lptr<T> pf = px + P::PLANE;
 
-  // Registers: PREV = new values of diagonal v-1, OWN = diagonal v (old in,
-  // new out; element Lw is the old value one past the segment), NEXT = old
-  // values of v+1, F / EDGE = slowness and band-edge values of v.  LD, LF,
-  // LE are the loads in flight for the next step.
-  T PREV[kLW], OWN[kLW + 1], NEXT[kLW + 1], F[kLW], EDGE[kLW];
-  T LD[kLW + 1], LF[kLW], LE[kLW];
-
+  // Memory index of the lowest node of the segment of column `col`.
+  auto seg_of = [&](const DCol& col) -> int {
+    const int64_t lo = fwd ? col.node0 + c0 : col.node0 - c0 - (cnt - 1);
+    return (col.valid && cnt > 0) ? (int)lo : -1;
+  };
+  // One past the segment (old value), or kOOB.
+  auto x_off = [&](const DCol& col) -> uint32_t {
+    return boff<T>(col.valid && has_x, col.node0 + (int64_t)(c0 + Lw) * S.tC);
+  };
   // Band-edge neighbour of lane 0 ((a-1, b), new) and lane H-1 ((a+1, b),
-  // old) of column `col`, loaded into `buf`.
+  // old) of column `col`, one load per node.
   auto load_edge = [&](T(&buf)[kLW], const DCol& col) {
     const bool eu = lane_first && col.valid && col.a > 0;
     const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
-    load_seg(buf, S.u, col, eu ? -S.tA : S.tA, eu || ed, c0, Lw, L, S.tC);
+    load_seg(buf, S.u, col, eu ? -S.tA : S.tA, eu || ed, c0, cnt, L, S.tC);
   };
+  auto row = [&](int j) { return fwd ? j : cnt - 1 - j; };
+
+  // Registers: OWN = old values of diagonal v in, new values out; OWNX /
+  // NEXTX = old value one past the segment of v / v+1.  FD, FF, LE, LDX are
+  // the loads in flight for the next step.
+  T OWN[kLW], LE[kLW];
+  T OWNX, NEXTX, LDX;
+  Flight<T> FD, FF;
 
   const int v0 = -w;  // diagonal of step 0
   DCol cv = dcol(g, v0, p, S.base0, S.tA, S.tB);
   DCol cv1 = dcol(g, v0 + 1, p, S.base0, S.tA, S.tB);
   DCol cv2 = dcol(g, v0 + 2, p, S.base0, S.tA, S.tB);
-  // prologue: OWN(v0) arrives through NEXT, NEXT(v0+1) through LD
-  load_seg(NEXT, S.u, cv, 0, cv.valid, c0, Lw + 1, L, S.tC);
-  load_seg(LD, S.u, cv1, 0, cv1.valid, c0, Lw + 1, L, S.tC);
-  load_seg(LF, S.f, cv, 0, cv.valid, c0, Lw, L, S.tC);
+  // prologue: plane x = old(v0); in flight: old(v0+1), f(v0), edges(v0)
+  seg_issue(FD, S.u, seg_of(cv), cnt, p);
+  NEXTX = bload(S.u, x_off(cv), T(0));
+  seg_land(FD, px, cnt, p);
+  seg_issue(FD, S.u, seg_of(cv1), cnt, p);
+  LDX = bload(S.u, x_off(cv1), T(0));
+  seg_issue(FF, S.f, seg_of(cv), cnt, p);
   load_edge(LE, cv);
-#pragma unroll
-  for (int e = 0; e < kLW; ++e) {
-    OWN[e] = T(0);
-    PREV[e] = T(0);
-  }
-  OWN[kLW] = T(0);
 
   const int steps = g.V + kDW - 1;
   for (int t = 0; t < steps; ++t) {
     const int v = v0 + t;
-    // rotate the register pipeline by one diagonal
+    const bool active = v >= 0 && v < g.V;
+    // rotate by one diagonal: OWN <- old(v) from plane x, which then
+    // receives old(v+1); plane f receives f(v)
+    if (active && snap) seg_store<T>(px, S.snap, seg_of(cv), cnt, p);
 #pragma unroll
-    for (int e = 0; e < kLW; ++e) {
-      PREV[e] = OWN[e];
-      OWN[e] = NEXT[e];
-      NEXT[e] = LD[e];
-      F[e] = LF[e];
-      EDGE[e] = LE[e];
+    for (int j = 0; j < kLW; ++j)
+      if (j < cnt) OWN[j] = px[row(j) * P::RS + p + 1];
+    OWNX = NEXTX;
+    NEXTX = LDX;
+    seg_land(FD, px, cnt, p);
+    seg_land(FF, pf, cnt, p);
+    // band-edge neighbours of v at lanes -1 (plane n) and H (plane x)
+    if (lane_first) {
+#pragma unroll
+      for (int j = 0; j < kLW; ++j)
+        if (j < cnt) pn[row(j) * P::RS] = LE[j];
     }
-    OWN[kLW] = NEXT[kLW];
-    NEXT[kLW] = LD[kLW];
+    if (lane_last) {
+#pragma unroll
+      for (int j = 0; j < kLW; ++j)
+        if (j < cnt) px[row(j) * P::RS + g.H + 1] = LE[j];
+    }
     // loads for the next step: old values of v+2, slowness and edges of v+1
-    load_seg(LD, S.u, cv2, 0, cv2.valid, c0, Lw + 1, L, S.tC);
-    load_seg(LF, S.f, cv1, 0, cv1.valid, c0, Lw, L, S.tC);
+    seg_issue(FD, S.u, seg_of(cv2), cnt, p);
+    LDX = bload(S.u, x_off(cv2), T(0));
+    seg_issue(FF, S.f, seg_of(cv1), cnt, p);
     load_edge(LE, cv1);
 
-    if (v >= 0 && v < g.V) {
-      if (snap) store_seg(OWN, S.snap, cv, c0, Lw, L, S.tC);
+    if (active) {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : T(0);
       const int ra = S.flipA ? g.Ma - 1 - cv.a : cv.a;
       const int rb = S.flipB ? g.Nb - 1 - cv.b : cv.b;
@@ -216,30 +359,34 @@ __device__ __forceinline__ void diag_sweep(const DSweep<T>& S_in, bool snap, lpt
       T xo = T(0);
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
-        if (e < Lw) {
+        if (e < cnt) {
           const int c = c0 + e;
-          // cross-lane moves run with the full wave active
-          const T sl = lane_shift<kDppWaveShr1>(PREV[e]);
-          const T sr = lane_shift<kDppWaveShl1>(NEXT[e]);
-          const T ua = lane_first ? EDGE[e] : sl;
-          const T da = lane_last ? EDGE[e] : sr;
-          const T ub = PREV[e], db = NEXT[e];
+          const int rw = row(e) * P::RS;
+          const T ua = pn[rw + p];      // lane p-1, new
+          const T ub = pn[rw + p + 1];  // lane p, new
+          const T da = px[rw + p + 2];  // lane p+1, old
+          const T db = px[rw + p + 1];  // lane p, old
+          const T fe = pf[rw + p + 1];
           const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
-          const T dc = OWN[e + 1];
-          if (cv.valid && c < L) {
+          const T dc = (e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : OWNX;
+          if (cv.valid) {
             const int rc = S.flipC ? L - 1 - c : c;
             const T ma = axis_min(ua, da, S.flipA, af, al);
             const T mb = axis_min(ub, db, S.flipB, bf, bl);
             const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == L - 1);
             const T ux = g.a_is_i ? ma : mb;
             const T uy = g.a_is_i ? mb : ma;
-            const T cand = solution3d(ux, uy, mc, F[e], S.h);
+            const T cand = solution3d(ux, uy, mc, fe, S.h);
             OWN[e] = smin(cand, OWN[e]);
           }
-          if (e == Lw - 1) xo = OWN[e];
+          if (e == cnt - 1) xo = OWN[e];
         }
       }
-      store_seg(OWN, S.u, cv, c0, Lw, L, S.tC);
+      // plane n: new values of v (all reads of v-1 are done)
+#pragma unroll
+      for (int j = 0; j < kLW; ++j)
+        if (j < cnt) pn[row(j) * P::RS + p + 1] = OWN[j];
+      seg_store<T>(pn, S.u, seg_of(cv), cnt, p);
       xch[(t & 1) * kDW * 64 + w * 64 + p] = xo;
     }
     cv = cv1;
@@ -253,6 +400,7 @@ template <typename T>
 __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __restrict__ evs,
                                                              int max_iter) {
   __shared__ T xch[2 * kDW * 64];
+  __shared__ T planes[kDW * 3 * Pieces<T>::PLANE];
   __shared__ uint64_t red[kDW];
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
@@ -286,7 +434,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
       S.tC = S.flipC ? -1 : 1;
       S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
-      diag_sweep<T>(S, sw == 0 && it > 0, (lptr<T>)xch);
+      diag_sweep<T>(S, sw == 0 && it > 0, (lptr<T>)xch, (lptr<T>)planes);
     }
     // err = max |u - u_old| over the event (coalesced pass after the 8 sweeps)
     uint64_t key = 0;

@@ -37,10 +37,10 @@
 // instructions are never skipped: masked lanes use an out-of-range buffer
 // offset.
 //
-// Iteration (Eikonal3D.cpp:71-88): u = u0 is a copy pass at the start; the
-// snapshot u_old is written during the first sweep of each later
-// iteration; err = max|u - u_old| is a coalesced pass after the eighth sweep
-// against the snapshot (u0 for the first iteration).
+// Iteration (Eikonal3D.cpp:71-88): u = u0 is a copy pass at the start;
+// err = max|u - u_old| is a coalesced pass after the eighth sweep against
+// the snapshot (u0 for the first iteration), which also writes the next
+// snapshot u_old = u.
 #include "buffer_ops.hpp"
 #include "eikonal_device.hpp"
 #include "kernels.hpp"
@@ -104,7 +104,6 @@ struct DSweep {
   T h;
   rsrc_t u;     // travel times (read old / write new in place)
   rsrc_t f;
-  rsrc_t snap;  // snapshot written during the first sweep of iteration >= 1
 };
 
 // cnt values of column `col` from canonical c0 on (element j <- c0 + j),
@@ -122,18 +121,19 @@ __device__ __forceinline__ void load_seg(T (&buf)[NE], rsrc_t r, const DCol& col
 // ---- per-wave LDS planes and coalesced segment transfers ------------------
 //
 // Each wave keeps three planes in LDS, each holding one value per (node of
-// the segment, column of the wave): element m (memory order) of lane q's
-// segment sits at row m, position q + 1.  Position 0 stands for lane -1 and
-// position H + 1 for lane H: the band-edge neighbours of lanes 0 and H-1 are
-// written there, so every lane reads its a-neighbours at positions p and
-// p + 2 without exceptions.  Rows are padded so the piece writes spread over
-// the banks.
+// the segment, column of the wave): node j (canonical order along the
+// sweep) of lane q's segment sits at row j, position q + 1.  Position 0
+// stands for lane -1 and position H + 1 for lane H: the band-edge
+// neighbours of lanes 0 and H-1 are written there, so every lane reads its
+// a-neighbours at positions p and p + 2 without exceptions.  Row kLW is
+// scratch for writes that have no row.  Rows are padded so the piece writes
+// spread over the banks.
 //
 // Segments move between memory and a plane in 16-byte pieces: lane L of
 // transfer i handles piece L % LPC of the segment of lane i*CPI + L / LPC,
 // so the LPC lanes of one segment read or write one contiguous run.  A
 // segment whose length is not a multiple of the piece is finished by its
-// own lane, element by element.
+// own lane, element by element.  FULL: every segment has kLW nodes.
 
 using u32x4 = __attribute__((ext_vector_type(4))) unsigned int;
 
@@ -143,7 +143,7 @@ struct Pieces {
   static constexpr int LPC = kLW / EP;                // lanes per segment
   static constexpr int CPI = 64 / LPC;                // segments per transfer
   static constexpr int RS = 64 + 32 / (int)sizeof(T);  // plane row stride (elements)
-  static constexpr int PLANE = kLW * RS;              // plane size (elements)
+  static constexpr int PLANE = (kLW + 1) * RS;        // kLW rows + a scratch row
 };
 
 __device__ __forceinline__ u32x4 bload4(rsrc_t r, uint32_t off) {
@@ -194,83 +194,113 @@ struct Flight {
   T tail[Pieces<T>::EP - 1];
 };
 
+// Canonical row of memory element m of a segment of cnt nodes.
+template <bool FULL>
+__device__ __forceinline__ int canon(int m, int cnt, bool fwd) {
+  return fwd ? m : (FULL ? kLW : cnt) - 1 - m;
+}
+
 // Issues the loads of every lane's segment: `seg` = memory index of the
 // lowest node of the lane's segment (-1: none), cnt = nodes per segment.
-template <typename T>
+template <typename T, bool FULL>
 __device__ __forceinline__ void seg_issue(Flight<T>& fl, rsrc_t r, int seg, int cnt, int p) {
   using P = Pieces<T>;
+  if (FULL) cnt = kLW;
 #pragma unroll
   for (int i = 0; i < P::LPC; ++i) {
     const int q = i * P::CPI + p / P::LPC;
     const int m0 = (p % P::LPC) * P::EP;
     const int sq = __shfl(seg, q);
-    const bool ok = sq >= 0 && m0 + P::EP <= cnt;
+    const bool ok = sq >= 0 && (FULL || m0 + P::EP <= cnt);
     fl.pc[i] = bload4(r, ok ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB);
   }
-  const int nfull = (cnt / P::EP) * P::EP;
+  if (!FULL) {
+    const int nfull = (cnt / P::EP) * P::EP;
 #pragma unroll
-  for (int t = 0; t < P::EP - 1; ++t) {
-    const int m = nfull + t;
-    fl.tail[t] = bload(r, boff<T>(seg >= 0 && m < cnt, (int64_t)seg + m), T(0));
+    for (int t = 0; t < P::EP - 1; ++t) {
+      const int m = nfull + t;
+      fl.tail[t] = bload(r, boff<T>(seg >= 0 && m < cnt, (int64_t)seg + m), T(0));
+    }
   }
 }
 
-// Writes arrived loads into a plane.
-template <typename T>
-__device__ __forceinline__ void seg_land(const Flight<T>& fl, lptr<T> pl, int cnt, int p) {
+// Writes arrived loads into a plane (canonical rows).
+template <typename T, bool FULL>
+__device__ __forceinline__ void seg_land(const Flight<T>& fl, lptr<T> pl, int cnt, bool fwd,
+                                         int p) {
   using P = Pieces<T>;
 #pragma unroll
   for (int i = 0; i < P::LPC; ++i) {
     const int q = i * P::CPI + p / P::LPC;
     const int m0 = (p % P::LPC) * P::EP;
 #pragma unroll
-    for (int e = 0; e < P::EP; ++e) pl[(m0 + e) * P::RS + q + 1] = piece_get<T>(fl.pc[i], e);
+    for (int e = 0; e < P::EP; ++e) {
+      const int m = m0 + e;
+      const int j = (FULL || m < cnt) ? canon<FULL>(m, cnt, fwd) : kLW;
+      pl[j * P::RS + q + 1] = piece_get<T>(fl.pc[i], e);
+    }
   }
-  const int nfull = (cnt / P::EP) * P::EP;
+  if (!FULL) {
+    const int nfull = (cnt / P::EP) * P::EP;
 #pragma unroll
-  for (int t = 0; t < P::EP - 1; ++t)
-    if (nfull + t < cnt) pl[(nfull + t) * P::RS + p + 1] = fl.tail[t];
+    for (int t = 0; t < P::EP - 1; ++t) {
+      const int m = nfull + t;
+      const int j = m < cnt ? canon<FULL>(m, cnt, fwd) : kLW;
+      pl[j * P::RS + p + 1] = fl.tail[t];
+    }
+  }
 }
 
-// Stores every lane's segment from a plane.
-template <typename T>
-__device__ __forceinline__ void seg_store(lptr<const T> pl, rsrc_t r, int seg, int cnt, int p) {
+// Stores every lane's segment from a plane (canonical rows).
+template <typename T, bool FULL>
+__device__ __forceinline__ void seg_store(lptr<const T> pl, rsrc_t r, int seg, int cnt, bool fwd,
+                                          int p) {
   using P = Pieces<T>;
+  if (FULL) cnt = kLW;
 #pragma unroll
   for (int i = 0; i < P::LPC; ++i) {
     const int q = i * P::CPI + p / P::LPC;
     const int m0 = (p % P::LPC) * P::EP;
     const int sq = __shfl(seg, q);
-    const bool ok = sq >= 0 && m0 + P::EP <= cnt;
+    const bool ok = sq >= 0 && (FULL || m0 + P::EP <= cnt);
     T x[P::EP];
 #pragma unroll
-    for (int e = 0; e < P::EP; ++e) x[e] = pl[(m0 + e) * P::RS + q + 1];
+    for (int e = 0; e < P::EP; ++e) {
+      const int m = m0 + e;
+      const int j = (FULL || m < cnt) ? canon<FULL>(m, cnt, fwd) : kLW;
+      x[e] = pl[j * P::RS + q + 1];
+    }
     bstore4(r, ok ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB, piece_make<T>(x));
   }
-  const int nfull = (cnt / P::EP) * P::EP;
+  if (!FULL) {
+    const int nfull = (cnt / P::EP) * P::EP;
 #pragma unroll
-  for (int t = 0; t < P::EP - 1; ++t) {
-    const int m = nfull + t;
-    if (m < cnt) bstore(r, boff<T>(seg >= 0, (int64_t)seg + m), pl[m * P::RS + p + 1]);
+    for (int t = 0; t < P::EP - 1; ++t) {
+      const int m = nfull + t;
+      const int j = m < cnt ? canon<FULL>(m, cnt, fwd) : kLW;
+      bstore(r, boff<T>(seg >= 0 && m < cnt, (int64_t)seg + m), pl[j * P::RS + p + 1]);
+    }
   }
 }
 
 }  // namespace
 
-// One sweep of one event by the whole workgroup.  `snap`: also store every
-// node's old value to the snapshot.
-template <typename T>
-__device__ __forceinline__ void diag_sweep(const DSweep<T>& S_in, bool snap, lptr<T> xch,
-                                           lptr<T> planes) {
+// One sweep of one event by the whole workgroup (the steps of wave w).
+//
+// Every global memory instruction of the step loop is issued on every step
+// (inactive lanes and steps use out-of-range offsets), and a step's new
+// values are stored at the start of the next step, before any load is
+// waited for.  So the compiler's wait counts stay exact on every path into
+// the loop body: landing a load never waits for a store issued after it.
+template <typename T, bool FULL>
+__device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
+                                                lptr<T> xch, lptr<T> planes) {
   using P = Pieces<T>;
-  // Local copy: keeps the fields in registers.
-  const DSweep<T> S = S_in;
   const DGeom& g = S.g;
-  const int w = threadIdx.x >> 6, p = threadIdx.x & 63;
+  const int p = threadIdx.x & 63;
   const int L = g.L, Lw = g.Lw;
-  const int c0 = w * Lw;  // canonical start of this wave's segment
-  int cnt = L - c0;       // nodes in this wave's segment (uniform)
-  cnt = cnt < 0 ? 0 : (cnt > Lw ? Lw : cnt);
+  if (FULL) cnt = kLW;
+  const int c0 = w * Lw;      // canonical start of this wave's segment
   const bool fwd = !S.flipC;  // the segment runs forward in memory
   const bool lane_first = (p == 0);
   const bool lane_last = (p == g.H - 1);
@@ -291,13 +321,12 @@ __device__ __forceinline__ void diag_sweep(const DSweep<T>& S_in, bool snap, lpt
     return boff<T>(col.valid && has_x, col.node0 + (int64_t)(c0 + Lw) * S.tC);
   };
   // Band-edge neighbour of lane 0 ((a-1, b), new) and lane H-1 ((a+1, b),
-  // old) of column `col`, one load per node.
+  // old) of column `col`, one load per node (canonical order).
   auto load_edge = [&](T(&buf)[kLW], const DCol& col) {
     const bool eu = lane_first && col.valid && col.a > 0;
     const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
     load_seg(buf, S.u, col, eu ? -S.tA : S.tA, eu || ed, c0, cnt, L, S.tC);
   };
-  auto row = [&](int j) { return fwd ? j : cnt - 1 - j; };
 
   // Registers: OWN = old values of diagonal v in, new values out; OWNX /
   // NEXTX = old value one past the segment of v / v+1.  FD, FF, LE, LDX are
@@ -311,64 +340,66 @@ __device__ __forceinline__ void diag_sweep(const DSweep<T>& S_in, bool snap, lpt
   DCol cv1 = dcol(g, v0 + 1, p, S.base0, S.tA, S.tB);
   DCol cv2 = dcol(g, v0 + 2, p, S.base0, S.tA, S.tB);
   // prologue: plane x = old(v0); in flight: old(v0+1), f(v0), edges(v0)
-  seg_issue(FD, S.u, seg_of(cv), cnt, p);
+  seg_issue<T, FULL>(FD, S.u, seg_of(cv), cnt, p);
   NEXTX = bload(S.u, x_off(cv), T(0));
-  seg_land(FD, px, cnt, p);
-  seg_issue(FD, S.u, seg_of(cv1), cnt, p);
+  seg_land<T, FULL>(FD, px, cnt, fwd, p);
+  seg_issue<T, FULL>(FD, S.u, seg_of(cv1), cnt, p);
   LDX = bload(S.u, x_off(cv1), T(0));
-  seg_issue(FF, S.f, seg_of(cv), cnt, p);
+  seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
   load_edge(LE, cv);
 
   const int steps = g.V + kDW - 1;
+  int seg_prev = -1;  // segment of the previous step's new values (-1: none)
   for (int t = 0; t < steps; ++t) {
     const int v = v0 + t;
     const bool active = v >= 0 && v < g.V;
+    // new values of v-1 (plane n) to memory
+    seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
     // rotate by one diagonal: OWN <- old(v) from plane x, which then
     // receives old(v+1); plane f receives f(v)
-    if (active && snap) seg_store<T>(px, S.snap, seg_of(cv), cnt, p);
 #pragma unroll
     for (int j = 0; j < kLW; ++j)
-      if (j < cnt) OWN[j] = px[row(j) * P::RS + p + 1];
+      if (FULL || j < cnt) OWN[j] = px[j * P::RS + p + 1];
     OWNX = NEXTX;
     NEXTX = LDX;
-    seg_land(FD, px, cnt, p);
-    seg_land(FF, pf, cnt, p);
+    seg_land<T, FULL>(FD, px, cnt, fwd, p);
+    seg_land<T, FULL>(FF, pf, cnt, fwd, p);
     // band-edge neighbours of v at lanes -1 (plane n) and H (plane x)
     if (lane_first) {
 #pragma unroll
       for (int j = 0; j < kLW; ++j)
-        if (j < cnt) pn[row(j) * P::RS] = LE[j];
+        if (FULL || j < cnt) pn[j * P::RS] = LE[j];
     }
     if (lane_last) {
 #pragma unroll
       for (int j = 0; j < kLW; ++j)
-        if (j < cnt) px[row(j) * P::RS + g.H + 1] = LE[j];
+        if (FULL || j < cnt) px[j * P::RS + g.H + 1] = LE[j];
     }
     // loads for the next step: old values of v+2, slowness and edges of v+1
-    seg_issue(FD, S.u, seg_of(cv2), cnt, p);
+    seg_issue<T, FULL>(FD, S.u, seg_of(cv2), cnt, p);
     LDX = bload(S.u, x_off(cv2), T(0));
-    seg_issue(FF, S.f, seg_of(cv1), cnt, p);
+    seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
     load_edge(LE, cv1);
 
+    T xo = T(0);
     if (active) {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : T(0);
       const int ra = S.flipA ? g.Ma - 1 - cv.a : cv.a;
       const int rb = S.flipB ? g.Nb - 1 - cv.b : cv.b;
       const bool af = ra == 0, al = ra == g.Ma - 1;
       const bool bf = rb == 0, bl = rb == g.Nb - 1;
-      T xo = T(0);
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
-        if (e < cnt) {
+        if (FULL || e < cnt) {
           const int c = c0 + e;
-          const int rw = row(e) * P::RS;
+          const int rw = e * P::RS;
           const T ua = pn[rw + p];      // lane p-1, new
           const T ub = pn[rw + p + 1];  // lane p, new
           const T da = px[rw + p + 2];  // lane p+1, old
           const T db = px[rw + p + 1];  // lane p, old
           const T fe = pf[rw + p + 1];
           const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
-          const T dc = (e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : OWNX;
+          const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : OWNX;
           if (cv.valid) {
             const int rc = S.flipC ? L - 1 - c : c;
             const T ma = axis_min(ua, da, S.flipA, af, al);
@@ -379,21 +410,39 @@ __device__ __forceinline__ void diag_sweep(const DSweep<T>& S_in, bool snap, lpt
             const T cand = solution3d(ux, uy, mc, fe, S.h);
             OWN[e] = smin(cand, OWN[e]);
           }
-          if (e == cnt - 1) xo = OWN[e];
+          if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
         }
       }
-      // plane n: new values of v (all reads of v-1 are done)
-#pragma unroll
-      for (int j = 0; j < kLW; ++j)
-        if (j < cnt) pn[row(j) * P::RS + p + 1] = OWN[j];
-      seg_store<T>(pn, S.u, seg_of(cv), cnt, p);
-      xch[(t & 1) * kDW * 64 + w * 64 + p] = xo;
     }
+    // plane n: new values of v (all reads of v-1 are done)
+#pragma unroll
+    for (int j = 0; j < kLW; ++j)
+      if (FULL || j < cnt) pn[j * P::RS + p + 1] = OWN[j];
+    seg_prev = active ? seg_of(cv) : -1;
+    xch[(t & 1) * kDW * 64 + w * 64 + p] = xo;
     cv = cv1;
     cv1 = cv2;
     cv2 = dcol(g, v + 3, p, S.base0, S.tA, S.tB);
     __syncthreads();
   }
+  seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
+  __syncthreads();  // the next sweep reads these values
+}
+
+// One sweep of one event by the whole workgroup.  Waves whose segment has
+// kLW nodes take the fixed-length variant; the others (and idle waves) the
+// general one.  Both run the same number of steps and barriers.
+template <typename T>
+__device__ __forceinline__ void diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> planes) {
+  // the wave index is uniform across the wave: keep it (and everything
+  // derived from it) in scalar registers
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
+  cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
+  if (cnt == kLW)
+    diag_sweep_wave<T, true>(S, w, cnt, xch, planes);
+  else
+    diag_sweep_wave<T, false>(S, w, cnt, xch, planes);
 }
 
 template <typename T>
@@ -418,7 +467,6 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   S.h = (T)ev.h;
   S.u = make_rsrc(ev.u, bytes);
   S.f = make_rsrc(ev.f, bytes);
-  S.snap = make_rsrc(ev.uold, bytes);
   const T tol = (T)ev.tol;
   int it = 0;
   T err = T(0);
@@ -434,16 +482,20 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
       S.tC = S.flipC ? -1 : 1;
       S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
-      diag_sweep<T>(S, sw == 0 && it > 0, (lptr<T>)xch, (lptr<T>)planes);
+      diag_sweep<T>(S, (lptr<T>)xch, (lptr<T>)planes);
     }
-    // err = max |u - u_old| over the event (coalesced pass after the 8 sweeps)
+    // err = max |u - u_old| over the event (coalesced pass after the 8
+    // sweeps), which also leaves u_old = u for the next iteration
     uint64_t key = 0;
     {
       const gptr<const T> ua = (gptr<const T>)ev.u;
       const gptr<const T> sr = (gptr<const T>)((it == 0) ? ev.u0 : ev.uold);
+      const gptr<T> so = (gptr<T>)ev.uold;
       for (int64_t q = threadIdx.x; q < N; q += blockDim.x) {
-        const uint64_t kq = maxkey(Real<T>::fabs_(ua[q] - sr[q]));
+        const T uq = ua[q];
+        const uint64_t kq = maxkey(Real<T>::fabs_(uq - sr[q]));
         key = kq > key ? kq : key;
+        so[q] = uq;
       }
     }
     for (int off = 32; off > 0; off >>= 1) {

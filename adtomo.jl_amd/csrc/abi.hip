@@ -184,9 +184,9 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
     return rc;
   DevBuf ws;
   const size_t a = ((4 * bytes + 255) / 256) * 256;  // per-node records (4 values)
-  const size_t c = 0;
+  const size_t c = ((size_t(N) * 2 + 255) / 256) * 256;  // 16-bit node states
   const size_t q = ((size_t(N) * 4 + 255) / 256) * 256;
-  if ((rc = ws.alloc(a + 2 * c + q + 256 + sizeof(adt_info), s))) return rc;
+  if ((rc = ws.alloc(a + c + q + 256 + sizeof(adt_info), s))) return rc;
   char* base = (char*)ws.p;
   adt::EventDesc d{};
   d.m = (int)m;
@@ -198,12 +198,12 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   d.grad_u = dg;
   d.u = (void*)du;
   d.lam = base;
-  d.code = nullptr;
+  d.code = (uint8_t*)(base + a);
   d.indeg = nullptr;
-  d.queue = (int32_t*)(base + a + 2 * c);
+  d.queue = (int32_t*)(base + a + c);
   d.grad_u0 = dgu0;
   d.grad_f = dgf;
-  const size_t od = a + 2 * c + q;
+  const size_t od = a + c + q;
   d.info = (adt_info*)(base + od + 256);
   ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
   adt::adj3_kahn_kernel<T><<<1, kWG, 0, s>>>((const adt::EventDesc*)(base + od));
@@ -353,7 +353,7 @@ struct BatchPlan {
       if (x.l > max_l) max_l = (int)x.l;
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
       o_lam[e] = (flags & ADT_ADJOINT) ? take(4 * N * sizeof(T)) : 0;  // per-node records
-      o_code[e] = 0;
+      o_code[e] = (flags & ADT_ADJOINT) ? take(2 * N) : 0;              // 16-bit node states
       o_indeg[e] = 0;
       o_queue[e] = (flags & ADT_ADJOINT) ? take(4 * N) : 0;
     }
@@ -389,7 +389,7 @@ struct BatchPlan {
     for (auto& d : descs) {
       d.uold = base + (size_t)d.uold;
       d.lam = base + (size_t)d.lam;
-      d.code = nullptr;
+      d.code = (uint8_t*)(base + (size_t)d.code);
       d.indeg = nullptr;
       d.queue = (int32_t*)(base + (size_t)d.queue);
       d.info = (adt_info*)(base + (size_t)d.info);

@@ -29,27 +29,35 @@ __device__ __forceinline__ unsigned dec_byte(uint8_t* base, int64_t idx) {
   return (old >> sh) & 0xffu;
 }
 
-// Per-node record of the 3D traversal: everything a node reads from a
-// neighbour (u, lambda, upwind code) sits in one cache line.
-//   meta bits 0-7: upwind code (see adjoint_device.hpp), bits 8-15: number
-//   of feeding neighbours not yet final.
-template <typename T> struct Rec;
-template <> struct alignas(32) Rec<double> {
-  double u;
-  double lam;
-  uint32_t meta;
-  uint32_t pad;
-  double g;
-};
-template <> struct alignas(16) Rec<float> {
-  float u;
-  float lam;
-  uint32_t meta;
-  float g;
+// ----------------------------------------------------------------- 3D ---
+//
+// Per-node state of the 3D traversal (built by a coalesced prep pass):
+//   rec[p] = {lambda, w_x, w_y, w_z}: lambda starts as g (0 on Z) and is
+//            replaced by the solution; w_d = 2 (u_p - u_parent_d) is p's
+//            coefficient toward its upwind neighbour along an active
+//            dimension d (0 otherwise).  A node r resolves as
+//              lambda_r = (g_r + sum_{feeders c} w_{c,d} lambda_c) / D_r,
+//              D_r = w_x + w_y + w_z over r's active dimensions,
+//            reading only its own record and its feeders' records.
+//   meta[p] (16 bits): bits 0-5 upwind code (2 bits per dimension: 0
+//            inactive, 1 argmin at -1, 2 argmin at +1), bit 6 Z, bits 7-12
+//            feeder mask in neighbour order (i-1, i+1, j-1, j+1, k-1, k+1),
+//            bits 13-15 number of feeders not yet final.  Z nodes start
+//            with 7, so releases (at most 6) never make them ready.
+// Releasing a neighbour is one 32-bit atomic on the word holding its meta.
+template <typename T>
+struct alignas(4 * sizeof(T)) Rec {
+  T lam;
+  T w[3];
 };
 
-// Upwind code of node p (Eikonal3D.cpp:105-171): 0x40 for Z, else the
-// argmin side of every active dimension.
+constexpr uint32_t kCntShift = 13;
+constexpr uint32_t kZ = 0x40u;
+
+__device__ __forceinline__ uint32_t meta_shift(int64_t p) { return (p & 1) ? 16u : 0u; }
+
+// Upwind code of node p (Eikonal3D.cpp:105-171): kZ for Z, else the argmin
+// side of every active dimension.
 template <typename T>
 __device__ __forceinline__ uint32_t code3(gptr<const T> u, gptr<const T> u0, int m, int n, int l,
                                          int64_t p) {
@@ -58,7 +66,7 @@ __device__ __forceinline__ uint32_t code3(gptr<const T> u, gptr<const T> u0, int
   const int ex[3] = {m, n, l};
   const int64_t st[3] = {nl, l, 1};
   const T up = u[p];
-  if (up == u0[p]) return 0x40u;
+  if (up == u0[p]) return kZ;
   uint32_t cd = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -77,87 +85,74 @@ __device__ __forceinline__ uint32_t code3(gptr<const T> u, gptr<const T> u0, int
     }
     if (up > umin) cd |= side << (2 * d);
   }
-  return cd ? cd : 0x40u;
+  return cd ? cd : kZ;
 }
 
-// Values node r reads to resolve itself: its own record and its six
-// neighbours' (u, lambda, code).  Loaded for several nodes before any is
-// resolved, so the loads overlap.
+// Values node r reads to resolve itself, loaded for several nodes before
+// any is resolved so that the loads overlap.
 template <typename T>
 struct Gathered {
   int64_t r;
   bool valid;
-  int co[3];
-  T ur, g;
-  uint32_t cr;
-  T nu[6], nlam[6];
-  uint32_t nmeta[6];
+  uint32_t mr;
+  Rec<T> own;
+  T flam[6], fw[6];
 };
 
 template <typename T>
-__device__ __forceinline__ void gather3(gptr<const Rec<T>> rec, int m, int n, int l, int64_t r,
-                                        bool valid, Gathered<T>& G) {
+__device__ __forceinline__ void gather3(gptr<const Rec<T>> rec, gptr<const uint16_t> meta, int n,
+                                        int l, int64_t r, bool valid, Gathered<T>& G) {
   const int64_t nl = (int64_t)n * l;
+  const int64_t st[3] = {nl, l, 1};
   G.r = valid ? r : 0;
   G.valid = valid;
   r = G.r;
-  G.co[0] = (int)(r / nl);
-  G.co[1] = (int)((r / l) % n);
-  G.co[2] = (int)(r % l);
-  const int ex[3] = {m, n, l};
-  const int64_t st[3] = {nl, l, 1};
-  G.ur = rec[r].u;
-  G.g = rec[r].g;
-  G.cr = rec[r].meta & 0xffu;
+  G.mr = valid ? (uint32_t)meta[r] : 0u;
+  G.own.lam = rec[r].lam;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) G.own.w[d] = rec[r].w[d];
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     const int d = q >> 1;
-    const bool plus = q & 1;
-    const bool in = plus ? (G.co[d] < ex[d] - 1) : (G.co[d] > 0);
-    const int64_t c = in ? (plus ? r + st[d] : r - st[d]) : r;
-    G.nu[q] = rec[c].u;
-    G.nlam[q] = rec[c].lam;
-    const uint32_t mc = rec[c].meta & 0xffu;
-    G.nmeta[q] = in ? mc : 0x40u;
+    const bool feeds = (G.mr >> (7 + q)) & 1u;
+    const int64_t c = feeds ? ((q & 1) ? r + st[d] : r - st[d]) : r;
+    G.flam[q] = feeds ? rec[c].lam : T(0);
+    G.fw[q] = feeds ? rec[c].w[d] : T(0);
   }
 }
 
-// Resolves a gathered node: lambda_r from its feeding neighbours (all
-// final) is stored, and the node's upwind neighbours are released: old[d]
-// receives the previous unresolved count of the upwind neighbour along d
-// (0 if none was released), tgt[d] its index.
+// Resolves a gathered node (Eikonal3D.cpp:149-196 restated as the
+// triangular solve): lambda_r is stored and the upwind neighbours are
+// released; old[d] receives the released neighbour's previous meta word
+// (0 if none), tgt[d] its index.
 template <typename T>
-__device__ __forceinline__ void resolve3(gptr<Rec<T>> rec, int n, int l, const Gathered<T>& G,
-                                         uint32_t (&old)[3], int64_t (&tgt)[3]) {
+__device__ __forceinline__ void resolve3(gptr<Rec<T>> rec, gptr<uint32_t> meta_w, int n, int l,
+                                         const Gathered<T>& G, uint32_t (&old)[3],
+                                         int64_t (&tgt)[3]) {
   const int64_t nl = (int64_t)n * l;
   const int64_t st[3] = {nl, l, 1};
   const int64_t r = G.r;
-  T num = G.g;
+  T num = G.own.lam;
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    if ((G.mr >> (7 + q)) & 1u) num = num + G.fw[q] * G.flam[q];
   T D = T(0);
   bool have = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    // neighbour at -1 feeds r if its argmin along d is at +1, and vice versa
-    if (!(G.nmeta[2 * d] & 0x40u) && ((G.nmeta[2 * d] >> (2 * d)) & 3u) == 2u)
-      num = num + (T(2.0) * (G.nu[2 * d] - G.ur)) * G.nlam[2 * d];
-    if (!(G.nmeta[2 * d + 1] & 0x40u) && ((G.nmeta[2 * d + 1] >> (2 * d)) & 3u) == 1u)
-      num = num + (T(2.0) * (G.nu[2 * d + 1] - G.ur)) * G.nlam[2 * d + 1];
-    const uint32_t sd = (G.cr >> (2 * d)) & 3u;
-    if (sd) {
-      const T t = T(2.0) * (G.ur - G.nu[sd == 1 ? 2 * d : 2 * d + 1]);
-      D = have ? D + t : t;
+    if ((G.mr >> (2 * d)) & 3u) {
+      D = have ? D + G.own.w[d] : G.own.w[d];
       have = true;
     }
   }
   if (G.valid) rec[r].lam = num / D;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const uint32_t sd = (G.cr >> (2 * d)) & 3u;
-    const int q = sd == 1 ? 2 * d : 2 * d + 1;
-    const bool rel = G.valid && sd && !(G.nmeta[q] & 0x40u);
+    const uint32_t sd = (G.mr >> (2 * d)) & 3u;
     tgt[d] = sd == 1 ? r - st[d] : r + st[d];
-    old[d] = rel ? __hip_atomic_fetch_sub(&rec[tgt[d]].meta, 1u << 8, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT)
+    const bool rel = G.valid && sd;
+    old[d] = rel ? __hip_atomic_fetch_sub(&meta_w[tgt[d] >> 1], 1u << (kCntShift + meta_shift(tgt[d])),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                  : 0u;
   }
 }
@@ -166,7 +161,8 @@ __device__ __forceinline__ void push_ready(const uint32_t (&old)[3], const int64
                                            gptr<int32_t> queue, int* qtail) {
 #pragma unroll
   for (int d = 0; d < 3; ++d)
-    if (((old[d] >> 8) & 0xffu) == 1u) queue[atomicAdd(qtail, 1)] = (int32_t)tgt[d];
+    if (((old[d] >> (kCntShift + meta_shift(tgt[d]))) & 7u) == 1u)
+      queue[atomicAdd(qtail, 1)] = (int32_t)tgt[d];
 }
 
 template <typename T>
@@ -183,41 +179,58 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
   const gptr<T> gu0 = (gptr<T>)ev.grad_u0;
   const gptr<int32_t> queue = (gptr<int32_t>)ev.queue;
   const gptr<Rec<T>> rec = (gptr<Rec<T>>)ev.lam;
+  const gptr<uint16_t> meta = (gptr<uint16_t>)ev.code;
+  const gptr<uint32_t> meta_w = (gptr<uint32_t>)ev.code;
   if (threadIdx.x == 0) {
     qtail = 0;
     nonz = 0;
   }
+  // pass 1: upwind codes; grad_u0 = g where u == u0 (Eikonal3D.cpp:105-110)
   for (int64_t p = threadIdx.x; p < N; p += blockDim.x) {
-    const T gp = g[p];
-    if (gu0) gu0[p] = (u[p] == u0[p]) ? gp : T(0);  // Eikonal3D.cpp:105-110
-    rec[p].u = u[p];
-    rec[p].lam = T(0);
-    rec[p].meta = code3<T>(u, u0, m, n, l, p);
-    rec[p].g = gp;
+    if (gu0) gu0[p] = (u[p] == u0[p]) ? g[p] : T(0);
+    meta[p] = (uint16_t)code3<T>(u, u0, m, n, l, p);
   }
   __syncthreads();
+  // pass 2: feeders, counters, coefficients; nodes without feeders start.
+  // Other threads rewrite neighbouring meta entries meanwhile, but only
+  // bits 7-15; the code bits read here never change.
   const int ex[3] = {m, n, l};
   const int64_t st[3] = {nl, l, 1};
   int my_nonz = 0;
   for (int64_t p = threadIdx.x; p < N; p += blockDim.x) {
-    const uint32_t cr = rec[p].meta;
-    if (cr & 0x40u) continue;
-    ++my_nonz;
+    const uint32_t cr = meta[p];
     const int co[3] = {(int)(p / nl), (int)((p / l) % n), (int)(p % l)};
-    uint32_t cnt = 0;
+    uint32_t fm = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (co[d] > 0) {
-        const uint32_t cc = rec[p - st[d]].meta;
-        cnt += (!(cc & 0x40u) && ((cc >> (2 * d)) & 3u) == 2u) ? 1u : 0u;
+        const uint32_t cc = meta[p - st[d]];
+        fm |= (!(cc & kZ) && ((cc >> (2 * d)) & 3u) == 2u) ? (1u << (2 * d)) : 0u;
       }
       if (co[d] < ex[d] - 1) {
-        const uint32_t cc = rec[p + st[d]].meta;
-        cnt += (!(cc & 0x40u) && ((cc >> (2 * d)) & 3u) == 1u) ? 1u : 0u;
+        const uint32_t cc = meta[p + st[d]];
+        fm |= (!(cc & kZ) && ((cc >> (2 * d)) & 3u) == 1u) ? (2u << (2 * d)) : 0u;
       }
     }
-    rec[p].meta = cr | (cnt << 8);
-    if (cnt == 0) queue[atomicAdd(&qtail, 1)] = (int32_t)p;
+    Rec<T> rp;
+    const T up = u[p];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t sd = (cr >> (2 * d)) & 3u;
+      // 2 (u_r - u_argmin): Eikonal3D.cpp:149-166
+      rp.w[d] = sd ? T(2.0) * (up - u[sd == 1 ? p - st[d] : p + st[d]]) : T(0);
+    }
+    const bool z = cr & kZ;
+    rp.lam = z ? T(0) : g[p];
+    rec[p].lam = rp.lam;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) rec[p].w[d] = rp.w[d];
+    const uint32_t cnt = z ? 7u : (uint32_t)__builtin_popcount(fm);
+    meta[p] = (uint16_t)((cr & 0x7fu) | (fm << 7) | (cnt << kCntShift));
+    if (!z) {
+      ++my_nonz;
+      if (cnt == 0) queue[atomicAdd(&qtail, 1)] = (int32_t)p;
+    }
   }
   atomicAdd(&nonz, my_nonz);
   __syncthreads();
@@ -230,12 +243,12 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
       const bool v0 = i0 < tail, v1 = i1 < tail;
       const int64_t r0 = queue[v0 ? i0 : head], r1 = queue[v1 ? i1 : head];
       Gathered<T> G0, G1;
-      gather3((gptr<const Rec<T>>)rec, m, n, l, r0, v0, G0);
-      gather3((gptr<const Rec<T>>)rec, m, n, l, r1, v1, G1);
+      gather3((gptr<const Rec<T>>)rec, (gptr<const uint16_t>)meta, n, l, r0, v0, G0);
+      gather3((gptr<const Rec<T>>)rec, (gptr<const uint16_t>)meta, n, l, r1, v1, G1);
       uint32_t o0[3], o1[3];
       int64_t t0[3], t1[3];
-      resolve3(rec, n, l, G0, o0, t0);
-      resolve3(rec, n, l, G1, o1, t1);
+      resolve3(rec, meta_w, n, l, G0, o0, t0);
+      resolve3(rec, meta_w, n, l, G1, o1, t1);
       push_ready(o0, t0, queue, &qtail);
       push_ready(o1, t1, queue, &qtail);
     }

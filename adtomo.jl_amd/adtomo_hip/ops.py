@@ -282,15 +282,17 @@ def eikonal3d_batched(u0s, fs, hs, tols, grad_us=None, forward=True, us=None,
 
 
 class EikonalGraph:
-    """hipGraph of the batched forward (+ adjoint) over fixed buffers.
+    """hipGraph of the batched forward and/or adjoint over fixed buffers.
 
     Build once, then ``launch()`` repeatedly; the event buffers (inputs and
     outputs, all kept alive by this object) are read/written in place."""
 
-    def __init__(self, u0s, fs, hs, tols, grad_us=None, forward=True):
+    def __init__(self, u0s, fs, hs, tols, grad_us=None, forward=True, us=None):
         torch = _torch()
         self.u0s, self.fs, self.grad_us = u0s, fs, grad_us
-        self.us = [torch.empty_like(f) for f in fs]
+        # travel times: written by the forward, read by the adjoint (pass the
+        # forward's buffers to build an adjoint-only graph)
+        self.us = list(us) if us is not None else [torch.empty_like(f) for f in fs]
         self.grad_u0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
         self.grad_fs = [torch.empty_like(f) for f in fs] if grad_us is not None else None
         self.nev = len(fs)
@@ -354,7 +356,8 @@ def eikonal3d_grad_plain(grad_u, u, u0, f, h):
 
 
 def selftest_arith(op: int, a, b=None):
-    """Runs one device arithmetic op of the path on torch device tensors."""
+    """Runs one device arithmetic op of the path on torch device tensors
+    (op 6: plain copy, the reference kernel for memory counters)."""
     torch = _torch()
     n = a.numel() // (3 if op == 4 else 2 if op == 5 else 1)
     out = torch.empty(n, dtype=a.dtype, device=a.device)

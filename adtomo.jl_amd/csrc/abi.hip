@@ -721,18 +721,20 @@ int adt_graph_destroy(adt_graph* g) {
 
 int adt_selftest_arith_f64(int32_t op, const double* a, const double* b, double* out, int64_t n,
                            void* stream) {
-  if (n < 0 || !a || !out) return fail(ADT_EINVAL, "selftest: bad arguments");
+  if (n < 0 || !a || !out || op < 0 || op > 6 || ((op == 3 || (op >= 4 && op <= 5)) && !b))
+    return fail(ADT_EINVAL, "selftest: bad arguments");
   hipStream_t s = (hipStream_t)stream;
-  adt::arith_kernel<double><<<256, 256, 0, s>>>(op, a, b, out, n);
+  adt::arith_kernel<double><<<op == 6 ? 4096 : 256, 256, 0, s>>>(op, a, b, out, n);
   ADT_HIP(hipGetLastError());
   ADT_HIP(hipStreamSynchronize(s));
   return ADT_OK;
 }
 int adt_selftest_arith_f32(int32_t op, const float* a, const float* b, float* out, int64_t n,
                            void* stream) {
-  if (n < 0 || !a || !out) return fail(ADT_EINVAL, "selftest: bad arguments");
+  if (n < 0 || !a || !out || op < 0 || op > 6 || ((op == 3 || (op >= 4 && op <= 5)) && !b))
+    return fail(ADT_EINVAL, "selftest: bad arguments");
   hipStream_t s = (hipStream_t)stream;
-  adt::arith_kernel<float><<<256, 256, 0, s>>>(op, a, b, out, n);
+  adt::arith_kernel<float><<<op == 6 ? 4096 : 256, 256, 0, s>>>(op, a, b, out, n);
   ADT_HIP(hipGetLastError());
   ADT_HIP(hipStreamSynchronize(s));
   return ADT_OK;

@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "adtomo.jl_amd")]
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the forward kernel on this workload (256 events),
+# measured with rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy kernel of
+# known size: profiles/pmc_traffic_fwd3_diag_adj3_kahn_256x128x128x64.md
+FWD_TRAFFIC_BYTES_256 = 144.4e9 + 58.6e9
 
 
 def log(*a):
@@ -37,7 +41,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nev", type=int, default=256)
     ap.add_argument("--plain-events", type=int, default=1)
-    ap.add_argument("--cpu-events", type=int, default=0, help="0 = one per host thread")
+    ap.add_argument("--cpu-events", type=int, default=0, help="0 = 20 per host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--fp32", action="store_true", help="also time the FP32 variant")
@@ -57,10 +61,11 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.calib_copy:
-        # 2^27 doubles = 1 GiB read + 1 GiB written by one copy kernel
+        # 2^27 doubles = 1 GiB read + 1 GiB written by the library's plain
+        # copy kernel (arith_kernel<double>, op 6)
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
+        torch.cuda.synchronize()
+        b = A.selftest_arith(6, a)
         torch.cuda.synchronize()
         del a, b
     shape = (128, 128, 64)
@@ -102,22 +107,26 @@ def main():
     value = nev / (ms / 1e3)
     log(f"batched fwd+adj: {ms:.2f} ms/step, {value:.1f} event-solves/s (wall {wall:.2f}s)")
 
-    # dominant kernel (batched forward) timed alone with events on its stream
-    fwd_ms = []
-    adj_ms = []
-    for rep in range(max(2, args.steps)):
-        a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        a0.record(s)
-        A.eikonal3d_batched(u0d, fs, [h] * nev, [tol] * nev, return_info=False, us=G.us)
-        a1.record(s)
-        A.eikonal3d_batched(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd, forward=False,
-                            us=G.us, return_info=False)
-        a2.record(s)
+    # the two kernels timed alone: forward-only and adjoint-only graphs on
+    # the same buffers, HIP events on the launch stream
+    Gf = A.EikonalGraph(u0d, fs, [h] * nev, [tol] * nev, us=G.us)
+    Ga = A.EikonalGraph(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd, forward=False, us=G.us)
+
+    def time_graph(graph, reps):
+        graph.launch()
         torch.cuda.synchronize()
-        fwd_ms.append(a0.elapsed_time(a1))
-        adj_ms.append(a1.elapsed_time(a2))
-    fwd_ms = float(np.mean(fwd_ms[1:]))
-    adj_ms = float(np.mean(adj_ms[1:]))
+        b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        b0.record(s)
+        for _ in range(reps):
+            graph.launch()
+        b1.record(s)
+        torch.cuda.synchronize()
+        return b0.elapsed_time(b1) / reps
+
+    fwd_ms = time_graph(Gf, max(2, args.steps))
+    adj_ms = time_graph(Ga, max(2, args.steps))
+    Gf.close()
+    Ga.close()
     s_el = 8
     fwd_bytes = float(np.sum(8 * Ks * N * (s_el + 2 * s_el)))
     adj_bytes = float(nev * N * 6 * s_el)
@@ -145,9 +154,8 @@ def main():
     if not args.no_cpu:
         from oracle import oracle
         threads = min(16, os.cpu_count() or 1)
-        ce = args.cpu_events or threads
-        sub_u0 = [host_np for host_np in
-                  (synth.point_source_u0(shape, srcs[e]) for e in range(ce))]
+        ce = args.cpu_events or 20 * threads  # about 10 s on the host
+        sub_u0 = [synth.point_source_u0(shape, srcs[e % nev]) for e in range(ce)]
         sub_g = [synth.sparse_adjoint_source(shape, 60, seed=2024 * 1000 + e) for e in range(ce)]
         tc = time.perf_counter()
         oracle.eikonal3d_batch(sub_u0, [f] * ce, sub_g, [h] * ce, tol, threads)
@@ -174,9 +182,10 @@ def main():
                                "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
                    "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
                    "adj_sweeps_mean": float(sweeps.mean())},
-        "roofline": {"bound": "hbm", "kernel": "fwd3_event_kernel<double>",
+        "roofline": {"bound": "hbm", "kernel": "fwd3_diag_kernel<double>",
                      "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(fwd_gbs / HBM_PEAK_GBS, 5), "traffic": None},
+                     "frac": round(fwd_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": FWD_TRAFFIC_BYTES_256 if nev == 256 else None},
         "cpu_baseline": cpu,
         "plain_event_solves_per_s": plain,
         "batched_vs_plain": (value / plain) if plain else None,

@@ -151,7 +151,9 @@ int adt_graph_destroy(adt_graph *g);
 /* ---- Diagnostics: device arithmetic of the path, for bit-exact checks
  * against host IEEE results.  op 0: sqrt(a); 1: a/3; 2: a/2; 3: a/b;
  * 4: 3D local solve (a = 3 values, b = f,h per element);
- * 5: 2D local solve (a = 2 values, b = f,h per element). */
+ * 5: 2D local solve (a = 2 values, b = f,h per element);
+ * 6: plain copy out = a (a streaming kernel of known traffic, n elements
+ *    read and written once, for calibrating memory counters). */
 int adt_selftest_arith_f64(int32_t op, const double *a, const double *b, double *out,
                            int64_t n, void *stream);
 int adt_selftest_arith_f32(int32_t op, const float *a, const float *b, float *out, int64_t n,

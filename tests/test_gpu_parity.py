@@ -61,18 +61,34 @@ def test_device_arith_is_ieee():
         np.nextafter(np.arange(1, 2000, dtype=np.float64) ** 2, 0),
         np.nextafter(np.arange(1, 2000, dtype=np.float64) ** 2, np.inf)])
     x = np.concatenate([x, -x[:1000]])
+    # the range edges of the fast square root and division by 3, signed
+    # zeros, NaN, and random bit patterns over every exponent
+    edges = np.array([2.0 ** -767, 2.0 ** -1000, 2.0 ** 1000, 2.0 ** 1023, 1.7976931348623157e308,
+                      -0.0, np.nan, -np.inf, 3.0 * 2.0 ** -1000, 3.0 * 2.0 ** 999])
+    edges = np.concatenate([edges, np.nextafter(edges, np.inf), np.nextafter(edges, -np.inf)])
+    bits = r.integers(0, 2 ** 63, 400000, dtype=np.int64).view(np.float64)
+    x = np.concatenate([x, edges, -edges, bits, -bits])
     for op, ref in ((0, np.sqrt), (1, lambda v: v / 3.0), (2, lambda v: v / 2.0)):
         with np.errstate(invalid="ignore"):
             want = ref(x)
         got = host(A.selftest_arith(op, dev(x)))
         assert bits_equal(got, want), op
+    x = x[np.isfinite(x)]
     y = r.uniform(0.1, 100, x.size)
     got = host(A.selftest_arith(3, dev(x), dev(y)))
     assert bits_equal(got, x / y)
-    x32 = x.astype(np.float32)
     with np.errstate(invalid="ignore", over="ignore"):
+        x32 = np.concatenate([x.astype(np.float32),
+                              r.integers(0, 2 ** 31, 400000, dtype=np.int32).view(np.float32)])
+        x32 = np.concatenate([x32, -x32])
         assert bits_equal(host(A.selftest_arith(0, dev(x32))), np.sqrt(x32))
         assert bits_equal(host(A.selftest_arith(1, dev(x32))), x32 / np.float32(3))
+
+
+def test_selftest_copy():
+    import adtomo_hip as A
+    x = synth.rng(9).standard_normal(1 << 20)
+    assert bits_equal(host(A.selftest_arith(6, dev(x))), x)
 
 
 def test_device_local_solvers_match_numpy():

@@ -283,6 +283,34 @@ __device__ __forceinline__ void seg_store(lptr<const T> pl, rsrc_t r, int seg, i
   }
 }
 
+// Band-edge segments of a full-length wave: lane 0's (a-1, b) column and
+// lane H-1's (a+1, b) column, both kLW nodes, moved as 2 x LPC pieces by
+// lanes 0 .. 2*LPC-1 of one transfer.  `eseg` = memory index of the lowest
+// node of the lane's band-edge segment (lanes 0 and H-1; -1: none).
+template <typename T>
+__device__ __forceinline__ u32x4 edge_issue(rsrc_t r, int eseg, int hm1, int p) {
+  using P = Pieces<T>;
+  const int src = p < P::LPC ? 0 : hm1;
+  const int sq = __shfl(eseg, src);
+  const int m0 = (p % P::LPC) * P::EP;
+  const bool ok = p < 2 * P::LPC && sq >= 0;
+  return bload4(r, ok ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB);
+}
+
+// Writes the band-edge pieces: lane 0's segment to plane n position 0,
+// lane H-1's to plane x position H + 1 (canonical rows).
+template <typename T>
+__device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> px, int H,
+                                          bool fwd, int p) {
+  using P = Pieces<T>;
+  if (p < 2 * P::LPC) {
+    const int m0 = (p % P::LPC) * P::EP;
+    lptr<T> dst = p < P::LPC ? pn : px + H + 1;
+#pragma unroll
+    for (int e = 0; e < P::EP; ++e) dst[canon<true>(m0 + e, kLW, fwd) * P::RS] = piece_get<T>(pc, e);
+  }
+}
+
 }  // namespace
 
 // One sweep of one event by the whole workgroup (the steps of wave w).
@@ -331,9 +359,24 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
   // Registers: OWN = old values of diagonal v in, new values out; OWNX /
   // NEXTX = old value one past the segment of v / v+1.  FD, FF, LE, LDX are
   // the loads in flight for the next step.
-  T OWN[kLW], LE[kLW];
+  T OWN[kLW], LE[kLW];  // LE: band-edge values (general variant)
+  u32x4 EF;             // band-edge pieces (full-length variant)
   T OWNX, NEXTX, LDX;
   Flight<T> FD, FF;
+  // band-edge segment of lane 0 / lane H-1 for column `col` (full variant)
+  auto eseg_of = [&](const DCol& col) -> int {
+    const bool eu = lane_first && col.valid && col.a > 0;
+    const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
+    const int64_t d = eu ? -S.tA : S.tA;
+    const int64_t lo = fwd ? col.node0 + d + c0 : col.node0 + d - c0 - (kLW - 1);
+    return (eu || ed) ? (int)lo : -1;
+  };
+  auto issue_edge = [&](const DCol& col) {
+    if constexpr (FULL)
+      EF = edge_issue<T>(S.u, eseg_of(col), g.H - 1, p);
+    else
+      load_edge(LE, col);
+  };
 
   const int v0 = -w;  // diagonal of step 0
   DCol cv = dcol(g, v0, p, S.base0, S.tA, S.tB);
@@ -346,7 +389,7 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
   seg_issue<T, FULL>(FD, S.u, seg_of(cv1), cnt, p);
   LDX = bload(S.u, x_off(cv1), T(0));
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
-  load_edge(LE, cv);
+  issue_edge(cv);
 
   const int steps = g.V + kDW - 1;
   int seg_prev = -1;  // segment of the previous step's new values (-1: none)
@@ -365,24 +408,27 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
     seg_land<T, FULL>(FD, px, cnt, fwd, p);
     seg_land<T, FULL>(FF, pf, cnt, fwd, p);
     // band-edge neighbours of v at lanes -1 (plane n) and H (plane x)
-    if (lane_first) {
+    if constexpr (FULL) {
+      edge_land<T>(EF, pn, px, g.H, fwd, p);
+    } else {
+      if (lane_first) {
 #pragma unroll
-      for (int j = 0; j < kLW; ++j)
-        if (FULL || j < cnt) pn[j * P::RS] = LE[j];
-    }
-    if (lane_last) {
+        for (int j = 0; j < kLW; ++j)
+          if (j < cnt) pn[j * P::RS] = LE[j];
+      }
+      if (lane_last) {
 #pragma unroll
-      for (int j = 0; j < kLW; ++j)
-        if (FULL || j < cnt) px[j * P::RS + g.H + 1] = LE[j];
+        for (int j = 0; j < kLW; ++j)
+          if (j < cnt) px[j * P::RS + g.H + 1] = LE[j];
+      }
     }
-    // loads for the next step: old values of v+2, slowness and edges of v+1
-    seg_issue<T, FULL>(FD, S.u, seg_of(cv2), cnt, p);
-    LDX = bload(S.u, x_off(cv2), T(0));
-    seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
-    load_edge(LE, cv1);
 
+    // Node updates of diagonal v (lanes of inactive steps hold no valid
+    // column and skip them).  The loads for the next step (old values of
+    // v+2, slowness and band edges of v+1) are issued between the first
+    // nodes, every step, so the memory pipeline works alongside the math.
     T xo = T(0);
-    if (active) {
+    {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : T(0);
       const int ra = S.flipA ? g.Ma - 1 - cv.a : cv.a;
       const int rb = S.flipB ? g.Nb - 1 - cv.b : cv.b;
@@ -390,6 +436,10 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
       const bool bf = rb == 0, bl = rb == g.Nb - 1;
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
+        if (e == 1) seg_issue<T, FULL>(FD, S.u, seg_of(cv2), cnt, p);
+        if (e == 2) LDX = bload(S.u, x_off(cv2), T(0));
+        if (e == 3) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
+        if (e == 4) issue_edge(cv1);
         if (FULL || e < cnt) {
           const int c = c0 + e;
           const int rw = e * P::RS;

@@ -283,6 +283,38 @@ __device__ __forceinline__ void seg_store(lptr<const T> pl, rsrc_t r, int seg, i
   }
 }
 
+// Publishes canonical node 0 of every lane's segment from a transfer in
+// flight (once it has arrived) to dst[lane]: the old value the wave
+// upstream along k needs one past its own segment.
+template <typename T, bool FULL>
+__device__ __forceinline__ void seg_publish(const Flight<T>& fl, lptr<T> dst, int cnt, bool fwd,
+                                            int p) {
+  using P = Pieces<T>;
+  if (FULL) cnt = kLW;
+  const int mp = fwd ? 0 : cnt - 1;  // memory element holding canonical node 0
+  const int nfull = (cnt / P::EP) * P::EP;
+  if (FULL || mp < nfull) {
+    const int kp = mp / P::EP, ep = mp % P::EP;
+    if (p % P::LPC == kp) {
+#pragma unroll
+      for (int i = 0; i < P::LPC; ++i) {
+        const int q = i * P::CPI + p / P::LPC;
+        T x = piece_get<T>(fl.pc[i], 0);
+#pragma unroll
+        for (int e = 1; e < P::EP; ++e)
+          if (e == ep) x = piece_get<T>(fl.pc[i], e);
+        dst[q] = x;
+      }
+    }
+  } else {
+    T x = fl.tail[0];
+#pragma unroll
+    for (int t = 1; t < P::EP - 1; ++t)
+      if (t == mp - nfull) x = fl.tail[t];
+    dst[p] = x;
+  }
+}
+
 // Band-edge segments of a full-length wave: lane 0's (a-1, b) column and
 // lane H-1's (a+1, b) column, both kLW nodes, moved as 2 x LPC pieces by
 // lanes 0 .. 2*LPC-1 of one transfer.  `eseg` = memory index of the lowest
@@ -322,7 +354,7 @@ __device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> p
 // the loop body: landing a load never waits for a store issued after it.
 template <typename T, bool FULL>
 __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
-                                                lptr<T> xch, lptr<T> planes) {
+                                                lptr<T> xch, lptr<T> xold, lptr<T> planes) {
   using P = Pieces<T>;
   const DGeom& g = S.g;
   const int p = threadIdx.x & 63;
@@ -332,7 +364,7 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
   const bool fwd = !S.flipC;  // the segment runs forward in memory
   const bool lane_first = (p == 0);
   const bool lane_last = (p == g.H - 1);
-  const bool has_x = c0 + Lw < L;  // a node exists one past the segment
+  const bool has_x = c0 + Lw < L && w + 1 < kDW;  // a node exists one past the segment
   // planes of this wave: new values of v-1 (then v), old values of v+1,
   // slowness of v
   lptr<T> pn = planes + w * 3 * P::PLANE;
@@ -344,10 +376,6 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
     const int64_t lo = fwd ? col.node0 + c0 : col.node0 - c0 - (cnt - 1);
     return (col.valid && cnt > 0) ? (int)lo : -1;
   };
-  // One past the segment (old value), or kOOB.
-  auto x_off = [&](const DCol& col) -> uint32_t {
-    return boff<T>(col.valid && has_x, col.node0 + (int64_t)(c0 + Lw) * S.tC);
-  };
   // Band-edge neighbour of lane 0 ((a-1, b), new) and lane H-1 ((a+1, b),
   // old) of column `col`, one load per node (canonical order).
   auto load_edge = [&](T(&buf)[kLW], const DCol& col) {
@@ -356,12 +384,13 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
     load_seg(buf, S.u, col, eu ? -S.tA : S.tA, eu || ed, c0, cnt, L, S.tC);
   };
 
-  // Registers: OWN = old values of diagonal v in, new values out; OWNX /
-  // NEXTX = old value one past the segment of v / v+1.  FD, FF, LE, LDX are
-  // the loads in flight for the next step.
+  // Registers: OWN = old values of diagonal v in, new values out.  FD, FF,
+  // EF / LE are the loads in flight for the next step.  The old value one
+  // past the segment comes from the next wave, which publishes canonical
+  // node 0 of its transfer in flight (old values of our diagonal v at
+  // step T-1) to xold.
   T OWN[kLW], LE[kLW];  // LE: band-edge values (general variant)
   u32x4 EF;             // band-edge pieces (full-length variant)
-  T OWNX, NEXTX, LDX;
   Flight<T> FD, FF;
   // band-edge segment of lane 0 / lane H-1 for column `col` (full variant)
   auto eseg_of = [&](const DCol& col) -> int {
@@ -382,14 +411,15 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
   DCol cv = dcol(g, v0, p, S.base0, S.tA, S.tB);
   DCol cv1 = dcol(g, v0 + 1, p, S.base0, S.tA, S.tB);
   DCol cv2 = dcol(g, v0 + 2, p, S.base0, S.tA, S.tB);
-  // prologue: plane x = old(v0); in flight: old(v0+1), f(v0), edges(v0)
+  // prologue: plane x = old(v0); in flight: old(v0+1), f(v0), edges(v0);
+  // node 0 of old(v0+1) published for the wave upstream (its step 0)
   seg_issue<T, FULL>(FD, S.u, seg_of(cv), cnt, p);
-  NEXTX = bload(S.u, x_off(cv), T(0));
   seg_land<T, FULL>(FD, px, cnt, fwd, p);
   seg_issue<T, FULL>(FD, S.u, seg_of(cv1), cnt, p);
-  LDX = bload(S.u, x_off(cv1), T(0));
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
   issue_edge(cv);
+  seg_publish<T, FULL>(FD, xold + kDW * 64 + w * 64, cnt, fwd, p);
+  __syncthreads();
 
   const int steps = g.V + kDW - 1;
   int seg_prev = -1;  // segment of the previous step's new values (-1: none)
@@ -403,8 +433,6 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
 #pragma unroll
     for (int j = 0; j < kLW; ++j)
       if (FULL || j < cnt) OWN[j] = px[j * P::RS + p + 1];
-    OWNX = NEXTX;
-    NEXTX = LDX;
     seg_land<T, FULL>(FD, px, cnt, fwd, p);
     seg_land<T, FULL>(FF, pf, cnt, fwd, p);
     // band-edge neighbours of v at lanes -1 (plane n) and H (plane x)
@@ -430,6 +458,7 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
     T xo = T(0);
     {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : T(0);
+      const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : T(0);
       const int ra = S.flipA ? g.Ma - 1 - cv.a : cv.a;
       const int rb = S.flipB ? g.Nb - 1 - cv.b : cv.b;
       const bool af = ra == 0, al = ra == g.Ma - 1;
@@ -437,7 +466,6 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
         if (e == 1) seg_issue<T, FULL>(FD, S.u, seg_of(cv2), cnt, p);
-        if (e == 2) LDX = bload(S.u, x_off(cv2), T(0));
         if (e == 3) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
         if (e == 4) issue_edge(cv1);
         if (FULL || e < cnt) {
@@ -449,7 +477,7 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
           const T db = px[rw + p + 1];  // lane p, old
           const T fe = pf[rw + p + 1];
           const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
-          const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : OWNX;
+          const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
           if (cv.valid) {
             const int rc = S.flipC ? L - 1 - c : c;
             const T ma = axis_min(ua, da, S.flipA, af, al);
@@ -468,6 +496,8 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
 #pragma unroll
     for (int j = 0; j < kLW; ++j)
       if (FULL || j < cnt) pn[j * P::RS + p + 1] = OWN[j];
+    // node 0 of old(v+2), which the wave upstream needs at step T+1
+    seg_publish<T, FULL>(FD, xold + (t & 1) * kDW * 64 + w * 64, cnt, fwd, p);
     seg_prev = active ? seg_of(cv) : -1;
     xch[(t & 1) * kDW * 64 + w * 64 + p] = xo;
     cv = cv1;
@@ -483,22 +513,24 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
 // kLW nodes take the fixed-length variant; the others (and idle waves) the
 // general one.  Both run the same number of steps and barriers.
 template <typename T>
-__device__ __forceinline__ void diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> planes) {
+__device__ __forceinline__ void diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> xold,
+                                           lptr<T> planes) {
   // the wave index is uniform across the wave: keep it (and everything
   // derived from it) in scalar registers
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
   if (cnt == kLW)
-    diag_sweep_wave<T, true>(S, w, cnt, xch, planes);
+    diag_sweep_wave<T, true>(S, w, cnt, xch, xold, planes);
   else
-    diag_sweep_wave<T, false>(S, w, cnt, xch, planes);
+    diag_sweep_wave<T, false>(S, w, cnt, xch, xold, planes);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __restrict__ evs,
                                                              int max_iter) {
   __shared__ T xch[2 * kDW * 64];
+  __shared__ T xold[2 * kDW * 64];
   __shared__ T planes[kDW * 3 * Pieces<T>::PLANE];
   __shared__ uint64_t red[kDW];
   const EventDesc& ev = evs[blockIdx.x];
@@ -532,7 +564,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
       S.tC = S.flipC ? -1 : 1;
       S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
-      diag_sweep<T>(S, (lptr<T>)xch, (lptr<T>)planes);
+      diag_sweep<T>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
     }
     // err = max |u - u_old| over the event (coalesced pass after the 8
     // sweeps), which also leaves u_old = u for the next iteration

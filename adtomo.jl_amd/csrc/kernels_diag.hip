@@ -463,6 +463,18 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
       const int rb = S.flipB ? g.Nb - 1 - cv.b : cv.b;
       const bool af = ra == 0, al = ra == g.Ma - 1;
       const bool bf = rb == 0, bl = rb == g.Nb - 1;
+      // The a and b neighbours are read straight from the planes: the sweep
+      // direction and which of a, b is the x axis only choose addresses.
+      // Canonical a-1 (new) / a+1 (old) sit at positions p / p+2, canonical
+      // b-1 (new) / b+1 (old) at position p+1 of planes n / x.
+      const lptr<T> a_up = pn + p, a_dn = px + p + 2;
+      const lptr<T> b_up = pn + p + 1, b_dn = px + p + 1;
+      const lptr<T> a_plus = S.flipA ? a_up : a_dn, a_minus = S.flipA ? a_dn : a_up;
+      const lptr<T> b_plus = S.flipB ? b_up : b_dn, b_minus = S.flipB ? b_dn : b_up;
+      const lptr<T> x_plus = g.a_is_i ? a_plus : b_plus, x_minus = g.a_is_i ? a_minus : b_minus;
+      const lptr<T> y_plus = g.a_is_i ? b_plus : a_plus, y_minus = g.a_is_i ? b_minus : a_minus;
+      const bool xf = g.a_is_i ? af : bf, xl = g.a_is_i ? al : bl;
+      const bool yf = g.a_is_i ? bf : af, yl = g.a_is_i ? bl : al;
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
         if (e == 1) seg_issue<T, FULL>(FD, S.u, seg_of(cv2), cnt, p);
@@ -471,20 +483,17 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
         if (FULL || e < cnt) {
           const int c = c0 + e;
           const int rw = e * P::RS;
-          const T ua = pn[rw + p];      // lane p-1, new
-          const T ub = pn[rw + p + 1];  // lane p, new
-          const T da = px[rw + p + 2];  // lane p+1, old
-          const T db = px[rw + p + 1];  // lane p, old
+          // std::min(u(+1), u(-1)) per axis, one-sided at the edges
+          // (Eikonal3D.cpp:47-52)
+          const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
           const T fe = pf[rw + p + 1];
           const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
           const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
           if (cv.valid) {
             const int rc = S.flipC ? L - 1 - c : c;
-            const T ma = axis_min(ua, da, S.flipA, af, al);
-            const T mb = axis_min(ub, db, S.flipB, bf, bl);
+            const T ux = xf ? xp : (xl ? xm : smin(xp, xm));
+            const T uy = yf ? yp : (yl ? ym : smin(yp, ym));
             const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == L - 1);
-            const T ux = g.a_is_i ? ma : mb;
-            const T uy = g.a_is_i ? mb : ma;
             const T cand = solution3d(ux, uy, mc, fe, S.h);
             OWN[e] = smin(cand, OWN[e]);
           }

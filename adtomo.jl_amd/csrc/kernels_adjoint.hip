@@ -51,6 +51,35 @@ struct alignas(4 * sizeof(T)) Rec {
   T w[3];
 };
 
+// Coordinates (i, j, k) of node p, advanced by a fixed stride without
+// divisions (a grid-stride loop over the flat index).
+struct Walk {
+  int co[3];
+  int step[3];
+  int ex[3];
+  __device__ Walk(int64_t p, int64_t stride, int m, int n, int l) {
+    const int64_t nl = (int64_t)n * l;
+    co[0] = (int)(p / nl);
+    co[1] = (int)((p / l) % n);
+    co[2] = (int)(p % l);
+    step[0] = (int)(stride / nl);
+    step[1] = (int)((stride / l) % n);
+    step[2] = (int)(stride % l);
+    ex[0] = m;
+    ex[1] = n;
+    ex[2] = l;
+  }
+  __device__ __forceinline__ void advance() {
+    co[2] += step[2];
+    int c = co[2] >= ex[2];
+    co[2] -= c ? ex[2] : 0;
+    co[1] += step[1] + c;
+    c = co[1] >= ex[1];
+    co[1] -= c ? ex[1] : 0;
+    co[0] += step[0] + c;
+  }
+};
+
 constexpr uint32_t kCntShift = 13;
 constexpr uint32_t kZ = 0x40u;
 
@@ -60,9 +89,8 @@ __device__ __forceinline__ uint32_t meta_shift(int64_t p) { return (p & 1) ? 16u
 // side of every active dimension.
 template <typename T>
 __device__ __forceinline__ uint32_t code3(gptr<const T> u, gptr<const T> u0, int m, int n, int l,
-                                         int64_t p) {
+                                         int64_t p, const int (&co)[3]) {
   const int64_t nl = (int64_t)n * l;
-  const int co[3] = {(int)(p / nl), (int)((p / l) % n), (int)(p % l)};
   const int ex[3] = {m, n, l};
   const int64_t st[3] = {nl, l, 1};
   const T up = u[p];
@@ -186,9 +214,12 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
     nonz = 0;
   }
   // pass 1: upwind codes; grad_u0 = g where u == u0 (Eikonal3D.cpp:105-110)
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x) {
-    if (gu0) gu0[p] = (u[p] == u0[p]) ? g[p] : T(0);
-    meta[p] = (uint16_t)code3<T>(u, u0, m, n, l, p);
+  {
+    Walk wk(threadIdx.x, blockDim.x, m, n, l);
+    for (int64_t p = threadIdx.x; p < N; p += blockDim.x, wk.advance()) {
+      if (gu0) gu0[p] = (u[p] == u0[p]) ? g[p] : T(0);
+      meta[p] = (uint16_t)code3<T>(u, u0, m, n, l, p, wk.co);
+    }
   }
   __syncthreads();
   // pass 2: feeders, counters, coefficients; nodes without feeders start.
@@ -197,9 +228,10 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
   const int ex[3] = {m, n, l};
   const int64_t st[3] = {nl, l, 1};
   int my_nonz = 0;
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x) {
+  Walk wk(threadIdx.x, blockDim.x, m, n, l);
+  for (int64_t p = threadIdx.x; p < N; p += blockDim.x, wk.advance()) {
     const uint32_t cr = meta[p];
-    const int co[3] = {(int)(p / nl), (int)((p / l) % n), (int)(p % l)};
+    const int(&co)[3] = wk.co;
     uint32_t fm = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {

@@ -61,6 +61,18 @@ class AdtEvent3d(ctypes.Structure):
     ]
 
 
+class AdtSource3d(ctypes.Structure):
+    _fields_ = [("x", _d), ("y", _d), ("z", _d), ("scale", _d), ("u0", _p)]
+
+
+class AdtMisfit3d(ctypes.Structure):
+    _fields_ = [
+        ("m", _i64), ("n", _i64), ("l", _i64),
+        ("u", _p), ("tobs", _p), ("weight", _p),
+        ("tcal", _p), ("grad_u", _p), ("loss", _p),
+    ]
+
+
 class AdtError(RuntimeError):
     pass
 
@@ -75,6 +87,12 @@ _SIGS = {
     "adt_eikonal3d_bwd": [_p, _p, _p, _p, "R", _i64, _i64, _i64, _p, _p, _p, _i32, _p],
     "adt_eikonal3d_batched": [_p, _i32, _i32, _p, _p],
     "adt_selftest_arith": [_i32, _p, _p, _p, _i64, _p],
+    "adt_source_u0_3d_batched": [_p, _i32, _p, _i64, _i64, _i64, "R", _p],
+    "adt_misfit3d_batched": [_p, _i32, _p, _p, _p, _i64, _p],
+    "adt_model3d": [_p, _p, _i64, _p, _p, _p],
+    "adt_model3d_grad": [_p, _p, _p, _p, _i64, _p, _p],
+    "adt_smooth3d_l1": [_p, _i64, _i64, _i64, _i32, _i32, "R", _p, _p, _p],
+    "adt_sum_arrays": [_p, _i32, _i64, _p, _p],
 }
 
 

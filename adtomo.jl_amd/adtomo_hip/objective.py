@@ -1,0 +1,135 @@
+"""The tomography objective around the Eikonal solves, on the device.
+
+Mirrors scripts/inversion.jl (the caller of src/eikonal_op.jl):
+
+  source_u0     :48-60   8-corner initial travel times per station
+  misfit        :64-105  trilinear gather of the travel times at the events,
+                         quality-weighted L2 misfit (tobs == -1 skipped) and
+                         its adjoint source dL/du for the Eikonal adjoint
+  model         :42-43   fvar = 2 sigmoid(x) - 1 + vel0, f = 1 / fvar
+  model_grad             chain rule back to x
+  smooth_l1     :107-121 lambda * sum |fvar - box(fvar)| (periodic box mean)
+  sum_arrays             sum of per-station gradients (fixed order)
+
+Arguments are torch tensors on a HIP device (float64 or float32, contiguous,
+(m, n, l) row-major); coordinates are the reference's 1-based reals.  Every
+call runs HIP kernels; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .ops import _sfx, _stream, _torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _check_dev(*ts):
+    for t in ts:
+        if t is not None and (not t.is_cuda or not t.is_contiguous()):
+            raise ValueError("expected contiguous device tensors")
+
+
+def source_u0(stations, vel0, h: float, scale: float = 1.0, out=None):
+    """Initial travel times for each station (x, y, z) (1-based reals).
+
+    Returns a list of (m, n, l) tensors (or fills ``out``)."""
+    torch = _torch()
+    _check_dev(vel0)
+    m, n, l = vel0.shape
+    stations = [tuple(map(float, s)) for s in stations]
+    if out is None:
+        out = [torch.empty_like(vel0) for _ in stations]
+    tab = (L.AdtSource3d * max(len(stations), 1))()
+    for e, (x, y, z) in enumerate(stations):
+        tab[e] = L.AdtSource3d(x, y, z, float(scale), out[e].data_ptr())
+    fn = getattr(L.lib(), "adt_source_u0_3d_batched" + _sfx(vel0.dtype))
+    L.check(fn(ctypes.byref(tab), len(stations), _ptr(vel0), m, n, l, h, _stream()), "source_u0")
+    return out
+
+
+def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None):
+    """Receiver gather + misfit for each station.
+
+    ``us``: list of travel-time tensors; ``rx, ry, rz``: event coordinates
+    (host sequences, 1-based); ``tobs``/``weights``: per-station tensors of
+    length len(rx).  ``grad_out``: optional list of tensors to receive
+    dL/du.  Returns dict(loss=tensor[nstation], tcal=[...], grad_u=[...])."""
+    torch = _torch()
+    ns = len(us)
+    nrec = len(rx)
+    dt = us[0].dtype
+    _check_dev(*us, *tobs, *weights)
+    loss = torch.zeros(ns, dtype=dt, device=us[0].device)
+    tcals = [torch.empty(nrec, dtype=dt, device=us[0].device) for _ in range(ns)] if tcal else None
+    if grad:
+        grads = grad_out if grad_out is not None else [torch.empty_like(u) for u in us]
+        _check_dev(*grads)
+    else:
+        grads = None
+    tab = (L.AdtMisfit3d * max(ns, 1))()
+    esz = loss.element_size()
+    for e, u in enumerate(us):
+        m, n, l = u.shape
+        tab[e] = L.AdtMisfit3d(m, n, l, u.data_ptr(), tobs[e].data_ptr(), weights[e].data_ptr(),
+                               tcals[e].data_ptr() if tcal else None,
+                               grads[e].data_ptr() if grad else None, loss.data_ptr() + e * esz)
+    ax = np.ascontiguousarray(rx, dtype=np.float64)
+    ay = np.ascontiguousarray(ry, dtype=np.float64)
+    az = np.ascontiguousarray(rz, dtype=np.float64)
+    fn = getattr(L.lib(), "adt_misfit3d_batched" + _sfx(dt))
+    rc = fn(ctypes.byref(tab), ns, ax.ctypes.data_as(ctypes.c_void_p),
+            ay.ctypes.data_as(ctypes.c_void_p), az.ctypes.data_as(ctypes.c_void_p), nrec, _stream())
+    L.check(rc, "misfit")
+    return {"loss": loss, "tcal": tcals, "grad_u": grads}
+
+
+def model(x, vel0, slowness=True):
+    """(fvar, f): fvar = 2 sigmoid(x) - 1 + vel0, f = 1/fvar."""
+    torch = _torch()
+    _check_dev(x, vel0)
+    fvar = torch.empty_like(x)
+    f = torch.empty_like(x) if slowness else None
+    fn = getattr(L.lib(), "adt_model3d" + _sfx(x.dtype))
+    L.check(fn(_ptr(x), _ptr(vel0), x.numel(), _ptr(fvar), _ptr(f), _stream()), "model3d")
+    return fvar, f
+
+
+def model_grad(x, fvar, grad_f=None, grad_fvar=None):
+    """d/dx through fvar(x) and f = 1/fvar."""
+    torch = _torch()
+    _check_dev(x, fvar, grad_f, grad_fvar)
+    gx = torch.empty_like(x)
+    fn = getattr(L.lib(), "adt_model3d_grad" + _sfx(x.dtype))
+    L.check(fn(_ptr(x), _ptr(fvar), _ptr(grad_f), _ptr(grad_fvar), x.numel(), _ptr(gx), _stream()),
+            "model3d_grad")
+    return gx
+
+
+def smooth_l1(fvar, sh: int, sv: int, lam: float, grad=True):
+    """(value, grad_fvar) of lambda * sum |fvar - box(fvar)|."""
+    torch = _torch()
+    _check_dev(fvar)
+    m, n, l = fvar.shape
+    value = torch.zeros(1, dtype=fvar.dtype, device=fvar.device)
+    g = torch.empty_like(fvar) if grad else None
+    fn = getattr(L.lib(), "adt_smooth3d_l1" + _sfx(fvar.dtype))
+    L.check(fn(_ptr(fvar), m, n, l, sh, sv, lam, _ptr(g), _ptr(value), _stream()), "smooth3d_l1")
+    return value, g
+
+
+def sum_arrays(arrays, out=None):
+    """Sum of equally shaped device tensors, in list order."""
+    torch = _torch()
+    _check_dev(*arrays)
+    if out is None:
+        out = torch.empty_like(arrays[0])
+    ptrs = (ctypes.c_void_p * len(arrays))(*[a.data_ptr() for a in arrays])
+    fn = getattr(L.lib(), "adt_sum_arrays" + _sfx(arrays[0].dtype))
+    L.check(fn(ptrs, len(arrays), arrays[0].numel(), _ptr(out), _stream()), "sum_arrays")
+    return out

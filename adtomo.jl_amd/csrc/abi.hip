@@ -719,6 +719,209 @@ int adt_graph_destroy(adt_graph* g) {
   return ADT_OK;
 }
 
+}  // extern "C"
+
+// ----------------------------------------------------------- objective ---
+
+namespace {
+
+template <typename T>
+int source_u0_impl(const adt_source3d* src, int32_t nsrc, const T* vel0, int64_t m, int64_t n,
+                   int64_t l, T h, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (nsrc < 0 || (nsrc > 0 && (!src || !vel0))) return fail(ADT_EINVAL, "source_u0: bad arguments");
+  if (nsrc == 0) return ADT_OK;
+  const double ext[3] = {(double)m, (double)n, (double)l};
+  for (int s = 0; s < nsrc; ++s) {
+    const double c[3] = {src[s].x, src[s].y, src[s].z};
+    if (!src[s].u0) return fail(ADT_EINVAL, "source_u0: NULL u0 for station " + std::to_string(s));
+    for (int d = 0; d < 3; ++d)
+      if (!(c[d] >= 1.0 && c[d] <= ext[d]))
+        return fail(ADT_EINVAL, "source_u0: station " + std::to_string(s) +
+                                    " outside the 1-based grid");
+  }
+  hipStream_t st = (hipStream_t)stream;
+  DevBuf tab;
+  if ((rc = tab.alloc(sizeof(adt_source3d) * nsrc, st))) return rc;
+  ADT_HIP(hipMemcpyAsync(tab.p, src, sizeof(adt_source3d) * nsrc, hipMemcpyHostToDevice, st));
+  ADT_HIP(hipStreamSynchronize(st));  // the table is the caller's host memory
+  adt::source_u0_kernel<T><<<nsrc, 256, 0, st>>>((const adt_source3d*)tab.p, vel0, (int)m, (int)n,
+                                                 (int)l, h);
+  ADT_HIP(hipGetLastError());
+  return ADT_OK;
+}
+
+template <typename T>
+int misfit_impl(const adt_misfit3d* items, int32_t nitem, const double* rx, const double* ry,
+                const double* rz, int64_t nrec, void* stream) {
+  if (nitem < 0 || nrec < 0 || nrec > (int64_t(1) << 30) ||
+      (nitem > 0 && nrec > 0 && (!items || !rx || !ry || !rz)))
+    return fail(ADT_EINVAL, "misfit: bad arguments");
+  if (nitem == 0) return ADT_OK;
+  for (int i = 0; i < nitem; ++i) {
+    const adt_misfit3d& it = items[i];
+    int rc = check_dims3(it.m, it.n, it.l);
+    if (rc) return rc;
+    if (!it.u || !it.loss || (nrec > 0 && (!it.tobs || !it.weight)))
+      return fail(ADT_EINVAL, "misfit: NULL buffer for solve " + std::to_string(i));
+    for (int64_t j = 0; j < nrec; ++j) {
+      const double c[3] = {rx[j], ry[j], rz[j]};
+      const double ext[3] = {(double)it.m, (double)it.n, (double)it.l};
+      for (int d = 0; d < 3; ++d)
+        if (!(c[d] >= 1.0 && c[d] <= ext[d]))
+          return fail(ADT_EINVAL, "misfit: receiver " + std::to_string(j) +
+                                      " outside the 1-based grid of solve " + std::to_string(i));
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  DevBuf tab, coords;
+  int rc;
+  if ((rc = tab.alloc(sizeof(adt_misfit3d) * nitem, st))) return rc;
+  if ((rc = coords.alloc(sizeof(double) * 3 * (nrec ? nrec : 1), st))) return rc;
+  double* dc = (double*)coords.p;
+  ADT_HIP(hipMemcpyAsync(tab.p, items, sizeof(adt_misfit3d) * nitem, hipMemcpyHostToDevice, st));
+  if (nrec) {
+    ADT_HIP(hipMemcpyAsync(dc, rx, sizeof(double) * nrec, hipMemcpyHostToDevice, st));
+    ADT_HIP(hipMemcpyAsync(dc + nrec, ry, sizeof(double) * nrec, hipMemcpyHostToDevice, st));
+    ADT_HIP(hipMemcpyAsync(dc + 2 * nrec, rz, sizeof(double) * nrec, hipMemcpyHostToDevice, st));
+  }
+  adt::misfit_kernel<T><<<nitem, 256, 0, st>>>((const adt_misfit3d*)tab.p, dc, dc + nrec,
+                                                dc + 2 * nrec, (int)nrec);
+  ADT_HIP(hipGetLastError());
+  // the staged coordinates are host copies: wait before they go out of scope
+  ADT_HIP(hipStreamSynchronize(st));
+  return ADT_OK;
+}
+
+int grid_for(int64_t N) {
+  const int64_t b = (N + 255) / 256;
+  return (int)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+template <typename T>
+int model_impl(const T* x, const T* vel0, int64_t N, T* fvar, T* f, void* stream) {
+  if (N < 0 || (N > 0 && (!x || !vel0 || !fvar))) return fail(ADT_EINVAL, "model3d: bad arguments");
+  if (N == 0) return ADT_OK;
+  adt::model_kernel<T><<<grid_for(N), 256, 0, (hipStream_t)stream>>>(x, vel0, N, fvar, f);
+  ADT_HIP(hipGetLastError());
+  return ADT_OK;
+}
+
+template <typename T>
+int model_grad_impl(const T* x, const T* fvar, const T* grad_f, const T* grad_fvar, int64_t N,
+                    T* grad_x, void* stream) {
+  if (N < 0 || (N > 0 && (!x || !fvar || !grad_x)))
+    return fail(ADT_EINVAL, "model3d_grad: bad arguments");
+  if (N == 0) return ADT_OK;
+  adt::model_grad_kernel<T><<<grid_for(N), 256, 0, (hipStream_t)stream>>>(x, fvar, grad_f,
+                                                                          grad_fvar, N, grad_x);
+  ADT_HIP(hipGetLastError());
+  return ADT_OK;
+}
+
+template <typename T>
+int smooth_impl(const T* fvar, int64_t m, int64_t n, int64_t l, int32_t sh, int32_t sv, T lambda,
+                T* grad_fvar, T* value, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!fvar || !value) return fail(ADT_EINVAL, "smooth3d: NULL buffer");
+  if (sh < 1 || sv < 1 || !(sh & 1) || !(sv & 1) || sh > m || sh > n || sv > l)
+    return fail(ADT_EINVAL, "smooth3d: window sizes must be odd and fit the grid");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t N = m * n * l;
+  const int nb = grid_for(N);
+  DevBuf ws;
+  if ((rc = ws.alloc(sizeof(T) * (N + nb), st))) return rc;
+  T* sgn = (T*)ws.p;
+  T* partial = sgn + N;
+  const T wgt = T(1) / T((double)sh * sh * sv);
+  adt::smooth_pass1_kernel<T><<<nb, 256, 0, st>>>(fvar, (int)m, (int)n, (int)l, sh / 2, sv / 2, wgt,
+                                                  sgn, partial);
+  ADT_HIP(hipGetLastError());
+  if (grad_fvar) {
+    adt::smooth_pass2_kernel<T><<<nb, 256, 0, st>>>(sgn, (int)m, (int)n, (int)l, sh / 2, sv / 2,
+                                                    wgt, lambda, grad_fvar, partial, nb, value);
+  } else {
+    // value only: one thread sums the block partials
+    adt::smooth_pass2_kernel<T><<<1, 1, 0, st>>>(sgn, (int)m, (int)n, (int)l, sh / 2, sv / 2, wgt,
+                                                 lambda, nullptr, partial, nb, value);
+  }
+  ADT_HIP(hipGetLastError());
+  return ADT_OK;
+}
+
+template <typename T>
+int sum_arrays_impl(const T* const* arrays, int32_t narr, int64_t N, T* out, void* stream) {
+  if (narr < 1 || N < 0 || !arrays || !out) return fail(ADT_EINVAL, "sum_arrays: bad arguments");
+  for (int k = 0; k < narr; ++k)
+    if (!arrays[k]) return fail(ADT_EINVAL, "sum_arrays: NULL array");
+  if (N == 0) return ADT_OK;
+  hipStream_t st = (hipStream_t)stream;
+  DevBuf tab;
+  int rc;
+  if ((rc = tab.alloc(sizeof(T*) * narr, st))) return rc;
+  ADT_HIP(hipMemcpyAsync(tab.p, arrays, sizeof(T*) * narr, hipMemcpyHostToDevice, st));
+  ADT_HIP(hipStreamSynchronize(st));  // the pointer table is the caller's host memory
+  adt::sum_arrays_kernel<T><<<grid_for(N), 256, 0, st>>>((const T* const*)tab.p, narr, N, out);
+  ADT_HIP(hipGetLastError());
+  return ADT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adt_source_u0_3d_batched_f64(const adt_source3d* src, int32_t nsrc, const double* vel0,
+                                 int64_t m, int64_t n, int64_t l, double h, void* stream) {
+  return source_u0_impl<double>(src, nsrc, vel0, m, n, l, h, stream);
+}
+int adt_source_u0_3d_batched_f32(const adt_source3d* src, int32_t nsrc, const float* vel0,
+                                 int64_t m, int64_t n, int64_t l, float h, void* stream) {
+  return source_u0_impl<float>(src, nsrc, vel0, m, n, l, h, stream);
+}
+int adt_misfit3d_batched_f64(const adt_misfit3d* items, int32_t nitem, const double* rx,
+                             const double* ry, const double* rz, int64_t nrec, void* stream) {
+  return misfit_impl<double>(items, nitem, rx, ry, rz, nrec, stream);
+}
+int adt_misfit3d_batched_f32(const adt_misfit3d* items, int32_t nitem, const double* rx,
+                             const double* ry, const double* rz, int64_t nrec, void* stream) {
+  return misfit_impl<float>(items, nitem, rx, ry, rz, nrec, stream);
+}
+int adt_model3d_f64(const double* x, const double* vel0, int64_t N, double* fvar, double* f,
+                    void* stream) {
+  return model_impl<double>(x, vel0, N, fvar, f, stream);
+}
+int adt_model3d_f32(const float* x, const float* vel0, int64_t N, float* fvar, float* f,
+                    void* stream) {
+  return model_impl<float>(x, vel0, N, fvar, f, stream);
+}
+int adt_model3d_grad_f64(const double* x, const double* fvar, const double* grad_f,
+                         const double* grad_fvar, int64_t N, double* grad_x, void* stream) {
+  return model_grad_impl<double>(x, fvar, grad_f, grad_fvar, N, grad_x, stream);
+}
+int adt_model3d_grad_f32(const float* x, const float* fvar, const float* grad_f,
+                         const float* grad_fvar, int64_t N, float* grad_x, void* stream) {
+  return model_grad_impl<float>(x, fvar, grad_f, grad_fvar, N, grad_x, stream);
+}
+int adt_smooth3d_l1_f64(const double* fvar, int64_t m, int64_t n, int64_t l, int32_t sh,
+                        int32_t sv, double lambda, double* grad_fvar, double* value,
+                        void* stream) {
+  return smooth_impl<double>(fvar, m, n, l, sh, sv, lambda, grad_fvar, value, stream);
+}
+int adt_smooth3d_l1_f32(const float* fvar, int64_t m, int64_t n, int64_t l, int32_t sh,
+                        int32_t sv, float lambda, float* grad_fvar, float* value, void* stream) {
+  return smooth_impl<float>(fvar, m, n, l, sh, sv, lambda, grad_fvar, value, stream);
+}
+int adt_sum_arrays_f64(const double* const* arrays, int32_t narr, int64_t N, double* out,
+                       void* stream) {
+  return sum_arrays_impl<double>(arrays, narr, N, out, stream);
+}
+int adt_sum_arrays_f32(const float* const* arrays, int32_t narr, int64_t N, float* out,
+                       void* stream) {
+  return sum_arrays_impl<float>(arrays, narr, N, out, stream);
+}
+
 int adt_selftest_arith_f64(int32_t op, const double* a, const double* b, double* out, int64_t n,
                            void* stream) {
   if (n < 0 || !a || !out || op < 0 || op > 6 || ((op == 3 || (op >= 4 && op <= 5)) && !b))

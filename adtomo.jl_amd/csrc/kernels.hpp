@@ -44,6 +44,29 @@ struct Event2Desc {
   adt_info* info;
 };
 
+// Objective kernels use the public descriptors directly (plain C layout).
+using SourceDesc = adt_source3d;
+using MisfitDesc = adt_misfit3d;
+
+template <typename T>
+__global__ void source_u0_kernel(const SourceDesc* src, const T* vel0, int m, int n, int l, T h);
+template <typename T>
+__global__ void misfit_kernel(const MisfitDesc* d, const double* rx, const double* ry,
+                              const double* rz, int nrec);
+template <typename T>
+__global__ void model_kernel(const T* x, const T* vel0, int64_t N, T* fvar, T* f);
+template <typename T>
+__global__ void sum_arrays_kernel(const T* const* in, int nin, int64_t N, T* out);
+template <typename T>
+__global__ void smooth_pass1_kernel(const T* fvar, int m, int n, int l, int rh, int rv, T wgt,
+                                    T* sgn, T* partial);
+template <typename T>
+__global__ void smooth_pass2_kernel(const T* sgn, int m, int n, int l, int rh, int rv, T wgt,
+                                    T lambda, T* grad, const T* partial, int npart, T* value);
+template <typename T>
+__global__ void model_grad_kernel(const T* x, const T* fvar, const T* grad_f, const T* grad_fvar,
+                                  int64_t N, T* grad_x);
+
 template <typename T>
 __global__ void fwd3_plane_kernel(T* u, const T* f, T h, int m, int n, int l, int dir, int s,
                                   int a_lo, int a_cnt);

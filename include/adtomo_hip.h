@@ -148,6 +148,76 @@ int adt_graph_launch(adt_graph *g, void *stream);
 int adt_graph_infos(adt_graph *g, adt_info *infos, void *stream);
 int adt_graph_destroy(adt_graph *g);
 
+/* ---- Objective around the path (scripts/inversion.jl), all on the device.
+ * Grid coordinates are 1-based and real-valued, as in the reference's
+ * station/event tables (x along i, y along j, z along k). */
+
+/* One station's initial travel times (scripts/inversion.jl:48-60):
+ * u0 = 1000, and at the 8 nodes around (x, y, z)
+ * u0 = |(x,y,z) - node| * h / vel0[node] * scale
+ * (scale = 1 for P; vp/vs for S, scripts/inversion_joint.jl:65-73). */
+typedef struct adt_source3d {
+  double x, y, z;  /* 1-based coordinates, 1 <= floor, ceil <= dim */
+  double scale;
+  void *u0;        /* out: [m*n*l] device */
+} adt_source3d;
+int adt_source_u0_3d_batched_f64(const adt_source3d *src, int32_t nsrc, const double *vel0,
+                                 int64_t m, int64_t n, int64_t l, double h, void *stream);
+int adt_source_u0_3d_batched_f32(const adt_source3d *src, int32_t nsrc, const float *vel0,
+                                 int64_t m, int64_t n, int64_t l, float h, void *stream);
+
+/* Receiver gather + misfit of one solve (station) against nrec receivers
+ * (events) at host coordinates (rx, ry, rz) (scripts/inversion.jl:64-105):
+ * tcal = trilinear interpolation of u (a dimension whose floor and ceil
+ * coincide is not interpolated), loss = sum over tobs != -1 of
+ * weight * (tobs - tcal)^2, grad_u = d loss / d u (dense, zero except at
+ * the interpolation nodes; contributions of receivers sharing a node are
+ * added atomically, so their order may vary between runs).  tcal and
+ * grad_u may be NULL; loss is a device scalar. */
+typedef struct adt_misfit3d {
+  int64_t m, n, l;
+  const void *u;       /* [m*n*l] travel times */
+  const void *tobs;    /* [nrec] observed times, -1 = missing */
+  const void *weight;  /* [nrec] quality weights */
+  void *tcal;          /* out: [nrec] */
+  void *grad_u;        /* out: [m*n*l] */
+  void *loss;          /* out: scalar */
+} adt_misfit3d;
+int adt_misfit3d_batched_f64(const adt_misfit3d *items, int32_t nitem, const double *rx,
+                             const double *ry, const double *rz, int64_t nrec, void *stream);
+int adt_misfit3d_batched_f32(const adt_misfit3d *items, int32_t nitem, const double *rx,
+                             const double *ry, const double *rz, int64_t nrec, void *stream);
+
+/* Model parametrisation (scripts/inversion.jl:42-43,61): fvar = 2 sigmoid(x)
+ * - 1 + vel0, f = 1 / fvar (f may be NULL). */
+int adt_model3d_f64(const double *x, const double *vel0, int64_t N, double *fvar, double *f,
+                    void *stream);
+int adt_model3d_f32(const float *x, const float *vel0, int64_t N, float *fvar, float *f,
+                    void *stream);
+/* Its adjoint: grad_x = (grad_fvar - grad_f / fvar^2) * 2 sigmoid(x)
+ * (1 - sigmoid(x)); grad_f or grad_fvar may be NULL. */
+int adt_model3d_grad_f64(const double *x, const double *fvar, const double *grad_f,
+                         const double *grad_fvar, int64_t N, double *grad_x, void *stream);
+int adt_model3d_grad_f32(const float *x, const float *fvar, const float *grad_f,
+                         const float *grad_fvar, int64_t N, float *grad_x, void *stream);
+
+/* Smoothing regulariser (scripts/inversion.jl:107-121): value = lambda *
+ * sum |fvar - box(fvar)| with box = the mean over an sh x sh x sv window
+ * (odd sizes) with periodic wrap; grad_fvar = d value / d fvar (may be
+ * NULL).  value is a device scalar. */
+int adt_smooth3d_l1_f64(const double *fvar, int64_t m, int64_t n, int64_t l, int32_t sh,
+                        int32_t sv, double lambda, double *grad_fvar, double *value,
+                        void *stream);
+int adt_smooth3d_l1_f32(const float *fvar, int64_t m, int64_t n, int64_t l, int32_t sh,
+                        int32_t sv, float lambda, float *grad_fvar, float *value, void *stream);
+
+/* out = sum over k < narr of arrays[k] (host array of device pointers),
+ * in that order (e.g. dL/df over the stations of one rank). */
+int adt_sum_arrays_f64(const double *const *arrays, int32_t narr, int64_t N, double *out,
+                       void *stream);
+int adt_sum_arrays_f32(const float *const *arrays, int32_t narr, int64_t N, float *out,
+                       void *stream);
+
 /* ---- Diagnostics: device arithmetic of the path, for bit-exact checks
  * against host IEEE results.  op 0: sqrt(a); 1: a/3; 2: a/2; 3: a/b;
  * 4: 3D local solve (a = 3 values, b = f,h per element);

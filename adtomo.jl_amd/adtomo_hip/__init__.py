@@ -7,11 +7,12 @@ The objective around the solves (scripts/inversion.jl) is in
 ``adtomo_hip.objective``; the inversion driver in ``adtomo_hip.inversion``.
 """
 from . import objective
-from .ops import (EikonalGraph, eikonal, eikonal3d, eikonal3d_batched, eikonal3d_grad,
-                  eikonal3d_grad_plain, eikonal3d_plain, eikonal_grad, last_info,
-                  selftest_arith)
+from .ops import (EikonalGraph, autotune, eikonal, eikonal3d, eikonal3d_batched, eikonal3d_grad,
+                  eikonal3d_grad_plain, eikonal3d_plain, eikonal_grad, get_tune, last_info,
+                  selftest_arith, set_tune)
 from ._lib import LIB_PATH, AdtError, lib
 
 __all__ = ["eikonal", "eikonal_grad", "eikonal3d", "eikonal3d_grad", "eikonal3d_batched",
            "EikonalGraph", "eikonal3d_plain", "eikonal3d_grad_plain", "selftest_arith",
-           "last_info", "lib", "LIB_PATH", "AdtError", "objective"]
+           "last_info", "lib", "LIB_PATH", "AdtError", "objective", "autotune", "get_tune",
+           "set_tune"]

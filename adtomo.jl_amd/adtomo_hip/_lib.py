@@ -70,7 +70,16 @@ class AdtMisfit3d(ctypes.Structure):
         ("m", _i64), ("n", _i64), ("l", _i64),
         ("u", _p), ("tobs", _p), ("weight", _p),
         ("tcal", _p), ("grad_u", _p), ("loss", _p),
+        ("huber_delta", _d),
     ]
+
+
+class AdtTune(ctypes.Structure):
+    _fields_ = [("diag_max_l", _i32), ("adj_wg", _i32), ("order", _i32),
+                ("fwd_ms", ctypes.c_float), ("adj_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class AdtError(RuntimeError):
@@ -93,6 +102,8 @@ _SIGS = {
     "adt_model3d_grad": [_p, _p, _p, _p, _i64, _p, _p],
     "adt_smooth3d_l1": [_p, _i64, _i64, _i64, _i32, _i32, "R", _p, _p, _p],
     "adt_sum_arrays": [_p, _i32, _i64, _p, _p],
+    "adt_slowness3d": [_p, _i64, "R", _p, _p],
+    "adt_slowness3d_grad": [_p, _i64, "R", _p, _p, _p, _p],
 }
 
 
@@ -122,6 +133,12 @@ def lib():
     L.adt_graph_infos.restype = _i32
     L.adt_graph_destroy.argtypes = [_p]
     L.adt_graph_destroy.restype = _i32
+    L.adt_get_tune.argtypes = [_i32, _p]
+    L.adt_get_tune.restype = _i32
+    L.adt_set_tune.argtypes = [_i32, _p]
+    L.adt_set_tune.restype = _i32
+    L.adt_autotune.argtypes = [_p, _i32, _i32, _i32, _i32, _p, _p]
+    L.adt_autotune.restype = _i32
     L.adt_version.restype = ctypes.c_char_p
     L.adt_last_error.restype = ctypes.c_char_p
     _lib = L

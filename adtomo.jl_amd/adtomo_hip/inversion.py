@@ -1,23 +1,32 @@
 """Travel-time tomography driver: the inversion loop of scripts/inversion.jl
-on the device, with a host L-BFGS and periodic checkpoints
-(src/mpi_optimize.jl:3-72).
+(P only) and scripts/inversion_joint.jl (joint P/S) on the device, with a
+host L-BFGS and periodic checkpoints (src/mpi_optimize.jl:3-72).
 
 One objective evaluation, all on the GPU except the optimiser state:
 
-    fvar, f   = model(x)                      scripts/inversion.jl:42-43,61
-    u_s       = eikonal3d(u0_s, f)            :47-62 (one batched launch)
-    loss_s, dL/du_s = misfit(u_s, events)     :64-105
+    fvar, f   = model(x)                      inversion.jl:42-43,61
+    f_S       = vpvs / fvar                   inversion_joint.jl:79 (joint only)
+    u_s       = eikonal3d(u0_s, f_phase)      inversion.jl:47-62 (batched launches)
+    loss_s, dL/du_s = misfit(u_s, events)     inversion.jl:64-105
     dL/df_s   = adjoint(u_s, dL/du_s)         (the EikonalThreeDGrad op)
-    reg, dR/dfvar = smooth_l1(fvar)           :107-121
-    loss = sum_s loss_s + reg,  dL/dx = model_grad(...)
+    reg, dR/dfvar = smooth_l1(fvar)           inversion.jl:107-121
+    loss = sum_s loss_s + reg,  dL/dx (and dL/dvpvs) by the chain rule
 
-The u0_s depend on the fixed starting model vel0 only (:48-60) and are
-built once.  The optimiser is SciPy's L-BFGS-B on the flattened x (the
+Stations are solved ``chunk`` at a time: each chunk's forward, misfit and
+adjoint run back to back over one set of slot buffers, and the per-station
+dL/df are summed into one accumulator, so device memory holds ``chunk``
+travel-time fields (plus their adjoint workspace), not one per station.
+Without chunking (``chunk`` = number of stations) the initial times u0_s,
+which depend on the fixed starting model only (inversion.jl:48-60), are
+built once; with chunking they are rebuilt on the device per chunk.
+
+The optimiser is SciPy's L-BFGS-B on the flattened parameters (the
 reference uses Optim.jl's LBFGS with a backtracking line search, which is
-not available here); the objective and its gradient are the reference's.
-Checkpoints: every ``steps`` gradient evaluations, x is saved as
-``iter_<k>.npy`` under ``checkpoint_dir`` (the reference writes
-``iter_<k>.h5``, src/mpi_optimize.jl:24-28).
+not available here); objective and gradient are the reference's.
+Checkpoints: every ``steps`` gradient evaluations the parameters are saved
+as ``iter_<k>.npz`` (keys ``x`` and, for the joint inversion, ``vpvs``)
+under ``checkpoint_dir`` (the reference writes ``iter_<k>.h5``,
+src/mpi_optimize.jl:24-28).
 """
 from __future__ import annotations
 
@@ -30,72 +39,147 @@ from .ops import EikonalGraph, _torch
 
 
 class TravelTimeInversion:
-    """P-wave travel-time inversion for one set of stations (sources)."""
+    """Travel-time inversion for a set of stations (the sources, by
+    reciprocity) and events (the receivers).
+
+    ``tobs`` / ``weights``: (nstation, nevent) arrays for P only, or dicts
+    {"P": ..., "S": ...} with ``phases=("P", "S")``; -1 marks a missing pick.
+    ``vpvs``: the starting vp/vs ratio of the joint inversion; it scales the
+    S initial times (fixed, inversion_joint.jl:65-73) and is a trainable
+    parameter of the S slowness (inversion_joint.jl:50-51,79).
+    ``huber_delta`` > 0 uses the Huber residual instead of the square.
+    """
 
     def __init__(self, vel0, h, stations, events, tobs, weights, smooth=(3, 3), lam=0.0,
-                 tol=1e-3, dtype="float64"):
+                 tol=1e-3, dtype="float64", phases=("P",), vpvs=1.7, huber_delta=0.0,
+                 chunk=None):
         torch = _torch()
         self.torch = torch
         self.dt = getattr(torch, dtype)
         dev = torch.device("cuda")
-        self.shape = tuple(vel0.shape)
+        self.shape = tuple(int(s) for s in vel0.shape)
+        self.N = int(np.prod(self.shape))
         self.h = float(h)
         self.vel0 = torch.as_tensor(np.ascontiguousarray(vel0), dtype=self.dt, device=dev)
         self.stations = [tuple(map(float, s)) for s in stations]
         ns = len(self.stations)
-        ex, ey, ez = (np.ascontiguousarray(a, dtype=np.float64) for a in events)
-        self.events = (ex, ey, ez)
-        tobs = np.asarray(tobs, dtype=np.float64).reshape(ns, len(ex))
-        weights = np.asarray(weights, dtype=np.float64).reshape(ns, len(ex))
-        self.tobs = [torch.as_tensor(tobs[s], dtype=self.dt, device=dev) for s in range(ns)]
-        self.weights = [torch.as_tensor(weights[s], dtype=self.dt, device=dev) for s in range(ns)]
+        self.events = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in events)
+        ne = len(self.events[0])
+        self.phases = tuple(phases)
+        if any(p not in ("P", "S") for p in self.phases) or not self.phases:
+            raise ValueError("phases must be a non-empty subset of ('P', 'S')")
+        self.joint = "S" in self.phases
+        if not isinstance(tobs, dict):
+            tobs, weights = {self.phases[0]: tobs}, {self.phases[0]: weights}
+        self.vpvs0 = float(vpvs)
+        self.huber = float(huber_delta)
         self.smooth = smooth
         self.lam = float(lam)
-        # fixed buffers shared by the captured graphs
+        self.tobs, self.weights = {}, {}
+        for p in self.phases:
+            t = np.asarray(tobs[p], dtype=np.float64).reshape(ns, ne)
+            w = np.asarray(weights[p], dtype=np.float64).reshape(ns, ne)
+            self.tobs[p] = [torch.as_tensor(t[s], dtype=self.dt, device=dev) for s in range(ns)]
+            self.weights[p] = [torch.as_tensor(w[s], dtype=self.dt, device=dev) for s in range(ns)]
+        # model buffers
         self.x = torch.zeros(self.shape, dtype=self.dt, device=dev)
         self.fvar = torch.empty_like(self.x)
-        self.f = torch.empty_like(self.x)
-        self.u0s = O.source_u0(self.stations, self.vel0, self.h)
-        self.grad_us = [torch.empty_like(self.x) for _ in range(ns)]
-        fs = [self.f] * ns
-        self.fwd = EikonalGraph(self.u0s, fs, [self.h] * ns, [tol] * ns)
-        self.adj = EikonalGraph(self.u0s, fs, [self.h] * ns, [tol] * ns, grad_us=self.grad_us,
-                                forward=False, us=self.fwd.us)
+        self.f = {"P": torch.empty_like(self.x)}
+        if self.joint:
+            self.f["S"] = torch.empty_like(self.x)
+        self.grad_f = {p: torch.empty_like(self.x) for p in self.phases}
+        # solve slots: `chunk` stations per batched launch and phase
+        self.chunk = ns if chunk is None else max(1, min(int(chunk), ns))
+        C = self.chunk
+        self.slots = {}
+        for p in self.phases:
+            u0s = [torch.empty_like(self.x) for _ in range(C)]
+            gus = [torch.empty_like(self.x) for _ in range(C)]
+            fs = [self.f[p]] * C
+            fwd = EikonalGraph(u0s, fs, [self.h] * C, [tol] * C)
+            adj = EikonalGraph(u0s, fs, [self.h] * C, [tol] * C, grad_us=gus, forward=False,
+                               us=fwd.us)
+            self.slots[p] = {"u0": u0s, "grad_u": gus, "fwd": fwd, "adj": adj}
+            if C == ns:
+                O.source_u0(self.stations, self.vel0, self.h, self._u0_scale(p), out=u0s)
         self.nevals = 0
+        self.last_loss = {}
 
-    def _load_x(self, xflat):
-        self.x.copy_(self.torch.as_tensor(np.asarray(xflat, dtype=np.float64).reshape(self.shape),
-                                          dtype=self.dt))
+    def _u0_scale(self, phase):
+        return 1.0 if phase == "P" else self.vpvs0
 
-    def objective(self, xflat, grad=True):
-        """(loss, dL/dx flattened) at model parameters x."""
+    @property
+    def nparams(self):
+        return self.N + (1 if self.joint else 0)
+
+    def initial_params(self):
+        """x = 0 (the starting model) and, for the joint inversion, vpvs."""
+        x0 = np.zeros(self.nparams)
+        if self.joint:
+            x0[-1] = self.vpvs0
+        return x0
+
+    def _solve_phase(self, phase):
+        """Forward, misfit and adjoint of every station of one phase; returns
+        the summed misfit and leaves sum_s dL/df_s in self.grad_f[phase]."""
+        S = self.slots[phase]
+        C, ns = self.chunk, len(self.stations)
+        loss = 0.0
+        for start in range(0, ns, C):
+            k = min(C, ns - start)
+            if C < ns:
+                O.source_u0(self.stations[start:start + k], self.vel0, self.h,
+                            self._u0_scale(phase), out=S["u0"][:k])
+            S["fwd"].launch()
+            mis = O.misfit(S["fwd"].us[:k], *self.events, self.tobs[phase][start:start + k],
+                           self.weights[phase][start:start + k], grad=True, tcal=False,
+                           grad_out=S["grad_u"][:k], huber_delta=self.huber)
+            loss += float(mis["loss"].cpu().numpy().sum())
+            S["adj"].launch()
+            parts = S["adj"].grad_fs[:k]
+            O.sum_arrays(parts if start == 0 else [self.grad_f[phase]] + parts,
+                         out=self.grad_f[phase])
+        return loss
+
+    def objective(self, params, grad=True):
+        """(loss, d loss / d params) at the flattened parameters."""
         torch = self.torch
-        self._load_x(xflat)
-        fvar, f = O.model(self.x, self.vel0)
-        self.fvar.copy_(fvar)
-        self.f.copy_(f)
-        self.fwd.launch()
-        mis = O.misfit(self.fwd.us, *self.events, self.tobs, self.weights, grad=grad, tcal=False,
-                       grad_out=self.grad_us if grad else None)
-        loss = float(mis["loss"].sum().item())
-        reg_val, reg_grad = O.smooth_l1(self.fvar, self.smooth[0], self.smooth[1], self.lam,
-                                        grad=grad)
-        loss += float(reg_val.item())
+        params = np.asarray(params, dtype=np.float64)
+        self.x.copy_(torch.as_tensor(params[:self.N].reshape(self.shape), dtype=self.dt))
+        vpvs = float(params[-1]) if self.joint else None
+        O.model(self.x, self.vel0, out=(self.fvar, self.f["P"]))
+        if self.joint:
+            O.slowness(self.fvar, vpvs, out=self.f["S"])
+        loss = 0.0
+        for p in self.phases:
+            self.last_loss[p] = self._solve_phase(p)
+            loss += self.last_loss[p]
+        reg_val, grad_fvar = O.smooth_l1(self.fvar, self.smooth[0], self.smooth[1], self.lam)
+        self.last_loss["reg"] = float(reg_val.cpu().numpy()[0])
+        loss += self.last_loss["reg"]
         self.nevals += 1
         if not grad:
             return loss
-        self.adj.launch()
-        grad_f = O.sum_arrays(self.adj.grad_fs)
-        gx = O.model_grad(self.x, self.fvar, grad_f=grad_f, grad_fvar=reg_grad)
-        torch.cuda.synchronize()
-        return loss, gx.double().cpu().numpy().reshape(-1)
+        g = np.empty(self.nparams)
+        if self.joint:
+            gv = O.slowness_grad(self.fvar, vpvs, self.grad_f["S"], grad_fvar=grad_fvar)
+            g[-1] = float(gv.cpu().numpy()[0])
+        gf = self.grad_f["P"] if "P" in self.phases else None
+        gx = O.model_grad(self.x, self.fvar, grad_f=gf, grad_fvar=grad_fvar)
+        g[:self.N] = gx.cpu().numpy().reshape(-1)
+        return loss, g
 
-    def travel_times(self):
-        """Travel times of the last forward, one (m, n, l) tensor per station."""
-        return self.fwd.us
+    def travel_times(self, phase="P"):
+        """Travel times of the last chunk's forward, one (m, n, l) tensor per slot."""
+        return self.slots[phase]["fwd"].us
+
+    def save_checkpoint(self, path, params):
+        params = np.asarray(params)
+        extra = {"vpvs": params[-1]} if self.joint else {}
+        np.savez(path, x=params[:self.N].reshape(self.shape), **extra)
 
     def run(self, iterations=10, steps=None, checkpoint_dir=None, x0=None):
-        """L-BFGS over x; returns (x, loss history)."""
+        """L-BFGS over the parameters; returns (params, loss history)."""
         from scipy.optimize import minimize
         history = []
         calls = {"grad": 0}
@@ -106,14 +190,14 @@ class TravelTimeInversion:
             calls["grad"] += 1
             if checkpoint_dir and steps and calls["grad"] % steps == 0:
                 os.makedirs(checkpoint_dir, exist_ok=True)
-                np.save(os.path.join(checkpoint_dir, f"iter_{calls['grad']}.npy"),
-                        np.asarray(xf).reshape(self.shape))
+                self.save_checkpoint(os.path.join(checkpoint_dir, f"iter_{calls['grad']}.npz"), xf)
             return L, g
 
-        x0 = np.zeros(int(np.prod(self.shape))) if x0 is None else np.asarray(x0).reshape(-1)
+        x0 = self.initial_params() if x0 is None else np.asarray(x0, dtype=np.float64).reshape(-1)
         res = minimize(fun, x0, jac=True, method="L-BFGS-B", options={"maxiter": iterations})
-        return res.x.reshape(self.shape), history
+        return res.x, history
 
     def close(self):
-        self.fwd.close()
-        self.adj.close()
+        for S in self.slots.values():
+            S["fwd"].close()
+            S["adj"].close()

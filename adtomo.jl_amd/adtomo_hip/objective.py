@@ -8,6 +8,7 @@ Mirrors scripts/inversion.jl (the caller of src/eikonal_op.jl):
                          its adjoint source dL/du for the Eikonal adjoint
   model         :42-43   fvar = 2 sigmoid(x) - 1 + vel0, f = 1 / fvar
   model_grad             chain rule back to x
+  slowness               f_s = vp/vs / fvar (scripts/inversion_joint.jl:79)
   smooth_l1     :107-121 lambda * sum |fvar - box(fvar)| (periodic box mean)
   sum_arrays             sum of per-station gradients (fixed order)
 
@@ -53,13 +54,15 @@ def source_u0(stations, vel0, h: float, scale: float = 1.0, out=None):
     return out
 
 
-def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None):
+def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None, huber_delta=0.0):
     """Receiver gather + misfit for each station.
 
     ``us``: list of travel-time tensors; ``rx, ry, rz``: event coordinates
     (host sequences, 1-based); ``tobs``/``weights``: per-station tensors of
     length len(rx).  ``grad_out``: optional list of tensors to receive
-    dL/du.  Returns dict(loss=tensor[nstation], tcal=[...], grad_u=[...])."""
+    dL/du.  ``huber_delta`` > 0 replaces the squared residual by the Huber
+    form (r^2 inside +-delta, 2 delta |r| - delta^2 outside).  Returns
+    dict(loss=tensor[nstation], tcal=[...], grad_u=[...])."""
     torch = _torch()
     ns = len(us)
     nrec = len(rx)
@@ -78,7 +81,8 @@ def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None):
         m, n, l = u.shape
         tab[e] = L.AdtMisfit3d(m, n, l, u.data_ptr(), tobs[e].data_ptr(), weights[e].data_ptr(),
                                tcals[e].data_ptr() if tcal else None,
-                               grads[e].data_ptr() if grad else None, loss.data_ptr() + e * esz)
+                               grads[e].data_ptr() if grad else None, loss.data_ptr() + e * esz,
+                               float(huber_delta))
     ax = np.ascontiguousarray(rx, dtype=np.float64)
     ay = np.ascontiguousarray(ry, dtype=np.float64)
     az = np.ascontiguousarray(rz, dtype=np.float64)
@@ -89,12 +93,17 @@ def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None):
     return {"loss": loss, "tcal": tcals, "grad_u": grads}
 
 
-def model(x, vel0, slowness=True):
-    """(fvar, f): fvar = 2 sigmoid(x) - 1 + vel0, f = 1/fvar."""
+def model(x, vel0, slowness=True, out=None):
+    """(fvar, f): fvar = 2 sigmoid(x) - 1 + vel0, f = 1/fvar.  ``out``: an
+    optional (fvar, f) pair of tensors to fill."""
     torch = _torch()
     _check_dev(x, vel0)
-    fvar = torch.empty_like(x)
-    f = torch.empty_like(x) if slowness else None
+    if out is not None:
+        fvar, f = out
+        _check_dev(fvar, f)
+    else:
+        fvar = torch.empty_like(x)
+        f = torch.empty_like(x) if slowness else None
     fn = getattr(L.lib(), "adt_model3d" + _sfx(x.dtype))
     L.check(fn(_ptr(x), _ptr(vel0), x.numel(), _ptr(fvar), _ptr(f), _stream()), "model3d")
     return fvar, f
@@ -109,6 +118,28 @@ def model_grad(x, fvar, grad_f=None, grad_fvar=None):
     L.check(fn(_ptr(x), _ptr(fvar), _ptr(grad_f), _ptr(grad_fvar), x.numel(), _ptr(gx), _stream()),
             "model3d_grad")
     return gx
+
+
+def slowness(fvar, scale: float, out=None):
+    """f_s = scale / fvar (the S slowness pvs ./ fvar)."""
+    torch = _torch()
+    _check_dev(fvar, out)
+    out = torch.empty_like(fvar) if out is None else out
+    fn = getattr(L.lib(), "adt_slowness3d" + _sfx(fvar.dtype))
+    L.check(fn(_ptr(fvar), fvar.numel(), float(scale), _ptr(out), _stream()), "slowness3d")
+    return out
+
+
+def slowness_grad(fvar, scale: float, grad_fs, grad_fvar=None, grad_scale=True):
+    """Adjoint of ``slowness``: adds -scale grad_fs / fvar^2 to ``grad_fvar``
+    in place (when given) and returns d/dscale as a device scalar (or None)."""
+    torch = _torch()
+    _check_dev(fvar, grad_fs, grad_fvar)
+    gs = torch.zeros(1, dtype=fvar.dtype, device=fvar.device) if grad_scale else None
+    fn = getattr(L.lib(), "adt_slowness3d_grad" + _sfx(fvar.dtype))
+    L.check(fn(_ptr(fvar), fvar.numel(), float(scale), _ptr(grad_fs), _ptr(grad_fvar), _ptr(gs),
+               _stream()), "slowness3d_grad")
+    return gs
 
 
 def smooth_l1(fvar, sh: int, sv: int, lam: float, grad=True):

@@ -281,6 +281,47 @@ def eikonal3d_batched(u0s, fs, hs, tols, grad_us=None, forward=True, us=None,
             "info": [infos[e].as_dict() for e in range(nev)] if infos is not None else None}
 
 
+def _dtype_code(dtype):
+    return 1 if str(dtype).endswith("float32") else 0
+
+
+def get_tune(dtype="float64"):
+    """Current launch setting of the 3D kernels for one precision."""
+    t = L.AdtTune()
+    L.check(L.lib().adt_get_tune(_dtype_code(dtype), ctypes.byref(t)), "get_tune")
+    return t.as_dict()
+
+
+def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None):
+    """Change the launch setting (unspecified fields keep their value)."""
+    t = L.AdtTune()
+    L.check(L.lib().adt_get_tune(_dtype_code(dtype), ctypes.byref(t)), "get_tune")
+    if diag_max_l is not None:
+        t.diag_max_l = int(diag_max_l)
+    if adj_wg is not None:
+        t.adj_wg = int(adj_wg)
+    if order is not None:
+        t.order = int(order)
+    L.check(L.lib().adt_set_tune(_dtype_code(dtype), ctypes.byref(t)), "set_tune")
+    return t.as_dict()
+
+
+def autotune(u0s, fs, hs, tols, grad_us=None, reps=3):
+    """Time the launch settings on these events and keep the fastest for
+    their precision (forward kernel split, adjoint workgroup size, event
+    order).  Returns the chosen setting with its measured times."""
+    torch = _torch()
+    us = [torch.empty_like(f) for f in fs]
+    gu0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
+    gfs = [torch.empty_like(f) for f in fs] if grad_us is not None else None
+    tab, dtype = _event_table(u0s, fs, hs, tols, grad_us, us, gu0s, gfs)
+    flags = L.ADT_FORWARD | (L.ADT_ADJOINT if grad_us is not None else 0)
+    best = L.AdtTune()
+    L.check(L.lib().adt_autotune(ctypes.byref(tab), len(fs), flags, _dtype_code(dtype), int(reps),
+                                 ctypes.byref(best), _stream()), "autotune")
+    return best.as_dict()
+
+
 class EikonalGraph:
     """hipGraph of the batched forward and/or adjoint over fixed buffers.
 

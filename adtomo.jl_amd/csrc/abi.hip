@@ -6,9 +6,11 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <list>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.hpp"
@@ -59,6 +61,27 @@ struct DevBuf {
     return ADT_OK;
   }
 };
+
+// ------------------------------------------------------------ tuning ---
+// Launch settings of the 3D kernels, one set per precision ([0] FP64,
+// [1] FP32), read at each launch / graph creation and written by
+// adt_set_tune / adt_autotune under a mutex.
+std::mutex g_tune_mu;
+adt_tune g_tune[2] = {{adt::kDiagMaxL, 1024, 1, 0.f, 0.f}, {adt::kDiagMaxL, 1024, 1, 0.f, 0.f}};
+
+template <typename T>
+constexpr int tune_index() {
+  return std::is_same<T, float>::value ? 1 : 0;
+}
+template <typename T>
+adt_tune current_tune() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_tune[tune_index<T>()];
+}
+bool tune_valid(const adt_tune& t) {
+  return t.diag_max_l >= 0 && t.diag_max_l <= adt::kDiagMaxL &&
+         (t.adj_wg == 256 || t.adj_wg == 512 || t.adj_wg == 1024) && (t.order == 0 || t.order == 1);
+}
 
 int check_dims3(int64_t m, int64_t n, int64_t l) {
   if (m < 2 || n < 2 || l < 2)
@@ -148,11 +171,13 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   d.uold = base;
   d.info = (adt_info*)(base + off_info);
   ADT_HIP(hipMemcpyAsync(base + off_desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
-  if (l <= adt::kDiagMaxL)
-    adt::fwd3_diag_kernel<T><<<1, kDiagWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3);
+  const adt_tune tn = current_tune<T>();
+  if (l <= tn.diag_max_l)
+    adt::fwd3_diag_kernel<T><<<1, kDiagWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3,
+                                                  tn.diag_max_l);
   else
     adt::fwd3_stream_kernel<T><<<1, kStreamWG, 0, s>>>((const adt::EventDesc*)(base + off_desc),
-                                                      kMaxIter3, adt::kDiagMaxL);
+                                                      kMaxIter3, tn.diag_max_l);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -183,7 +208,7 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
       (rc = st.out(grad_u0, bytes, hp, &dgu0, s)) || (rc = st.out(grad_f, bytes, hp, &dgf, s)))
     return rc;
   DevBuf ws;
-  const size_t a = ((4 * bytes + 255) / 256) * 256;  // per-node records (4 values)
+  const size_t a = ((adt::kAdjRecBytes<T> * size_t(N) + 255) / 256) * 256;  // per-node records
   const size_t c = ((size_t(N) * 2 + 255) / 256) * 256;  // 16-bit node states
   const size_t q = ((size_t(N) * 4 + 255) / 256) * 256;
   if ((rc = ws.alloc(a + c + q + 256 + sizeof(adt_info), s))) return rc;
@@ -206,7 +231,7 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   const size_t od = a + c + q;
   d.info = (adt_info*)(base + od + 256);
   ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
-  adt::adj3_kahn_kernel<T><<<1, kWG, 0, s>>>((const adt::EventDesc*)(base + od));
+  adt::adj3_kahn_kernel<T><<<1, current_tune<T>().adj_wg, 0, s>>>((const adt::EventDesc*)(base + od));
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -352,7 +377,7 @@ struct BatchPlan {
         return fail(ADT_EINVAL, "event " + std::to_string(e) + ": arrays of 4 GiB or more are not supported");
       if (x.l > max_l) max_l = (int)x.l;
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
-      o_lam[e] = (flags & ADT_ADJOINT) ? take(4 * N * sizeof(T)) : 0;  // per-node records
+      o_lam[e] = (flags & ADT_ADJOINT) ? take(N * adt::kAdjRecBytes<T>) : 0;  // per-node records
       o_code[e] = (flags & ADT_ADJOINT) ? take(2 * N) : 0;              // 16-bit node states
       o_indeg[e] = 0;
       o_queue[e] = (flags & ADT_ADJOINT) ? take(4 * N) : 0;
@@ -385,6 +410,14 @@ struct BatchPlan {
     }
     return ADT_OK;
   }
+  // Largest events first (by node count), so that the workgroups of the
+  // long solves start first and short ones fill in behind them.  The
+  // per-event info slots stay with their events.
+  void order_largest_first() {
+    std::stable_sort(descs.begin(), descs.end(), [](const adt::EventDesc& a, const adt::EventDesc& b) {
+      return (int64_t)a.m * a.n * a.l > (int64_t)b.m * b.n * b.l;
+    });
+  }
   void bind(char* base) {
     for (auto& d : descs) {
       d.uold = base + (size_t)d.uold;
@@ -398,19 +431,22 @@ struct BatchPlan {
 };
 
 template <typename T>
-int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, hipStream_t s) {
+int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, const adt_tune& tn,
+                 hipStream_t s) {
   if (nev == 0) return ADT_OK;
   if (flags & ADT_FORWARD) {
-    // events with l <= kDiagMaxL: diagonal streaming; longer ones: k-streaming
-    adt::fwd3_diag_kernel<T><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3);
-    ADT_HIP(hipGetLastError());
-    if (max_l > adt::kDiagMaxL) {
-      adt::fwd3_stream_kernel<T><<<nev, kStreamWG, 0, s>>>(ddesc, kMaxIter3, adt::kDiagMaxL);
+    // events with l <= diag_max_l: diagonal streaming; longer ones: k-streaming
+    if (tn.diag_max_l > 0) {
+      adt::fwd3_diag_kernel<T><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
+      ADT_HIP(hipGetLastError());
+    }
+    if (max_l > tn.diag_max_l) {
+      adt::fwd3_stream_kernel<T><<<nev, kStreamWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
       ADT_HIP(hipGetLastError());
     }
   }
   if (flags & ADT_ADJOINT) {
-    adt::adj3_kahn_kernel<T><<<nev, kWG, 0, s>>>(ddesc);
+    adt::adj3_kahn_kernel<T><<<nev, tn.adj_wg, 0, s>>>(ddesc);
     ADT_HIP(hipGetLastError());
   }
   return ADT_OK;
@@ -425,6 +461,8 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   int rc = plan.build(ev, nev, flags);
   if (rc) return rc;
   if (nev == 0) return ADT_OK;
+  const adt_tune tn = current_tune<T>();
+  if (tn.order) plan.order_largest_first();
   DevBuf ws;
   if ((rc = ws.alloc(plan.ws_bytes, s))) return rc;
   plan.bind((char*)ws.p);
@@ -432,7 +470,7 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   ADT_HIP(hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
                          hipMemcpyHostToDevice, s));
   if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags,
-                            plan.max_l, s)))
+                            plan.max_l, tn, s)))
     return rc;
   if (infos) {
     ADT_HIP(hipMemcpyAsync(infos, base + plan.off_info, sizeof(adt_info) * nev,
@@ -560,6 +598,117 @@ int bwd3_plain_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_
   return ADT_OK;
 }
 
+// -------------------------------------------------------------- autotune ---
+
+// Times the launch settings on the caller's events and keeps the fastest:
+// the forward kernel split (diagonal streaming up to diag_max_l, or
+// k-streaming for every event), the adjoint workgroup size, and the event
+// order.  Every setting computes the same results (the forward bit for bit,
+// the adjoint to rounding), so the caller's outputs hold a valid solve
+// afterwards.
+template <typename T>
+int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune* best,
+                  hipStream_t s) {
+  if (!(flags & (ADT_FORWARD | ADT_ADJOINT)))
+    return fail(ADT_EINVAL, "autotune: no ADT_FORWARD/ADT_ADJOINT flag");
+  if (flags & ADT_HOST_POINTERS) return fail(ADT_EINVAL, "autotune: device pointers only");
+  if (nev < 1) return fail(ADT_EINVAL, "autotune: no events");
+  reps = reps < 1 ? 1 : (reps > 50 ? 50 : reps);
+  const adt_tune cur = current_tune<T>();
+  int min_l = 1 << 30;
+  for (int e = 0; e < nev; ++e) min_l = std::min<int>(min_l, (int)ev[e].l);
+  std::vector<int> dmls = {adt::kDiagMaxL};
+  if (min_l <= adt::kDiagMaxL) dmls.push_back(0);  // k-streaming for every event
+  const int wgs[3] = {256, 512, 1024};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ADT_HIP(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(ADT_EHIP, std::string("autotune: hipEventCreate: ") + hipGetErrorString(he));
+  }
+  int rc = ADT_OK;
+  adt_tune win = cur;
+  float win_total = 3.4e38f;
+  for (int order = 0; order < 2 && rc == ADT_OK; ++order) {
+    BatchPlan<T> plan;
+    if ((rc = plan.build(ev, nev, flags))) break;
+    if (order) plan.order_largest_first();
+    DevBuf ws;
+    if ((rc = ws.alloc(plan.ws_bytes, s))) break;
+    plan.bind((char*)ws.p);
+    char* base = (char*)ws.p;
+    const adt::EventDesc* dd = (const adt::EventDesc*)(base + plan.off_desc);
+    he = hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
+                        hipMemcpyHostToDevice, s);
+    if (he != hipSuccess) {
+      rc = fail(ADT_EHIP, std::string("autotune: hipMemcpyAsync: ") + hipGetErrorString(he));
+      break;
+    }
+    // time `reps` launches of one setting after one untimed launch
+    auto time_it = [&](int fl, const adt_tune& t, float* ms) -> int {
+      int r = launch_batch<T>(dd, nev, fl, plan.max_l, t, s);
+      if (r) return r;
+      ADT_HIP(hipEventRecord(e0, s));
+      for (int k = 0; k < reps; ++k)
+        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, t, s))) return r;
+      ADT_HIP(hipEventRecord(e1, s));
+      ADT_HIP(hipEventSynchronize(e1));
+      ADT_HIP(hipEventElapsedTime(ms, e0, e1));
+      *ms /= (float)reps;
+      return ADT_OK;
+    };
+    adt_tune t = cur;
+    t.order = order;
+    float fbest = 0.f, abest = 0.f;
+    if (flags & ADT_FORWARD) {
+      fbest = 3.4e38f;
+      for (int dml : dmls) {
+        adt_tune c = t;
+        c.diag_max_l = dml;
+        float ms = 0.f;
+        if ((rc = time_it(ADT_FORWARD, c, &ms))) break;
+        if (ms < fbest) {
+          fbest = ms;
+          t.diag_max_l = dml;
+        }
+      }
+      if (rc) break;
+      // leave this order's best forward in the outputs for the adjoint
+      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD, plan.max_l, t, s))) break;
+    }
+    if (flags & ADT_ADJOINT) {
+      abest = 3.4e38f;
+      for (int wg : wgs) {
+        adt_tune c = t;
+        c.adj_wg = wg;
+        float ms = 0.f;
+        if ((rc = time_it(ADT_ADJOINT, c, &ms))) break;
+        if (ms < abest) {
+          abest = ms;
+          t.adj_wg = wg;
+        }
+      }
+      if (rc) break;
+    }
+    if (fbest + abest < win_total) {
+      win_total = fbest + abest;
+      win = t;
+      win.fwd_ms = fbest;
+      win.adj_ms = abest;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[tune_index<T>()] = win;
+  }
+  if (best) *best = win;
+  return ADT_OK;
+}
+
 // ----------------------------------------------------------------- graph ---
 
 struct GraphBase {
@@ -577,6 +726,8 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   BatchPlan<T> plan;
   int rc = plan.build(ev, nev, flags);
   if (rc) return rc;
+  const adt_tune tn = current_tune<T>();
+  if (tn.order) plan.order_largest_first();
   GraphBase* g = new GraphBase();
   g->nev = nev;
   g->off_info = plan.off_info;
@@ -598,7 +749,7 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, cs);
+    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, tn, cs);
     hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
     if (rc == ADT_OK && e2 != hipSuccess) e = e2;
   }
@@ -626,6 +777,27 @@ extern "C" {
 
 const char* adt_version(void) { return "adtomo_hip 0.1.0 (gfx950)"; }
 const char* adt_last_error(void) { return g_err.c_str(); }
+
+int adt_get_tune(int32_t dtype, adt_tune* out) {
+  if ((dtype != 0 && dtype != 1) || !out) return fail(ADT_EINVAL, "get_tune: bad arguments");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  *out = g_tune[dtype];
+  return ADT_OK;
+}
+int adt_set_tune(int32_t dtype, const adt_tune* t) {
+  if ((dtype != 0 && dtype != 1) || !t) return fail(ADT_EINVAL, "set_tune: bad arguments");
+  if (!tune_valid(*t))
+    return fail(ADT_EINVAL, "set_tune: diag_max_l must be in [0, 64], adj_wg 256/512/1024, order 0/1");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[dtype] = *t;
+  return ADT_OK;
+}
+int adt_autotune(const adt_event3d* events, int32_t nev, int32_t flags, int32_t dtype,
+                 int32_t reps, adt_tune* best, void* stream) {
+  if (dtype == 0) return autotune_impl<double>(events, nev, flags, reps, best, (hipStream_t)stream);
+  if (dtype == 1) return autotune_impl<float>(events, nev, flags, reps, best, (hipStream_t)stream);
+  return fail(ADT_EINVAL, "autotune: dtype must be 0 (FP64) or 1 (FP32)");
+}
 
 int adt_eikonal2d_fwd_f64(const double* f, int64_t m, int64_t n, int64_t srcx, int64_t srcy,
                           double h, double* u, adt_info* info, int32_t flags, void* stream) {
@@ -852,6 +1024,36 @@ int smooth_impl(const T* fvar, int64_t m, int64_t n, int64_t l, int32_t sh, int3
 }
 
 template <typename T>
+int slowness_impl(const T* fvar, int64_t N, T scale, T* f_s, void* stream) {
+  if (N < 0 || (N > 0 && (!fvar || !f_s))) return fail(ADT_EINVAL, "slowness3d: bad arguments");
+  if (N == 0) return ADT_OK;
+  adt::slowness_kernel<T><<<grid_for(N), 256, 0, (hipStream_t)stream>>>(fvar, N, scale, f_s);
+  ADT_HIP(hipGetLastError());
+  return ADT_OK;
+}
+
+template <typename T>
+int slowness_grad_impl(const T* fvar, int64_t N, T scale, const T* grad_fs, T* grad_fvar,
+                       T* grad_scale, void* stream) {
+  if (N < 0 || (N > 0 && (!fvar || !grad_fs)))
+    return fail(ADT_EINVAL, "slowness3d_grad: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = grid_for(N);
+  DevBuf ws;
+  int rc;
+  if ((rc = ws.alloc(sizeof(T) * nb, st))) return rc;
+  if (N > 0) {
+    adt::slowness_grad_kernel<T><<<nb, 256, 0, st>>>(fvar, N, scale, grad_fs, grad_fvar, (T*)ws.p);
+    ADT_HIP(hipGetLastError());
+  }
+  if (grad_scale) {
+    adt::sum_partials_kernel<T><<<1, 64, 0, st>>>((const T*)ws.p, N > 0 ? nb : 0, grad_scale);
+    ADT_HIP(hipGetLastError());
+  }
+  return ADT_OK;
+}
+
+template <typename T>
 int sum_arrays_impl(const T* const* arrays, int32_t narr, int64_t N, T* out, void* stream) {
   if (narr < 1 || N < 0 || !arrays || !out) return fail(ADT_EINVAL, "sum_arrays: bad arguments");
   for (int k = 0; k < narr; ++k)
@@ -912,6 +1114,20 @@ int adt_smooth3d_l1_f64(const double* fvar, int64_t m, int64_t n, int64_t l, int
 int adt_smooth3d_l1_f32(const float* fvar, int64_t m, int64_t n, int64_t l, int32_t sh,
                         int32_t sv, float lambda, float* grad_fvar, float* value, void* stream) {
   return smooth_impl<float>(fvar, m, n, l, sh, sv, lambda, grad_fvar, value, stream);
+}
+int adt_slowness3d_f64(const double* fvar, int64_t N, double scale, double* f_s, void* stream) {
+  return slowness_impl<double>(fvar, N, scale, f_s, stream);
+}
+int adt_slowness3d_f32(const float* fvar, int64_t N, float scale, float* f_s, void* stream) {
+  return slowness_impl<float>(fvar, N, scale, f_s, stream);
+}
+int adt_slowness3d_grad_f64(const double* fvar, int64_t N, double scale, const double* grad_fs,
+                            double* grad_fvar, double* grad_scale, void* stream) {
+  return slowness_grad_impl<double>(fvar, N, scale, grad_fs, grad_fvar, grad_scale, stream);
+}
+int adt_slowness3d_grad_f32(const float* fvar, int64_t N, float scale, const float* grad_fs,
+                            float* grad_fvar, float* grad_scale, void* stream) {
+  return slowness_grad_impl<float>(fvar, N, scale, grad_fs, grad_fvar, grad_scale, stream);
 }
 int adt_sum_arrays_f64(const double* const* arrays, int32_t narr, int64_t N, double* out,
                        void* stream) {

@@ -64,6 +64,13 @@ template <typename T>
 __global__ void smooth_pass2_kernel(const T* sgn, int m, int n, int l, int rh, int rv, T wgt,
                                     T lambda, T* grad, const T* partial, int npart, T* value);
 template <typename T>
+__global__ void slowness_kernel(const T* fvar, int64_t N, T scale, T* f_s);
+template <typename T>
+__global__ void slowness_grad_kernel(const T* fvar, int64_t N, T scale, const T* grad_fs,
+                                     T* grad_fvar, T* partial);
+template <typename T>
+__global__ void sum_partials_kernel(const T* partial, int npart, T* out);
+template <typename T>
 __global__ void model_grad_kernel(const T* x, const T* fvar, const T* grad_f, const T* grad_fvar,
                                   int64_t N, T* grad_x);
 
@@ -74,10 +81,14 @@ template <typename T>
 __global__ void fwd3_event_kernel(const EventDesc* evs, int max_iter);
 template <typename T>
 __global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter, int min_l);
+// Bytes of one node's record in the 3D adjoint workspace (lambda and three
+// coefficients).
+template <typename T>
+constexpr size_t kAdjRecBytes = 4 * sizeof(T);
 // Largest k extent handled by fwd3_diag_kernel (8 waves x 8 nodes).
 constexpr int kDiagMaxL = 64;
 template <typename T>
-__global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter);
+__global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
 template <typename T>
 __global__ void copy_kernel(T* dst, const T* src, int64_t N);
 template <typename T>

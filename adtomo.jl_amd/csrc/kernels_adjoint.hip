@@ -46,10 +46,12 @@ __device__ __forceinline__ unsigned dec_byte(uint8_t* base, int64_t idx) {
 //            with 7, so releases (at most 6) never make them ready.
 // Releasing a neighbour is one 32-bit atomic on the word holding its meta.
 template <typename T>
-struct alignas(4 * sizeof(T)) Rec {
+struct alignas(kAdjRecBytes<T>) Rec {
   T lam;
   T w[3];
 };
+static_assert(sizeof(Rec<double>) == kAdjRecBytes<double>, "record size");
+static_assert(sizeof(Rec<float>) == kAdjRecBytes<float>, "record size");
 
 // Coordinates (i, j, k) of node p, advanced by a fixed stride without
 // divisions (a grid-stride loop over the flat index).

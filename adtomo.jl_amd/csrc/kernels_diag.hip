@@ -537,14 +537,14 @@ __device__ __forceinline__ void diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr
 
 template <typename T>
 __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __restrict__ evs,
-                                                             int max_iter) {
+                                                             int max_iter, int max_l) {
   __shared__ T xch[2 * kDW * 64];
   __shared__ T xold[2 * kDW * 64];
   __shared__ T planes[kDW * 3 * Pieces<T>::PLANE];
   __shared__ uint64_t red[kDW];
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
-  if (l > kDW * kLW) return;  // handled by the k-streaming kernel
+  if (l > max_l || l > kDW * kLW) return;  // handled by the k-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
   {  // u = u0 (Eikonal3D.cpp:72)
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   }
 }
 
-template __global__ void fwd3_diag_kernel<double>(const EventDesc*, int);
-template __global__ void fwd3_diag_kernel<float>(const EventDesc*, int);
+template __global__ void fwd3_diag_kernel<double>(const EventDesc*, int, int);
+template __global__ void fwd3_diag_kernel<float>(const EventDesc*, int, int);
 
 }  // namespace adt

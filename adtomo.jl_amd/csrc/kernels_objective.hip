@@ -105,8 +105,9 @@ __device__ __forceinline__ void trilinear_adjoint(const Trilinear<T>& t, T g, T*
 }
 
 // One workgroup per solve (station).  loss = sum over receivers with
-// tobs != -1 of w * (tobs - t)^2 (scripts/inversion.jl:97-105), summed per
-// thread in receiver order and then over threads in a fixed tree order;
+// tobs != -1 of w * rho(tobs - t), rho(r) = r^2 (scripts/inversion.jl:97-105)
+// or, with huber_delta > 0, r^2 inside [-delta, delta] and 2 delta |r| -
+// delta^2 outside (continuous with its derivative), summed per thread in receiver order and then over threads in a fixed tree order;
 // grad_u = d loss / d u, zero except at the interpolation nodes.
 template <typename T>
 __global__ void __launch_bounds__(256) misfit_kernel(const MisfitDesc* __restrict__ d,
@@ -120,6 +121,7 @@ __global__ void __launch_bounds__(256) misfit_kernel(const MisfitDesc* __restric
   const T* __restrict__ w = (const T*)s.weight;
   T* grad_u = (T*)s.grad_u;
   T* tcal = (T*)s.tcal;
+  const T delta = (T)s.huber_delta;
   const int64_t N = (int64_t)s.m * s.n * s.l;
   if (grad_u)
     for (int64_t p = threadIdx.x; p < N; p += blockDim.x) grad_u[p] = T(0);
@@ -132,8 +134,17 @@ __global__ void __launch_bounds__(256) misfit_kernel(const MisfitDesc* __restric
     const T ob = tobs[j];
     if (ob != T(-1)) {
       const T r = ob - t;
-      acc = acc + w[j] * (r * r);
-      if (grad_u) trilinear_adjoint(tr, -(w[j] * (T(2.0) * r)), grad_u);
+      T rho, drho;  // rho(r) and d rho / d r
+      if (delta > T(0) && (r > delta || r < -delta)) {
+        const T ar = r < T(0) ? -r : r;
+        rho = T(2.0) * delta * ar - delta * delta;
+        drho = r < T(0) ? -(T(2.0) * delta) : T(2.0) * delta;
+      } else {
+        rho = r * r;
+        drho = T(2.0) * r;
+      }
+      acc = acc + w[j] * rho;
+      if (grad_u) trilinear_adjoint(tr, -(w[j] * drho), grad_u);
     }
   }
   part[threadIdx.x] = acc;
@@ -254,6 +265,50 @@ __global__ void model_grad_kernel(const T* __restrict__ x, const T* __restrict__
   }
 }
 
+// f_s = scale / fvar: the S-wave slowness pvs ./ fvar of
+// scripts/inversion_joint.jl:79 (one division, as written there).
+template <typename T>
+__global__ void slowness_kernel(const T* __restrict__ fvar, int64_t N, T scale,
+                                T* __restrict__ f_s) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (int64_t)gridDim.x * blockDim.x)
+    f_s[p] = scale / fvar[p];
+}
+
+// Adjoint of f_s = scale / fvar: grad_fvar += -scale grad_fs / fvar^2, and
+// per-block partial sums of d/dscale = sum grad_fs / fvar (fixed tree order).
+template <typename T>
+__global__ void __launch_bounds__(256) slowness_grad_kernel(const T* __restrict__ fvar, int64_t N,
+                                                            T scale, const T* __restrict__ grad_fs,
+                                                            T* __restrict__ grad_fvar,
+                                                            T* __restrict__ partial) {
+  __shared__ T part[256];
+  T acc = T(0);
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const T v = fvar[p], g = grad_fs[p];
+    if (grad_fvar) grad_fvar[p] = grad_fvar[p] - scale * g / (v * v);
+    acc = acc + g / v;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = part[0];
+}
+
+// out = sum of partial[0..npart) in order (one thread).
+template <typename T>
+__global__ void sum_partials_kernel(const T* __restrict__ partial, int npart, T* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    T s = T(0);
+    for (int b = 0; b < npart; ++b) s = s + partial[b];
+    *out = s;
+  }
+}
+
 #define ADT_OBJ_INSTANTIATE(T)                                                                    \
   template __global__ void source_u0_kernel<T>(const SourceDesc*, const T*, int, int, int, T);    \
   template __global__ void misfit_kernel<T>(const MisfitDesc*, const double*, const double*,      \
@@ -263,6 +318,9 @@ __global__ void model_grad_kernel(const T* __restrict__ x, const T* __restrict__
   template __global__ void smooth_pass1_kernel<T>(const T*, int, int, int, int, int, T, T*, T*); \
   template __global__ void smooth_pass2_kernel<T>(const T*, int, int, int, int, int, T, T, T*,   \
                                                   const T*, int, T*);                             \
+  template __global__ void slowness_kernel<T>(const T*, int64_t, T, T*);                         \
+  template __global__ void slowness_grad_kernel<T>(const T*, int64_t, T, const T*, T*, T*);       \
+  template __global__ void sum_partials_kernel<T>(const T*, int, T*);                            \
   template __global__ void model_grad_kernel<T>(const T*, const T*, const T*, const T*, int64_t, \
                                                 T*);
 

@@ -44,7 +44,9 @@ def main():
     ap.add_argument("--cpu-events", type=int, default=0, help="0 = 20 per host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--fp32", action="store_true", help="also time the FP32 variant")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 variant")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch setting instead of timing the candidates")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -82,7 +84,14 @@ def main():
     gd = [torch.from_numpy(a).cuda() for a in gs]
     del u0s, gs
 
-    # correctness gate on the timed path: event 0 against the batched eager run
+    # launch setting: timed on this workload by the library's autotuner
+    if args.no_autotune:
+        tune = A.get_tune("float64")
+    else:
+        t0 = time.time()
+        tune = A.autotune(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd, reps=1)
+        log(f"autotune ({time.time() - t0:.1f}s): {tune}")
+
     G = A.EikonalGraph(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd)
     for _ in range(args.warmup):
         G.launch()
@@ -137,6 +146,39 @@ def main():
     log(f"forward kernel {fwd_ms:.2f} ms -> {fwd_gbs:.1f} GB/s algorithmic; "
         f"adjoint kernel {adj_ms:.2f} ms -> {adj_gbs:.1f} GB/s")
 
+    # FP32 variant of the same step, with its error against FP64 (the FP64
+    # device results are bit-exact with the oracle's travel times)
+    fp32 = None
+    if not args.no_fp32:
+        ff = fd.float()
+        u0f = [a.float() for a in u0d]
+        gdf = [a.float() for a in gd]
+        if not args.no_autotune:
+            A.autotune(u0f, [ff] * nev, [h] * nev, [tol] * nev, grad_us=gdf, reps=1)
+        G32 = A.EikonalGraph(u0f, [ff] * nev, [h] * nev, [tol] * nev, grad_us=gdf)
+        ms32 = time_graph(G32, max(2, args.steps))
+        K32 = np.array([i["iterations"] for i in G32.infos()])
+        eu_max = eu_ss = u_max = eg_max = eg_ss = g_max = 0.0
+        for e in range(nev):
+            du = G32.us[e].double() - G.us[e]
+            dg = G32.grad_fs[e].double() - G.grad_fs[e]
+            eu_max = max(eu_max, float(du.abs().max()))
+            eg_max = max(eg_max, float(dg.abs().max()))
+            eu_ss += float((du * du).sum())
+            eg_ss += float((dg * dg).sum())
+            u_max = max(u_max, float(G.us[e].abs().max()))
+            g_max = max(g_max, float(G.grad_fs[e].abs().max()))
+        fp32 = {"event_solves_per_s": nev / (ms32 / 1e3), "ms_per_step": ms32,
+                "K_mean": float(K32.mean()),
+                "u_err_max_rel": eu_max / u_max, "u_err_rms_s": (eu_ss / (nev * N)) ** 0.5,
+                "grad_f_err_max_rel": eg_max / g_max,
+                "grad_f_err_rms": (eg_ss / (nev * N)) ** 0.5}
+        G32.close()
+        del u0f, gdf, ff
+        log(f"fp32: {ms32:.2f} ms/step -> {fp32['event_solves_per_s']:.1f} event-solves/s; "
+            f"travel time max rel err {fp32['u_err_max_rel']:.2e}, rms {fp32['u_err_rms_s']:.2e} s; "
+            f"grad_f max rel err {fp32['grad_f_err_max_rel']:.2e}")
+
     plain = None
     if not args.no_plain and args.plain_events > 0:
         pe = args.plain_events
@@ -181,12 +223,14 @@ def main():
         "config": {"workload": "256 point-source events, 128x128x64, h=0.5 km, layered "
                                "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
                    "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
-                   "adj_sweeps_mean": float(sweeps.mean())},
+                   "adj_sweeps_mean": float(sweeps.mean()),
+                   "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order")}},
         "roofline": {"bound": "hbm", "kernel": "fwd3_diag_kernel<double>",
                      "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fwd_gbs / HBM_PEAK_GBS, 5),
                      "traffic": FWD_TRAFFIC_BYTES_256 if nev == 256 else None},
         "cpu_baseline": cpu,
+        "fp32": fp32,
         "plain_event_solves_per_s": plain,
         "batched_vs_plain": (value / plain) if plain else None,
         "node_updates_per_s": node_updates / (fwd_ms / 1e3),

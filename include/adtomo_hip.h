@@ -148,6 +148,29 @@ int adt_graph_launch(adt_graph *g, void *stream);
 int adt_graph_infos(adt_graph *g, adt_info *infos, void *stream);
 int adt_graph_destroy(adt_graph *g);
 
+/* ---- Launch tuning of the 3D kernels (one setting per precision; dtype 0
+ * = FP64, 1 = FP32).  Read when a batched call runs or a graph is created;
+ * every setting gives the same travel times (bit for bit) and gradients
+ * (to rounding). */
+typedef struct adt_tune {
+  int32_t diag_max_l; /* forward: events with l <= diag_max_l (0..64) use the
+                         diagonal-streaming kernel, the others k-streaming */
+  int32_t adj_wg;     /* adjoint workgroup size: 256, 512 or 1024 */
+  int32_t order;      /* 0: events in the given order; 1: largest first */
+  float fwd_ms;       /* adt_autotune: measured forward / adjoint time per */
+  float adj_ms;       /* launch of the chosen setting (ms) */
+} adt_tune;
+int adt_get_tune(int32_t dtype, adt_tune *out);
+int adt_set_tune(int32_t dtype, const adt_tune *t);
+/* Times every setting on the given events (device pointers, as for
+ * adt_eikonal3d_batched_*; `reps` timed launches each), stores the fastest
+ * for `dtype` and returns it in `best` (may be NULL).  The events' outputs
+ * hold a valid solve afterwards.  With ADT_ADJOINT but not ADT_FORWARD, u
+ * must already hold the forward travel times.  The reference has no
+ * counterpart (its op is single-threaded CPU code). */
+int adt_autotune(const adt_event3d *events, int32_t nev, int32_t flags, int32_t dtype,
+                 int32_t reps, adt_tune *best, void *stream);
+
 /* ---- Objective around the path (scripts/inversion.jl), all on the device.
  * Grid coordinates are 1-based and real-valued, as in the reference's
  * station/event tables (x along i, y along j, z along k). */
@@ -170,8 +193,10 @@ int adt_source_u0_3d_batched_f32(const adt_source3d *src, int32_t nsrc, const fl
  * (events) at host coordinates (rx, ry, rz) (scripts/inversion.jl:64-105):
  * tcal = trilinear interpolation of u (a dimension whose floor and ceil
  * coincide is not interpolated), loss = sum over tobs != -1 of
- * weight * (tobs - tcal)^2, grad_u = d loss / d u (dense, zero except at
- * the interpolation nodes; contributions of receivers sharing a node are
+ * weight * rho(tobs - tcal) with rho(r) = r^2 (the reference) or, when
+ * huber_delta > 0, the Huber form rho(r) = r^2 for |r| <= delta and
+ * 2 delta |r| - delta^2 beyond; grad_u = d loss / d u (dense, zero except
+ * at the interpolation nodes; contributions of receivers sharing a node are
  * added atomically, so their order may vary between runs).  tcal and
  * grad_u may be NULL; loss is a device scalar. */
 typedef struct adt_misfit3d {
@@ -182,6 +207,7 @@ typedef struct adt_misfit3d {
   void *tcal;          /* out: [nrec] */
   void *grad_u;        /* out: [m*n*l] */
   void *loss;          /* out: scalar */
+  double huber_delta;  /* <= 0: squared residuals (scripts/inversion.jl:102) */
 } adt_misfit3d;
 int adt_misfit3d_batched_f64(const adt_misfit3d *items, int32_t nitem, const double *rx,
                              const double *ry, const double *rz, int64_t nrec, void *stream);
@@ -200,6 +226,17 @@ int adt_model3d_grad_f64(const double *x, const double *fvar, const double *grad
                          const double *grad_fvar, int64_t N, double *grad_x, void *stream);
 int adt_model3d_grad_f32(const float *x, const float *fvar, const float *grad_f,
                          const float *grad_fvar, int64_t N, float *grad_x, void *stream);
+
+/* S-wave slowness of the joint inversion (scripts/inversion_joint.jl:79):
+ * f_s = scale / fvar (scale = the vp/vs ratio). */
+int adt_slowness3d_f64(const double *fvar, int64_t N, double scale, double *f_s, void *stream);
+int adt_slowness3d_f32(const float *fvar, int64_t N, float scale, float *f_s, void *stream);
+/* Its adjoint: grad_fvar += -scale * grad_fs / fvar^2 (in place), and
+ * grad_scale (device scalar, may be NULL) = sum of grad_fs / fvar. */
+int adt_slowness3d_grad_f64(const double *fvar, int64_t N, double scale, const double *grad_fs,
+                            double *grad_fvar, double *grad_scale, void *stream);
+int adt_slowness3d_grad_f32(const float *fvar, int64_t N, float scale, const float *grad_fs,
+                            float *grad_fvar, float *grad_scale, void *stream);
 
 /* Smoothing regulariser (scripts/inversion.jl:107-121): value = lambda *
  * sum |fvar - box(fvar)| with box = the mean over an sh x sh x sv window

@@ -12,6 +12,9 @@ scale):
   * ``smooth_l1``   :107-121 lambda * sum |fvar - box(fvar)|, box = mean over
                              an sh x sh x sv window with periodic padding
   * ``model_grad``           chain rule of the two maps above
+  * ``slowness``    inversion_joint.jl:79 S slowness vp/vs / fvar, and its
+                             adjoint ``slowness_grad``
+  * ``huber``                Huber residual (delta > 0) for the robust misfit
 
 Arrays are (m, n, l) row-major (flat = i*n*l + j*l + k, the layout of
 src/eikonal_op.jl:24-32); coordinates are the reference's 1-based reals.
@@ -88,8 +91,17 @@ def interpolation_weights(shape, jx, jy, jz):
     return out
 
 
-def misfit(u, rx, ry, rz, tobs, w):
-    """scripts/inversion.jl:64-105 for one station: (tcal, loss, grad_u)."""
+def huber(r, delta):
+    """(rho, d rho / d r): r^2 inside [-delta, delta], 2 delta |r| - delta^2
+    outside; delta <= 0 gives the reference's r^2."""
+    if delta > 0 and abs(r) > delta:
+        return 2.0 * delta * abs(r) - delta * delta, (2.0 * delta if r > 0 else -(2.0 * delta))
+    return r * r, 2.0 * r
+
+
+def misfit(u, rx, ry, rz, tobs, w, delta=0.0):
+    """scripts/inversion.jl:64-105 for one station: (tcal, loss, grad_u);
+    delta > 0 selects the Huber residual."""
     tcal = np.array([interpolate(u, a, b, c) for a, b, c in zip(rx, ry, rz)])
     loss = 0.0
     grad = np.zeros_like(u)
@@ -97,8 +109,9 @@ def misfit(u, rx, ry, rz, tobs, w):
         if tobs[j] == -1:
             continue
         r = tobs[j] - tcal[j]
-        loss += w[j] * (r * r)
-        g = -(w[j] * (2.0 * r))
+        rho, drho = huber(r, delta)
+        loss += w[j] * rho
+        g = -(w[j] * drho)
         for key, wt in interpolation_weights(u.shape, rx[j], ry[j], rz[j]).items():
             grad[key] += g * wt
     return tcal, loss, grad
@@ -112,6 +125,16 @@ def model(x, vel0):
     """scripts/inversion.jl:42-43,61: fvar = 2 sigmoid(x) - 1 + vel0, f = 1/fvar."""
     fvar = 2.0 * sigmoid(x) - 1.0 + vel0
     return fvar, 1.0 / fvar
+
+
+def slowness(fvar, scale):
+    """scripts/inversion_joint.jl:79: S slowness pvs ./ fvar."""
+    return scale / fvar
+
+
+def slowness_grad(fvar, scale, grad_fs):
+    """(d/dfvar, d/dscale) of sum(grad_fs * scale / fvar)."""
+    return -scale * grad_fs / (fvar * fvar), float((grad_fs / fvar).sum())
 
 
 def box_mean(a, sh, sv):

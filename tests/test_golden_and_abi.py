@@ -76,3 +76,20 @@ def test_product_package_does_not_import_oracle():
     for p in glob.glob(os.path.join(pkg, "**", "*.py"), recursive=True):
         txt = open(p).read()
         assert "from oracle" not in txt and "import oracle" not in txt, p
+
+
+def test_tune_get_set_validation_without_device():
+    # the launch setting is host state: validated and stored without a device
+    import adtomo_hip as A
+    base = A.get_tune("float64")
+    try:
+        assert base["diag_max_l"] == 64 and base["adj_wg"] in (256, 512, 1024)
+        t = A.set_tune("float64", diag_max_l=0, adj_wg=256, order=0)
+        assert A.get_tune("float64")["adj_wg"] == 256 and t["diag_max_l"] == 0
+        assert A.get_tune("float32")["adj_wg"] == base["adj_wg"]  # per precision
+        for bad in ({"diag_max_l": 65}, {"diag_max_l": -1}, {"adj_wg": 300}, {"order": 2}):
+            with pytest.raises(A.AdtError):
+                A.set_tune("float64", **bad)
+    finally:
+        A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
+                   order=base["order"])

@@ -356,3 +356,70 @@ def test_graph_replays_match_eager():
         infos = G.infos()
         assert [i["iterations"] for i in infos] == [i["iterations"] for i in eager["info"]]
     G.close()
+
+
+# --------------------------------------------------------------- tuning ---
+
+def _mixed_batch(seed=300):
+    shapes = [(10, 12, 70), (20, 9, 30), (8, 8, 100), (12, 14, 64), (6, 7, 65), (9, 11, 7)]
+    fs, u0s, gs = [], [], []
+    for e, shp in enumerate(shapes):
+        r = synth.rng(seed + e)
+        fs.append(r.uniform(0.3, 3.0, size=shp))
+        u0s.append(synth.point_source_u0(shp, tuple(int(r.integers(0, s)) for s in shp)))
+        gs.append(synth.sparse_adjoint_source(shp, 6, seed + e))
+    return fs, u0s, gs
+
+
+@pytest.mark.parametrize("setting", [
+    {"diag_max_l": 64, "adj_wg": 1024, "order": 1},
+    {"diag_max_l": 0, "adj_wg": 256, "order": 0},
+    {"diag_max_l": 32, "adj_wg": 512, "order": 1},
+    {"diag_max_l": 7, "adj_wg": 256, "order": 1},
+])
+def test_every_tune_setting_matches_oracle(setting):
+    import adtomo_hip as A
+    base = A.get_tune("float64")
+    fs, u0s, gs = _mixed_batch()
+    try:
+        A.set_tune("float64", **setting)
+        res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
+                                  [1e-4] * len(fs), grad_us=[dev(a) for a in gs])
+        G = A.EikonalGraph([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
+                           [1e-4] * len(fs), grad_us=[dev(a) for a in gs])
+        G.launch()
+        ginfo = G.infos()
+        for e in range(len(fs)):
+            u_ref, it, _ = oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-4)
+            _, gf_ref, _ = oracle.eikonal3d_grad(gs[e], u_ref, u0s[e], fs[e], 0.5)
+            for r, info in ((res, res["info"][e]), (G, ginfo[e])):
+                us = r["u"] if r is res else r.us
+                gfs = r["grad_f"] if r is res else r.grad_fs
+                assert info["iterations"] == it, (setting, e)
+                assert bits_equal(host(us[e]), u_ref), (setting, e)
+                assert rel(host(gfs[e]), gf_ref) <= GRAD_RTOL, (setting, e)
+        G.close()
+    finally:
+        A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
+                   order=base["order"])
+
+
+def test_autotune_stores_a_valid_setting():
+    import adtomo_hip as A
+    base = A.get_tune("float64")
+    fs, u0s, gs = _mixed_batch(400)
+    try:
+        best = A.autotune([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
+                          [1e-3] * len(fs), grad_us=[dev(a) for a in gs], reps=2)
+        print("autotune:", best, flush=True)
+        assert best["diag_max_l"] in (0, 64) and best["adj_wg"] in (256, 512, 1024)
+        assert best["order"] in (0, 1) and best["fwd_ms"] > 0 and best["adj_ms"] > 0
+        cur = A.get_tune("float64")
+        assert all(cur[k] == best[k] for k in ("diag_max_l", "adj_wg", "order"))
+        res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
+                                  [1e-3] * len(fs))
+        for e in range(len(fs)):
+            assert bits_equal(host(res["u"][e]), oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-3)[0])
+    finally:
+        A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
+                   order=base["order"])

@@ -175,6 +175,40 @@ def test_sum_arrays_bit_exact():
     np.testing.assert_array_equal(out, ref)
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-4)])
+def test_misfit_huber_parity(dtype, tol):
+    from adtomo_hip import objective as O
+    us, (rx, ry, rz), tobs, w = _misfit_case(31, (12, 10, 8), 3, 60)
+    r = np.random.default_rng(32)
+    for s in range(3):  # residuals on both sides of delta
+        t = np.array([OO.interpolate(us[s], *p) for p in zip(rx, ry, rz)])
+        keep = tobs[s] == -1
+        tobs[s] = t + r.choice([-1, 1], len(t)) * r.uniform(0, 0.3, len(t))
+        tobs[s][keep] = -1.0
+    out = O.misfit([dev(u, dtype) for u in us], rx, ry, rz, [dev(t, dtype) for t in tobs],
+                   [dev(x, dtype) for x in w], huber_delta=0.1)
+    for s in range(3):
+        _, L, g = OO.misfit(us[s], rx, ry, rz, tobs[s], w[s], delta=0.1)
+        assert abs(host(out["loss"])[s] - L) <= tol * abs(L)
+        assert rel(host(out["grad_u"][s]), g) <= tol
+
+
+def test_slowness_and_grad_parity():
+    from adtomo_hip import objective as O
+    r = np.random.default_rng(33)
+    fvar = r.uniform(3, 8, (13, 11, 9))
+    gfs = r.normal(size=fvar.shape)
+    base = r.normal(size=fvar.shape)
+    fs = O.slowness(dev(fvar), 1.73)
+    np.testing.assert_array_equal(host(fs), OO.slowness(fvar, 1.73))
+    gfvar = dev(base)
+    gs = O.slowness_grad(dev(fvar), 1.73, dev(gfs), grad_fvar=gfvar)
+    rg, rs = OO.slowness_grad(fvar, 1.73, gfs)
+    assert rel(host(gfvar), base + rg) <= 1e-14
+    assert abs(host(gs)[0] - rs) <= 1e-12 * abs(rs)
+    assert O.slowness_grad(dev(fvar), 1.73, dev(gfs), grad_scale=False) is None
+
+
 # --------------------------------------------------- one inversion step ---
 
 def _inversion_case(seed=20, shape=(20, 18, 12), ns=3, ne=25):
@@ -236,5 +270,64 @@ def test_inversion_run_decreases_loss_and_checkpoints(tmp_path):
         inv.close()
     assert hist[-1] < hist[0]
     files = sorted(os.listdir(tmp_path))
-    assert files and all(f.startswith("iter_") and f.endswith(".npy") for f in files)
-    np.testing.assert_equal(np.load(tmp_path / files[0]).shape, vel0.shape)
+    assert files and all(f.startswith("iter_") and f.endswith(".npz") for f in files)
+    with np.load(tmp_path / files[0]) as z:
+        assert z["x"].shape == vel0.shape
+
+
+def _joint_case(seed=40, shape=(18, 16, 12), ns=5, ne=20):
+    r = np.random.default_rng(seed)
+    vel0, h, stations, ev, tobs_p, w_p = _inversion_case(seed, shape, ns, ne)
+    true_s = vel0 / 1.8
+    tobs_s = np.empty_like(tobs_p)
+    for s, st in enumerate(stations):
+        u0 = OO.source_u0(shape, *st, vel0, h, 1.7)
+        u, _, _ = oracle.eikonal3d(u0, 1.0 / true_s, h, 1e-3)
+        tobs_s[s] = [OO.interpolate(u, *p) for p in zip(*ev)]
+    tobs_s += r.normal(scale=0.05, size=tobs_s.shape)
+    tobs_s[2, 3] = -1.0
+    w_s = r.uniform(0.5, 1.5, tobs_s.shape)
+    return vel0, h, stations, ev, {"P": tobs_p, "S": tobs_s}, {"P": w_p, "S": w_s}
+
+
+def _oracle_joint(params, vel0, h, stations, ev, tobs, w, sh, sv, lam, delta):
+    shape = vel0.shape
+    N = vel0.size
+    x, vpvs = params[:N].reshape(shape), params[N]
+    fvar, fp = OO.model(x, vel0)
+    fs = OO.slowness(fvar, vpvs)
+    loss = 0.0
+    gf = {"P": np.zeros(shape), "S": np.zeros(shape)}
+    for ph, f, scale in (("P", fp, 1.0), ("S", fs, 1.7)):
+        for s, st in enumerate(stations):
+            u0 = OO.source_u0(shape, *st, vel0, h, scale)
+            u, _, _ = oracle.eikonal3d(u0, f, h, 1e-3)
+            _, L, gu = OO.misfit(u, *ev, tobs[ph][s], w[ph][s], delta=delta)
+            loss += L
+            gf[ph] += oracle.eikonal3d_grad(gu, u, u0, f, h)[1]
+    reg, greg = OO.smooth_l1(fvar, sh, sv, lam)
+    g_fvar_s, g_vpvs = OO.slowness_grad(fvar, vpvs, gf["S"])
+    gx = OO.model_grad(x, fvar, gf["P"], greg + g_fvar_s)
+    return loss + reg, np.concatenate([gx.reshape(-1), [g_vpvs]])
+
+
+@pytest.mark.parametrize("chunk", [None, 2])
+def test_joint_inversion_objective_matches_oracle(chunk):
+    from adtomo_hip.inversion import TravelTimeInversion
+    vel0, h, stations, ev, tobs, w = _joint_case()
+    inv = TravelTimeInversion(vel0, h, stations, ev, tobs, w, smooth=(3, 3), lam=0.02,
+                              phases=("P", "S"), vpvs=1.7, huber_delta=0.1, chunk=chunk)
+    try:
+        r = np.random.default_rng(41)
+        p0 = inv.initial_params()
+        p1 = p0.copy()
+        p1[:-1] = r.normal(scale=0.2, size=vel0.size)
+        p1[-1] = 1.76
+        for params in (p0, p1):
+            L, g = inv.objective(params)
+            RL, Rg = _oracle_joint(params, vel0, h, stations, ev, tobs, w, 3, 3, 0.02, 0.1)
+            assert abs(L - RL) <= 1e-12 * abs(RL)
+            assert rel(g[:-1], Rg[:-1]) <= 1e-9
+            assert abs(g[-1] - Rg[-1]) <= 1e-9 * abs(Rg[-1])
+    finally:
+        inv.close()

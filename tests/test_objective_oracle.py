@@ -187,3 +187,61 @@ def test_smooth_l1_constant_field_is_zero():
     v = np.full((4, 5, 6), 6.0)
     val, g = O.smooth_l1(v, 3, 3, 2.0)
     assert val == pytest.approx(0.0, abs=1e-12)
+
+
+# ------------------------------------------------- Huber and S slowness
+
+def test_huber_matches_square_inside_and_is_c1():
+    for r in (-0.05, 0.0, 0.07):
+        assert O.huber(r, 0.1) == (r * r, 2 * r)
+        assert O.huber(r, 0.0) == (r * r, 2 * r)
+    d = 0.1
+    for r in (0.3, -0.25):
+        rho, drho = O.huber(r, d)
+        assert rho == pytest.approx(2 * d * abs(r) - d * d, rel=1e-15)
+        assert drho == pytest.approx(2 * d * np.sign(r), rel=1e-15)
+    # continuity of value and slope at |r| = delta
+    eps = 1e-9
+    for s in (1, -1):
+        a, b = O.huber(s * (d - eps), d), O.huber(s * (d + eps), d)
+        assert a[0] == pytest.approx(b[0], abs=1e-9) and a[1] == pytest.approx(b[1], abs=1e-8)
+
+
+def test_huber_misfit_gradient_central_differences():
+    r = _rng(9)
+    shape = (4, 5, 4)
+    u = r.uniform(0, 2, shape)
+    rx, ry, rz = _events(r, shape, 12)
+    tobs = np.array([O.interpolate(u, a, b, c) for a, b, c in zip(rx, ry, rz)])
+    tobs += r.choice([-1, 1], 12) * r.uniform(0.0, 0.3, 12)  # both sides of delta
+    w = r.uniform(0.5, 2.0, 12)
+    _, loss, grad = O.misfit(u, rx, ry, rz, tobs, w, delta=0.1)
+    eps = 1e-7
+    for idx in list(np.ndindex(shape))[::2]:
+        up = u.copy()
+        up[idx] += eps
+        dn = u.copy()
+        dn[idx] -= eps
+        fd = (O.misfit(up, rx, ry, rz, tobs, w, delta=0.1)[1]
+              - O.misfit(dn, rx, ry, rz, tobs, w, delta=0.1)[1]) / (2 * eps)
+        assert fd == pytest.approx(grad[idx], rel=1e-5, abs=1e-7)
+    # a huge delta gives the squared misfit
+    assert O.misfit(u, rx, ry, rz, tobs, w, delta=1e9)[1] == O.misfit(u, rx, ry, rz, tobs, w)[1]
+
+
+def test_slowness_grad_central_differences():
+    r = _rng(10)
+    fvar = r.uniform(3, 8, (3, 4, 5))
+    a = r.normal(size=fvar.shape)
+    s = 1.73
+    gfvar, gs = O.slowness_grad(fvar, s, a)
+    eps = 1e-6
+    fd_s = ((a * O.slowness(fvar, s + eps)).sum() - (a * O.slowness(fvar, s - eps)).sum()) / (2 * eps)
+    assert fd_s == pytest.approx(gs, rel=1e-8)
+    for idx in list(np.ndindex(fvar.shape))[::7]:
+        fp = fvar.copy()
+        fp[idx] += eps
+        fm = fvar.copy()
+        fm[idx] -= eps
+        fd = ((a * O.slowness(fp, s)).sum() - (a * O.slowness(fm, s)).sum()) / (2 * eps)
+        assert fd == pytest.approx(gfvar[idx], rel=1e-6, abs=1e-10)

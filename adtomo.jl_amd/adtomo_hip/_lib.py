@@ -16,6 +16,7 @@ ADT_OK = 0
 ADT_HOST_POINTERS = 0x1
 ADT_FORWARD = 0x2
 ADT_ADJOINT = 0x4
+ADT_F32_COMPUTE_F64 = 0x8
 MAX_LOGGED_ITERS = 100
 
 _i32 = ctypes.c_int32

@@ -12,10 +12,11 @@ One objective evaluation, all on the GPU except the optimiser state:
     reg, dR/dfvar = smooth_l1(fvar)           inversion.jl:107-121
     loss = sum_s loss_s + reg,  dL/dx (and dL/dvpvs) by the chain rule
 
-Stations are solved ``chunk`` at a time: each chunk's forward, misfit and
-adjoint run back to back over one set of slot buffers, and the per-station
-dL/df are summed into one accumulator, so device memory holds ``chunk``
-travel-time fields (plus their adjoint workspace), not one per station.
+Stations are solved ``chunk`` at a time, all phases in the same launch:
+each chunk's forward, misfit and adjoint run back to back over one set of
+slot buffers, and the per-station dL/df are summed into one accumulator per
+phase, so device memory holds ``chunk`` x phases travel-time fields (plus
+their adjoint workspace), not one per station.
 Without chunking (``chunk`` = number of stations) the initial times u0_s,
 which depend on the fixed starting model only (inversion.jl:48-60), are
 built once; with chunking they are rebuilt on the device per chunk.
@@ -48,11 +49,13 @@ class TravelTimeInversion:
     S initial times (fixed, inversion_joint.jl:65-73) and is a trainable
     parameter of the S slowness (inversion_joint.jl:50-51,79).
     ``huber_delta`` > 0 uses the Huber residual instead of the square.
+    ``compute_f64`` with ``dtype="float32"``: FP32 storage with FP64 local
+    solves in the forward.
     """
 
     def __init__(self, vel0, h, stations, events, tobs, weights, smooth=(3, 3), lam=0.0,
                  tol=1e-3, dtype="float64", phases=("P",), vpvs=1.7, huber_delta=0.0,
-                 chunk=None):
+                 chunk=None, compute_f64=False):
         torch = _torch()
         self.torch = torch
         self.dt = getattr(torch, dtype)
@@ -88,20 +91,23 @@ class TravelTimeInversion:
         if self.joint:
             self.f["S"] = torch.empty_like(self.x)
         self.grad_f = {p: torch.empty_like(self.x) for p in self.phases}
-        # solve slots: `chunk` stations per batched launch and phase
+        # solve slots: `chunk` stations per batched launch, for every phase
+        # at once (slots [q*C, (q+1)*C) belong to phase q), so that one
+        # launch carries all phases' solves
         self.chunk = ns if chunk is None else max(1, min(int(chunk), ns))
         C = self.chunk
-        self.slots = {}
-        for p in self.phases:
-            u0s = [torch.empty_like(self.x) for _ in range(C)]
-            gus = [torch.empty_like(self.x) for _ in range(C)]
-            fs = [self.f[p]] * C
-            fwd = EikonalGraph(u0s, fs, [self.h] * C, [tol] * C)
-            adj = EikonalGraph(u0s, fs, [self.h] * C, [tol] * C, grad_us=gus, forward=False,
-                               us=fwd.us)
-            self.slots[p] = {"u0": u0s, "grad_u": gus, "fwd": fwd, "adj": adj}
-            if C == ns:
-                O.source_u0(self.stations, self.vel0, self.h, self._u0_scale(p), out=u0s)
+        P = len(self.phases)
+        self.u0s = [torch.empty_like(self.x) for _ in range(P * C)]
+        self.grad_us = [torch.empty_like(self.x) for _ in range(P * C)]
+        fs = [self.f[p] for p in self.phases for _ in range(C)]
+        hs, tols = [self.h] * (P * C), [tol] * (P * C)
+        self.fwd = EikonalGraph(self.u0s, fs, hs, tols, compute_f64=compute_f64)
+        self.adj = EikonalGraph(self.u0s, fs, hs, tols, grad_us=self.grad_us, forward=False,
+                                us=self.fwd.us)
+        if C == ns:
+            for q, p in enumerate(self.phases):
+                O.source_u0(self.stations, self.vel0, self.h, self._u0_scale(p),
+                            out=self.u0s[q * C:(q + 1) * C])
         self.nevals = 0
         self.last_loss = {}
 
@@ -119,26 +125,32 @@ class TravelTimeInversion:
             x0[-1] = self.vpvs0
         return x0
 
-    def _solve_phase(self, phase):
-        """Forward, misfit and adjoint of every station of one phase; returns
-        the summed misfit and leaves sum_s dL/df_s in self.grad_f[phase]."""
-        S = self.slots[phase]
+    def _solve(self, grad=True):
+        """Forward, misfit and (with ``grad``) adjoint of every station and
+        phase; returns the summed misfit per phase and leaves sum_s dL/df_s
+        in self.grad_f."""
         C, ns = self.chunk, len(self.stations)
-        loss = 0.0
+        loss = {p: 0.0 for p in self.phases}
         for start in range(0, ns, C):
             k = min(C, ns - start)
             if C < ns:
-                O.source_u0(self.stations[start:start + k], self.vel0, self.h,
-                            self._u0_scale(phase), out=S["u0"][:k])
-            S["fwd"].launch()
-            mis = O.misfit(S["fwd"].us[:k], *self.events, self.tobs[phase][start:start + k],
-                           self.weights[phase][start:start + k], grad=True, tcal=False,
-                           grad_out=S["grad_u"][:k], huber_delta=self.huber)
-            loss += float(mis["loss"].cpu().numpy().sum())
-            S["adj"].launch()
-            parts = S["adj"].grad_fs[:k]
-            O.sum_arrays(parts if start == 0 else [self.grad_f[phase]] + parts,
-                         out=self.grad_f[phase])
+                for q, p in enumerate(self.phases):
+                    O.source_u0(self.stations[start:start + k], self.vel0, self.h,
+                                self._u0_scale(p), out=self.u0s[q * C:q * C + k])
+            self.fwd.launch()
+            for q, p in enumerate(self.phases):
+                sl = slice(q * C, q * C + k)
+                mis = O.misfit(self.fwd.us[sl], *self.events, self.tobs[p][start:start + k],
+                               self.weights[p][start:start + k], grad=grad, tcal=False,
+                               grad_out=self.grad_us[sl] if grad else None,
+                               huber_delta=self.huber)
+                loss[p] += float(mis["loss"].cpu().numpy().sum())
+            if not grad:
+                continue
+            self.adj.launch()
+            for q, p in enumerate(self.phases):
+                parts = self.adj.grad_fs[q * C:q * C + k]
+                O.sum_arrays(parts if start == 0 else [self.grad_f[p]] + parts, out=self.grad_f[p])
         return loss
 
     def objective(self, params, grad=True):
@@ -150,10 +162,8 @@ class TravelTimeInversion:
         O.model(self.x, self.vel0, out=(self.fvar, self.f["P"]))
         if self.joint:
             O.slowness(self.fvar, vpvs, out=self.f["S"])
-        loss = 0.0
-        for p in self.phases:
-            self.last_loss[p] = self._solve_phase(p)
-            loss += self.last_loss[p]
+        self.last_loss = self._solve(grad)
+        loss = sum(self.last_loss.values())
         reg_val, grad_fvar = O.smooth_l1(self.fvar, self.smooth[0], self.smooth[1], self.lam)
         self.last_loss["reg"] = float(reg_val.cpu().numpy()[0])
         loss += self.last_loss["reg"]
@@ -171,7 +181,8 @@ class TravelTimeInversion:
 
     def travel_times(self, phase="P"):
         """Travel times of the last chunk's forward, one (m, n, l) tensor per slot."""
-        return self.slots[phase]["fwd"].us
+        q = self.phases.index(phase)
+        return self.fwd.us[q * self.chunk:(q + 1) * self.chunk]
 
     def save_checkpoint(self, path, params):
         params = np.asarray(params)
@@ -198,6 +209,5 @@ class TravelTimeInversion:
         return res.x, history
 
     def close(self):
-        for S in self.slots.values():
-            S["fwd"].close()
-            S["adj"].close()
+        self.fwd.close()
+        self.adj.close()

@@ -258,17 +258,23 @@ def _event_table(u0s, fs, hs, tols, grad_us, us, grad_u0s, grad_fs):
     return tab, dtype
 
 
+def _flags(forward, grad_us, compute_f64):
+    return ((L.ADT_FORWARD if forward else 0) | (L.ADT_ADJOINT if grad_us is not None else 0)
+            | (L.ADT_F32_COMPUTE_F64 if compute_f64 else 0))
+
+
 def eikonal3d_batched(u0s, fs, hs, tols, grad_us=None, forward=True, us=None,
-                      return_info=True):
+                      return_info=True, compute_f64=False):
     """Many independent 3D events (boxes of any size) in one persistent launch.
 
     Lists of contiguous (m, n, l) device tensors.  With ``grad_us`` the
-    adjoint runs too.  Returns dict(u, grad_u0, grad_f, info)."""
+    adjoint runs too.  ``compute_f64`` (float32 tensors): FP32 storage with
+    every forward local solve in FP64.  Returns dict(u, grad_u0, grad_f, info)."""
     torch = _torch()
     nev = len(fs)
     if us is None:
         us = [torch.empty_like(f) for f in fs]
-    flags = (L.ADT_FORWARD if forward else 0) | (L.ADT_ADJOINT if grad_us is not None else 0)
+    flags = _flags(forward, grad_us, compute_f64)
     gu0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
     gfs = [torch.empty_like(f) for f in fs] if grad_us is not None else None
     tab, dtype = _event_table(u0s, fs, hs, tols, grad_us, us, gu0s, gfs)
@@ -306,7 +312,7 @@ def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None):
     return t.as_dict()
 
 
-def autotune(u0s, fs, hs, tols, grad_us=None, reps=3):
+def autotune(u0s, fs, hs, tols, grad_us=None, reps=3, compute_f64=False):
     """Time the launch settings on these events and keep the fastest for
     their precision (forward kernel split, adjoint workgroup size, event
     order).  Returns the chosen setting with its measured times."""
@@ -315,7 +321,7 @@ def autotune(u0s, fs, hs, tols, grad_us=None, reps=3):
     gu0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
     gfs = [torch.empty_like(f) for f in fs] if grad_us is not None else None
     tab, dtype = _event_table(u0s, fs, hs, tols, grad_us, us, gu0s, gfs)
-    flags = L.ADT_FORWARD | (L.ADT_ADJOINT if grad_us is not None else 0)
+    flags = _flags(True, grad_us, compute_f64)
     best = L.AdtTune()
     L.check(L.lib().adt_autotune(ctypes.byref(tab), len(fs), flags, _dtype_code(dtype), int(reps),
                                  ctypes.byref(best), _stream()), "autotune")
@@ -328,7 +334,7 @@ class EikonalGraph:
     Build once, then ``launch()`` repeatedly; the event buffers (inputs and
     outputs, all kept alive by this object) are read/written in place."""
 
-    def __init__(self, u0s, fs, hs, tols, grad_us=None, forward=True, us=None):
+    def __init__(self, u0s, fs, hs, tols, grad_us=None, forward=True, us=None, compute_f64=False):
         torch = _torch()
         self.u0s, self.fs, self.grad_us = u0s, fs, grad_us
         # travel times: written by the forward, read by the adjoint (pass the
@@ -337,7 +343,7 @@ class EikonalGraph:
         self.grad_u0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
         self.grad_fs = [torch.empty_like(f) for f in fs] if grad_us is not None else None
         self.nev = len(fs)
-        flags = (L.ADT_FORWARD if forward else 0) | (L.ADT_ADJOINT if grad_us is not None else 0)
+        flags = _flags(forward, grad_us, compute_f64)
         tab, dtype = _event_table(u0s, fs, hs, tols, grad_us, self.us, self.grad_u0s, self.grad_fs)
         self._g = ctypes.c_void_p()
         rc = L.lib().adt_graph_create(ctypes.byref(tab), self.nev, flags,

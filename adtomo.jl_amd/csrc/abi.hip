@@ -137,6 +137,28 @@ struct Staged {
 
 // ------------------------------------------------------------ 3D forward ---
 
+// Forward launches; mix (FP32 only): FP64 local solves on FP32 storage.
+template <typename T>
+void launch_fwd_diag(bool mix, int grid, const adt::EventDesc* d, int max_l, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (mix) {
+      adt::fwd3_diag_kernel<float, true><<<grid, kDiagWG, 0, s>>>(d, kMaxIter3, max_l);
+      return;
+    }
+  }
+  adt::fwd3_diag_kernel<T, false><<<grid, kDiagWG, 0, s>>>(d, kMaxIter3, max_l);
+}
+template <typename T>
+void launch_fwd_stream(bool mix, int grid, const adt::EventDesc* d, int min_l, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (mix) {
+      adt::fwd3_stream_kernel<float, true><<<grid, kStreamWG, 0, s>>>(d, kMaxIter3, min_l);
+      return;
+    }
+  }
+  adt::fwd3_stream_kernel<T, false><<<grid, kStreamWG, 0, s>>>(d, kMaxIter3, min_l);
+}
+
 template <typename T>
 int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T tol, int verbose,
               T* u, adt_info* info, int flags, void* stream) {
@@ -172,12 +194,11 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   d.info = (adt_info*)(base + off_info);
   ADT_HIP(hipMemcpyAsync(base + off_desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
   const adt_tune tn = current_tune<T>();
+  const bool mix = flags & ADT_F32_COMPUTE_F64;
   if (l <= tn.diag_max_l)
-    adt::fwd3_diag_kernel<T><<<1, kDiagWG, 0, s>>>((const adt::EventDesc*)(base + off_desc), kMaxIter3,
-                                                  tn.diag_max_l);
+    launch_fwd_diag<T>(mix, 1, (const adt::EventDesc*)(base + off_desc), tn.diag_max_l, s);
   else
-    adt::fwd3_stream_kernel<T><<<1, kStreamWG, 0, s>>>((const adt::EventDesc*)(base + off_desc),
-                                                      kMaxIter3, tn.diag_max_l);
+    launch_fwd_stream<T>(mix, 1, (const adt::EventDesc*)(base + off_desc), tn.diag_max_l, s);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -436,12 +457,13 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, con
   if (nev == 0) return ADT_OK;
   if (flags & ADT_FORWARD) {
     // events with l <= diag_max_l: diagonal streaming; longer ones: k-streaming
+    const bool mix = flags & ADT_F32_COMPUTE_F64;
     if (tn.diag_max_l > 0) {
-      adt::fwd3_diag_kernel<T><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
+      launch_fwd_diag<T>(mix, nev, ddesc, tn.diag_max_l, s);
       ADT_HIP(hipGetLastError());
     }
     if (max_l > tn.diag_max_l) {
-      adt::fwd3_stream_kernel<T><<<nev, kStreamWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
+      launch_fwd_stream<T>(mix, nev, ddesc, tn.diag_max_l, s);
       ADT_HIP(hipGetLastError());
     }
   }
@@ -667,7 +689,7 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
         adt_tune c = t;
         c.diag_max_l = dml;
         float ms = 0.f;
-        if ((rc = time_it(ADT_FORWARD, c, &ms))) break;
+        if ((rc = time_it(ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), c, &ms))) break;
         if (ms < fbest) {
           fbest = ms;
           t.diag_max_l = dml;
@@ -675,7 +697,9 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
       }
       if (rc) break;
       // leave this order's best forward in the outputs for the adjoint
-      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD, plan.max_l, t, s))) break;
+      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), plan.max_l, t,
+                                s)))
+        break;
     }
     if (flags & ADT_ADJOINT) {
       abest = 3.4e38f;

@@ -79,7 +79,9 @@ __global__ void fwd3_plane_kernel(T* u, const T* f, T h, int m, int n, int l, in
                                   int a_lo, int a_cnt);
 template <typename T>
 __global__ void fwd3_event_kernel(const EventDesc* evs, int max_iter);
-template <typename T>
+// MIX (T = float only): FP32 storage with every local solve in FP64,
+// rounded once to FP32.
+template <typename T, bool MIX>
 __global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter, int min_l);
 // Bytes of one node's record in the 3D adjoint workspace (lambda and three
 // coefficients).
@@ -87,7 +89,7 @@ template <typename T>
 constexpr size_t kAdjRecBytes = 4 * sizeof(T);
 // Largest k extent handled by fwd3_diag_kernel (8 waves x 8 nodes).
 constexpr int kDiagMaxL = 64;
-template <typename T>
+template <typename T, bool MIX>
 __global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
 template <typename T>
 __global__ void copy_kernel(T* dst, const T* src, int64_t N);

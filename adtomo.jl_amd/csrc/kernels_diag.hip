@@ -102,6 +102,7 @@ struct DSweep {
   int64_t tA, tB, tC, base0;
   bool flipA, flipB, flipC;
   T h;
+  double hd;    // h for the FP64 local solve of the FP32-storage variant
   rsrc_t u;     // travel times (read old / write new in place)
   rsrc_t f;
 };
@@ -352,7 +353,7 @@ __device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> p
 // values are stored at the start of the next step, before any load is
 // waited for.  So the compiler's wait counts stay exact on every path into
 // the loop body: landing a load never waits for a store issued after it.
-template <typename T, bool FULL>
+template <typename T, bool FULL, bool MIX>
 __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
                                                 lptr<T> xch, lptr<T> xold, lptr<T> planes) {
   using P = Pieces<T>;
@@ -494,7 +495,11 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
             const T ux = xf ? xp : (xl ? xm : smin(xp, xm));
             const T uy = yf ? yp : (yl ? ym : smin(yp, ym));
             const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == L - 1);
-            const T cand = solution3d(ux, uy, mc, fe, S.h);
+            T cand;
+            if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
+              cand = (T)solution3d<double>(ux, uy, mc, fe, S.hd);
+            else
+              cand = solution3d(ux, uy, mc, fe, S.h);
             OWN[e] = smin(cand, OWN[e]);
           }
           if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
@@ -521,7 +526,7 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
 // One sweep of one event by the whole workgroup.  Waves whose segment has
 // kLW nodes take the fixed-length variant; the others (and idle waves) the
 // general one.  Both run the same number of steps and barriers.
-template <typename T>
+template <typename T, bool MIX>
 __device__ __forceinline__ void diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> xold,
                                            lptr<T> planes) {
   // the wave index is uniform across the wave: keep it (and everything
@@ -530,12 +535,12 @@ __device__ __forceinline__ void diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr
   int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
   if (cnt == kLW)
-    diag_sweep_wave<T, true>(S, w, cnt, xch, xold, planes);
+    diag_sweep_wave<T, true, MIX>(S, w, cnt, xch, xold, planes);
   else
-    diag_sweep_wave<T, false>(S, w, cnt, xch, xold, planes);
+    diag_sweep_wave<T, false, MIX>(S, w, cnt, xch, xold, planes);
 }
 
-template <typename T>
+template <typename T, bool MIX>
 __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __restrict__ evs,
                                                              int max_iter, int max_l) {
   __shared__ T xch[2 * kDW * 64];
@@ -556,6 +561,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   DSweep<T> S;
   S.g = make_dgeom(m, n, l);
   S.h = (T)ev.h;
+  S.hd = ev.h;
   S.u = make_rsrc(ev.u, bytes);
   S.f = make_rsrc(ev.f, bytes);
   const T tol = (T)ev.tol;
@@ -573,7 +579,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
       S.tC = S.flipC ? -1 : 1;
       S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
-      diag_sweep<T>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
+      diag_sweep<T, MIX>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
     }
     // err = max |u - u_old| over the event (coalesced pass after the 8
     // sweeps), which also leaves u_old = u for the next iteration
@@ -612,7 +618,8 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   }
 }
 
-template __global__ void fwd3_diag_kernel<double>(const EventDesc*, int, int);
-template __global__ void fwd3_diag_kernel<float>(const EventDesc*, int, int);
+template __global__ void fwd3_diag_kernel<double, false>(const EventDesc*, int, int);
+template __global__ void fwd3_diag_kernel<float, false>(const EventDesc*, int, int);
+template __global__ void fwd3_diag_kernel<float, true>(const EventDesc*, int, int);
 
 }  // namespace adt

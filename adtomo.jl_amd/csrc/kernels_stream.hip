@@ -105,6 +105,7 @@ struct Sweep {
   int64_t tA, tB, tC, base0;
   bool flipA, flipB, flipC;
   T h;
+  double hd;      // h for the FP64 local solve of the FP32-storage variant
   rsrc_t src;     // old values (u0 in the very first sweep)
   rsrc_t dst;     // u
   rsrc_t f;
@@ -174,7 +175,7 @@ __device__ __forceinline__ void glob_fetch(const Sweep<T>& S, Lane<T>& ln, int s
 // in two 8-node halves: F0 (elements 0-7, loaded beforehand) and F1
 // (elements 8-15, loaded at element 0); at element 8, F0 is refilled with
 // the first half of the next chunk (column ncol, offset nc0).
-template <typename T>
+template <typename T, bool MIX>
 __device__ __forceinline__ void process_chunk(const Sweep<T>& S, Lane<T>& ln, const Col& col,
                                               bool afirst, bool alast, bool bfirst, bool blast,
                                               int c0, T (&A)[kC], const T (&B)[kC],
@@ -213,7 +214,11 @@ __device__ __forceinline__ void process_chunk(const Sweep<T>& S, Lane<T>& ln, co
       const T ux = g.a_is_i ? ma : mb;
       const T uy = g.a_is_i ? mb : ma;
       const T fe = e < kC / 2 ? F0[e % (kC / 2)] : F1[e % (kC / 2)];
-      const T cand = solution3d(ux, uy, mc, fe, S.h);
+      T cand;
+      if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
+        cand = (T)solution3d<double>(ux, uy, mc, fe, S.hd);
+      else
+        cand = solution3d(ux, uy, mc, fe, S.h);
       un = smin(cand, own);
     }
     A[e] = un;
@@ -228,7 +233,7 @@ __device__ __forceinline__ void process_chunk(const Sweep<T>& S, Lane<T>& ln, co
 }  // namespace
 
 // One sweep of one event by the whole workgroup.
-template <typename T, bool SNAP>
+template <typename T, bool SNAP, bool MIX>
 __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_old) {
   // Local copy: keeps the fields in registers instead of re-reading them
   // through a generic pointer after every store.
@@ -310,7 +315,7 @@ __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_
       load_n(B, S.src, cn, cn0, L, S.tC);
     }
     if (SNAP) store_chunk(A, S.snap_w, cur, c0, L, S.tC);
-    process_chunk<T>(S, ln, cur, af, al, bf, bl, c0, A, B, F0, F1, cn, cn0, prev_new, lds_new,
+    process_chunk<T, MIX>(S, ln, cur, af, al, bf, bl, c0, A, B, F0, F1, cn, cn0, prev_new, lds_new,
                      lds_old);
   };
   for (int j = 0; j < chunks; j += 2) {
@@ -332,7 +337,7 @@ __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_
   __syncthreads();
 }
 
-template <typename T>
+template <typename T, bool MIX>
 __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* __restrict__ evs,
                                                               int max_iter, int min_l) {
   __shared__ T lds_new[2 * kW * 64];
@@ -346,6 +351,7 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
   Sweep<T> S;
   S.g = make_geom(m, n, l);
   S.h = (T)ev.h;
+  S.hd = ev.h;
   S.f = make_rsrc(ev.f, bytes);
   S.dst = make_rsrc(ev.u, bytes);
   S.snap_w = make_rsrc(ev.uold, bytes);
@@ -366,9 +372,9 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
       S.src = make_rsrc((it == 0 && sw == 0) ? ev.u0 : ev.u, bytes);
       if (sw == 0 && it > 0)
-        stream_sweep<T, true>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
+        stream_sweep<T, true, MIX>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
       else
-        stream_sweep<T, false>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
+        stream_sweep<T, false, MIX>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
     }
     // err = max |u - u_old| over the event (coalesced pass after the 8 sweeps)
     uint64_t key = 0;
@@ -403,7 +409,8 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
   }
 }
 
-template __global__ void fwd3_stream_kernel<double>(const EventDesc*, int, int);
-template __global__ void fwd3_stream_kernel<float>(const EventDesc*, int, int);
+template __global__ void fwd3_stream_kernel<double, false>(const EventDesc*, int, int);
+template __global__ void fwd3_stream_kernel<float, false>(const EventDesc*, int, int);
+template __global__ void fwd3_stream_kernel<float, true>(const EventDesc*, int, int);
 
 }  // namespace adt

@@ -12,7 +12,18 @@ bytes: forward 8*K*N*(s_f + 2*s_u) per event, adjoint
 N*(s_u+s_u0+s_f+s_g+s_gradf+s_gradu0)), and the CPU oracle timed on a bounded
 sample of the same events on the host cores.
 
+Other workloads (``--workload``), each printing its own JSON line:
+  uneven  256 events on boxes drawn from 48x48x24 .. 128x128x64 (FP64 and
+          FP32, errors against FP64), one hipGraph launch per step
+  joint   joint P/S gradient evaluation on 200x200x80 (h = 0.5 km): 100
+          surface stations x {P, S} = 200 solves, 500 events, Huber residual
+          (delta 0.1 s), picks with N(0, 0.05 s) noise; plus one L-BFGS step
+  large   400x400x160 FP32 storage with FP64 local solves (h = 0.25 km):
+          150 stations in chunks, 2000 events; against FP64 (and pure FP32)
+          on a 16-station subset
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nev 256]
+                       [--workload headline|uneven|joint|large]
 """
 import argparse
 import json
@@ -47,6 +58,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 variant")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch setting instead of timing the candidates")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "uneven", "joint", "large"])
+    ap.add_argument("--chunk", type=int, default=0, help="stations per batched launch (0: all)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -62,6 +76,8 @@ def main():
     from adtomo_hip import synth
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.workload != "headline":
+        return {"uneven": run_uneven, "joint": run_joint, "large": run_large}[args.workload](args)
     if args.calib_copy:
         # 2^27 doubles = 1 GiB read + 1 GiB written by the library's plain
         # copy kernel (arith_kernel<double>, op 6)
@@ -239,6 +255,220 @@ def main():
         "step_gbs_algorithmic": total_gbs,
     }
     print(json.dumps(out), flush=True)
+
+
+def _line(metric, value, unit, ms, args, dtype, config, **extra):
+    out = {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def _time(fn, steps):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / steps * 1e3
+
+
+def _rel_err(a, b):
+    """(max |a-b| / max |b|, rms(a-b)) over lists of device tensors."""
+    mx = ref = ss = cnt = 0.0
+    for x, y in zip(a, b):
+        d = x.double() - y.double()
+        mx = max(mx, float(d.abs().max()))
+        ref = max(ref, float(y.double().abs().max()))
+        ss += float((d * d).sum())
+        cnt += d.numel()
+    return mx / max(ref, 1e-300), (ss / max(cnt, 1)) ** 0.5
+
+
+def run_uneven(args):
+    """256 events on uneven boxes, FP64 and FP32 (one graph launch per step)."""
+    import numpy as np
+    import torch
+    import adtomo_hip as A
+    from adtomo_hip import synth
+    nev = args.nev
+    fs, u0s, gs, _ = synth.uneven_box_set(nev)
+    h, tol = 0.5, 1e-3
+    nodes = float(sum(f.size for f in fs))
+    res = {}
+    for name, dt, mix in (("f64", torch.float64, False), ("f32", torch.float32, False),
+                          ("mixed", torch.float32, True)):
+        fd = [torch.from_numpy(a).to("cuda", dt) for a in fs]
+        ud = [torch.from_numpy(a).to("cuda", dt) for a in u0s]
+        gd = [torch.from_numpy(a).to("cuda", dt) for a in gs]
+        tune = A.autotune(ud, fd, [h] * nev, [tol] * nev, grad_us=gd, reps=1, compute_f64=mix)
+        G = A.EikonalGraph(ud, fd, [h] * nev, [tol] * nev, grad_us=gd, compute_f64=mix)
+        for _ in range(args.warmup):
+            G.launch()
+        ms = _time(G.launch, args.steps)
+        K = np.array([i["iterations"] for i in G.infos()])
+        res[name] = (G, ms, K, tune)
+        log(f"uneven {name}: {ms:.2f} ms/step -> {nev / ms * 1e3:.1f} event-solves/s, K mean {K.mean():.2f}, {tune}")
+    G64, ms64, K64, tune64 = res["f64"]
+
+    def variant(name):
+        G, ms_, K, _ = res[name]
+        eu = _rel_err(G.us, G64.us)
+        eg = _rel_err(G.grad_fs, G64.grad_fs)
+        return {"event_solves_per_s": nev / ms_ * 1e3, "ms_per_step": ms_, "K_mean": float(K.mean()),
+                "u_err_max_rel": eu[0], "u_err_rms_s": eu[1], "grad_f_err_max_rel": eg[0],
+                "grad_f_err_rms": eg[1]}
+    _line("event-solves/s fwd+adjoint, 256 events on uneven boxes 48x48x24..128x128x64 (batched)",
+          nev / ms64 * 1e3, "event-solves/s", ms64, args, "f64",
+          {"workload": "256 events, boxes uniform in 48x48x24..128x128x64, h=0.5 km, layered "
+                       "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
+           "events": nev, "nodes_total": nodes, "K_mean": float(K64.mean()),
+           "launch_setting": {k: tune64[k] for k in ("diag_max_l", "adj_wg", "order")}},
+          fp32=variant("f32"), fp32_storage_fp64_solve=variant("mixed"))
+
+
+def _picks(vel, h, stations, events, chunk):
+    """Travel times (FP64, device) from each station to every event on
+    model ``vel``: (nstation, nevent) numpy array."""
+    import numpy as np
+    import torch
+    import adtomo_hip as A
+    from adtomo_hip import objective as O
+    v = torch.from_numpy(np.ascontiguousarray(vel)).cuda()
+    f = O.slowness(v, 1.0)
+    ne = len(events[0])
+    zero = torch.zeros(ne, dtype=torch.float64, device="cuda")
+    out = np.empty((len(stations), ne))
+    for s0 in range(0, len(stations), chunk):
+        st = stations[s0:s0 + chunk]
+        u0s = O.source_u0(st, v, h)
+        r = A.eikonal3d_batched(u0s, [f] * len(st), [h] * len(st), [1e-3] * len(st),
+                                return_info=False)
+        mis = O.misfit(r["u"], *events, [zero] * len(st), [zero] * len(st), grad=False)
+        for k in range(len(st)):
+            out[s0 + k] = mis["tcal"][k].cpu().numpy()
+        del r, u0s, mis
+    return out
+
+
+def run_joint(args):
+    """Joint P/S gradient evaluations and one L-BFGS step on 200x200x80."""
+    import numpy as np
+    from adtomo_hip import synth
+    from adtomo_hip.inversion import TravelTimeInversion
+    shape, h = (200, 200, 80), 0.5
+    t0 = time.time()
+    vp = synth.layered_grf_model(shape, h, 5.0, 7.0, 0.04, 5.0, seed=41)
+    vs = vp / 1.73 * (1.0 + synth.gaussian_random_field(shape, 5.0 / h, 0.02, seed=42))
+    vel0 = synth.layered_model(shape, 5.0, 7.0)
+    r = synth.rng(43)
+    stations = [(float(r.uniform(1, shape[0])), float(r.uniform(1, shape[1])), 1.0)
+                for _ in range(100)]
+    events = tuple(r.uniform(1, s, 500) for s in shape)
+    tp = _picks(vp, h, stations, events, 100) + r.normal(0, 0.05, (100, 500))
+    ts = _picks(vs, h, stations, events, 100) + r.normal(0, 0.05, (100, 500))
+    w = np.ones((100, 500))
+    log(f"joint workload built in {time.time() - t0:.1f}s")
+    inv = TravelTimeInversion(vel0, h, stations, events, {"P": tp, "S": ts}, {"P": w, "S": w},
+                              smooth=(3, 3), lam=0.1, phases=("P", "S"), vpvs=1.7,
+                              huber_delta=0.1, chunk=args.chunk or None)
+    p0 = inv.initial_params()
+    L0 = None
+    for _ in range(args.warmup):
+        L0, g0 = inv.objective(p0)
+    ms = _time(lambda: inv.objective(p0), args.steps)
+    log(f"joint: {ms:.1f} ms per gradient evaluation (200 solves fwd+adj + objective), loss {L0}")
+    from scipy.optimize import minimize
+    hist = []
+
+    def fun(x):
+        L, g = inv.objective(x)
+        hist.append(L)
+        return L, g
+    t = time.perf_counter()
+    res = minimize(fun, p0, jac=True, method="L-BFGS-B", options={"maxiter": 1})
+    t = time.perf_counter() - t
+    log(f"one L-BFGS step: {len(hist)} evaluations in {t:.2f}s, loss {hist[0]:.4g} -> {res.fun:.4g}, "
+        f"vp/vs {p0[-1]:.3f} -> {res.x[-1]:.4f}")
+    inv.close()
+    _line("joint P/S gradient evaluations: solves/s (fwd+adjoint) at 200x200x80",
+          200 / ms * 1e3, "solves/s", ms, args, "f64",
+          {"workload": "joint Vp/Vs gradient, 200x200x80, h=0.5 km, 100 stations x {P,S}, "
+                       "500 events, Huber delta 0.1 s, noise N(0, 0.05 s)",
+           "solves": 200, "events": 500, "chunk": args.chunk or 100},
+          lbfgs_step={"evaluations": len(hist), "seconds": t, "loss_before": hist[0],
+                      "loss_after": float(res.fun), "vpvs_after": float(res.x[-1])})
+
+
+def run_large(args):
+    """400x400x160 FP32 gradient in station chunks; FP64 check on 16 stations."""
+    import numpy as np
+    import torch
+    from adtomo_hip import synth
+    from adtomo_hip.inversion import TravelTimeInversion
+    shape, h = (400, 400, 160), 0.25
+    chunk = args.chunk or 50
+    t0 = time.time()
+    vtrue = synth.layered_grf_model(shape, h, 3.5, 7.5, 0.03, 5.0, seed=51)
+    vel0 = synth.layered_model(shape, 3.5, 7.5)
+    r = synth.rng(52)
+    ns, ne = 150, 2000
+    stations = [(float(r.uniform(1, shape[0])), float(r.uniform(1, shape[1])), 1.0)
+                for _ in range(ns)]
+    events = tuple(r.uniform(1, s, ne) for s in shape)
+    tobs = _picks(vtrue, h, stations, events, chunk) + r.normal(0, 0.05, (ns, ne))
+    del vtrue
+    w = np.ones((ns, ne))
+    log(f"large workload built in {time.time() - t0:.1f}s")
+    inv = TravelTimeInversion(vel0, h, stations, events, tobs, w, smooth=(3, 3), lam=0.1,
+                              dtype="float32", chunk=chunk, compute_f64=True)
+    p0 = inv.initial_params()
+    for _ in range(args.warmup):
+        inv.objective(p0)
+    ms = _time(lambda: inv.objective(p0), args.steps)
+    peak = torch.cuda.max_memory_allocated() / 2**30
+    log(f"large: {ms / 1e3:.2f} s per FP32-storage gradient evaluation ({ns} solves, chunks of "
+        f"{chunk}, FP64 local solves), peak torch device memory {peak:.1f} GiB")
+    inv.close()
+    del inv
+    torch.cuda.empty_cache()
+    # FP32 against FP64 on a 16-station subset
+    sub = slice(0, 16)
+    g = {}
+    for name, dt, mix in (("mixed", "float32", True), ("float32", "float32", False),
+                          ("float64", "float64", False)):
+        # no regulariser here: at x = 0 on a layered model fvar - box(fvar)
+        # is rounding noise, and its sign would dominate the comparison
+        iv = TravelTimeInversion(vel0, h, stations[sub], events, tobs[sub], w[sub], smooth=(3, 3),
+                                 lam=0.0, dtype=dt, compute_f64=mix)
+        t = time.perf_counter()
+        L, gr = iv.objective(p0)
+        t = time.perf_counter() - t
+        g[name] = (L, gr, [u.double() for u in iv.travel_times()], t)
+        iv.close()
+        del iv
+        torch.cuda.empty_cache()
+        log(f"16-station subset, {name}: {t:.2f} s")
+
+    def errs(name):
+        eu = _rel_err(g[name][2], g["float64"][2])
+        gd = np.abs(g[name][1] - g["float64"][1])
+        return {"u_err_max_rel": eu[0], "u_err_rms_s": eu[1],
+                "loss_rel": abs(g[name][0] - g["float64"][0]) / abs(g["float64"][0]),
+                "grad_err_max_rel": float(gd.max() / np.abs(g["float64"][1]).max()),
+                "grad_err_rms": float((gd ** 2).mean() ** 0.5), "seconds": g[name][3]}
+    _line("FP32-storage gradient evaluations: solves/s (fwd+adjoint) at 400x400x160",
+          ns / ms * 1e3, "solves/s", ms, args, "f32",
+          {"workload": "400x400x160 (25.6M nodes), h=0.25 km, layered 3.5-7.5 km/s + 3% GRF, "
+                       f"{ns} stations in chunks of {chunk}, {ne} events, FP32 storage with "
+                       "FP64 local solves",
+           "solves": ns, "events": ne, "chunk": chunk},
+          peak_torch_device_gib=peak,
+          vs_fp64_16_stations={"fp32_storage_fp64_solve": errs("mixed"),
+                               "fp32": errs("float32"), "fp64_seconds": g["float64"][3]})
 
 
 if __name__ == "__main__":

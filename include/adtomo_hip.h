@@ -48,6 +48,11 @@ extern "C" {
 /* Batched calls: run the forward, the adjoint, or both. */
 #define ADT_FORWARD 0x2
 #define ADT_ADJOINT 0x4
+/* _f32 calls: FP32 storage with every forward local solve computed in FP64
+ * and rounded once to FP32 (the pure binary32 forward loses accuracy to
+ * cancellation in the 3D quadratic when travel times are large against
+ * f*h).  No effect on _f64 calls. */
+#define ADT_F32_COMPUTE_F64 0x8
 
 #define ADT_MAX_LOGGED_ITERS 100
 

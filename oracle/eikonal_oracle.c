@@ -46,6 +46,60 @@
 #undef FABS
 #undef C
 
+/* FP32 storage with FP64 local solves (the library's ADT_F32_COMPUTE_F64
+ * variant): the neighbour minima are binary32 values, the local solve of
+ * Eikonal3D.cpp:11-28 runs in binary64 on them and its result is rounded
+ * once to binary32 before u = min(u_new, u); the stop rule is the binary32
+ * twin's.  Same loops as sweep3d / adt_oracle_eikonal3d_fwd. */
+#define U3(i, j, k) u[(i) * n * l + (j) * l + (k)]
+static void sweep3d_m32(float *u, const float *f, double h, int64_t m, int64_t n, int64_t l,
+                        int dI, int dJ, int dK)
+{
+    int64_t i0 = dI == 1 ? 0 : m - 1, i1 = dI == 1 ? m : -1;
+    int64_t j0 = dJ == 1 ? 0 : n - 1, j1 = dJ == 1 ? n : -1;
+    int64_t k0 = dK == 1 ? 0 : l - 1, k1 = dK == 1 ? l : -1;
+    for (int64_t i = i0; i != i1; i += dI)
+        for (int64_t j = j0; j != j1; j += dJ)
+            for (int64_t k = k0; k != k1; k += dK) {
+                float uxmin = i == 0 ? U3(i + 1, j, k)
+                              : (i == m - 1 ? U3(i - 1, j, k)
+                                            : smin_f32(U3(i + 1, j, k), U3(i - 1, j, k)));
+                float uymin = j == 0 ? U3(i, j + 1, k)
+                              : (j == n - 1 ? U3(i, j - 1, k)
+                                            : smin_f32(U3(i, j + 1, k), U3(i, j - 1, k)));
+                float uzmin = k == 0 ? U3(i, j, k + 1)
+                              : (k == l - 1 ? U3(i, j, k - 1)
+                                            : smin_f32(U3(i, j, k + 1), U3(i, j, k - 1)));
+                float u_new = (float)solution3d_f64((double)uxmin, (double)uymin, (double)uzmin,
+                                                    (double)f[i * n * l + j * l + k], h);
+                U3(i, j, k) = smin_f32(u_new, U3(i, j, k));
+            }
+}
+#undef U3
+
+int adt_oracle_eikonal3d_fwd_m32(const float *u0, const float *f, double h, int64_t m,
+                                 int64_t n, int64_t l, float tol, int max_iter, int verbose,
+                                 float *u, float *uold, int *iters_out, double *err_out)
+{
+    const int64_t N = m * n * l;
+    memcpy(u, u0, sizeof(float) * (size_t)N);
+    int it, ran = 0;
+    float err = 0;
+    for (it = 0; it < max_iter; it++) {
+        memcpy(uold, u, sizeof(float) * (size_t)N);
+        for (int s = 0; s < 8; s++)
+            sweep3d_m32(u, f, h, m, n, l, kDirs_f32[s][0], kDirs_f32[s][1], kDirs_f32[s][2]);
+        err = 0.0f;
+        for (int64_t p = 0; p < N; p++) err = smax_f32(fabsf(u[p] - uold[p]), err);
+        ran = it + 1;
+        if (verbose) printf("Iteration %d, Error = %0.6e\n", it, (double)err);
+        if (err < tol) break;
+    }
+    if (iters_out) *iters_out = ran;
+    if (err_out) *err_out = (double)err;
+    return 0;
+}
+
 /* Batched CPU baseline: `nev` independent 3D events with per-event boxes,
  * each solved serially (forward then adjoint) exactly as the reference does
  * per MPI rank.  Events are spread over `nthreads` POSIX threads, one event

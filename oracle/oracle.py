@@ -62,6 +62,10 @@ def lib():
             fn = getattr(L, "adt_oracle_eikonal3d_bwd" + sfx)
             fn.argtypes = [_p, _p, _p, _p, real, _i64, _i64, _i64, _p, _p, _p]
             fn.restype = _int
+        L.adt_oracle_eikonal3d_fwd_m32.argtypes = [_p, _p, _d, _i64, _i64, _i64, ctypes.c_float,
+                                                   _int, _int, _p, _p, ctypes.POINTER(_int),
+                                                   ctypes.POINTER(_d)]
+        L.adt_oracle_eikonal3d_fwd_m32.restype = _int
         L.adt_oracle_eikonal3d_batch_f64.argtypes = [
             _int, _p, _p, _p, _p, _p, _d, _p, _p, _p, _p, _int]
         L.adt_oracle_eikonal3d_batch_f64.restype = _int
@@ -129,6 +133,21 @@ def eikonal3d(u0, f, h: float, tol: float, dtype=np.float64, max_iter: int = 20,
     getattr(lib(), "adt_oracle_eikonal3d_fwd" + _sfx(dtype))(
         _ptr(u0), _ptr(f), h, m, n, l, tol, max_iter, int(verbose), _ptr(u), _ptr(uold),
         ctypes.byref(it), ctypes.byref(err))
+    return u, it.value, err.value
+
+
+def eikonal3d_m32(u0, f, h: float, tol: float, max_iter: int = 20):
+    """FP32-storage forward with FP64 local solves (the library's
+    ADT_F32_COMPUTE_F64 variant).  Returns (u float32, iters, err)."""
+    u0 = _c(u0, np.float32)
+    f = _c(f, np.float32)
+    m, n, l = u0.shape
+    u = np.empty_like(u0)
+    uold = np.empty_like(u0)
+    it = _int(0)
+    err = _d(0.0)
+    lib().adt_oracle_eikonal3d_fwd_m32(_ptr(u0), _ptr(f), h, m, n, l, tol, max_iter, 0, _ptr(u),
+                                       _ptr(uold), ctypes.byref(it), ctypes.byref(err))
     return u, it.value, err.value
 
 

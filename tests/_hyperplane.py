@@ -47,7 +47,9 @@ def _planes(shape):
     return a.reshape(-1)[order], b.reshape(-1)[order], c.reshape(-1)[order], bounds
 
 
-def sweep3d(u, f, h, d):
+def sweep3d(u, f, h, d, mixed=False):
+    """One sweep in hyperplane order.  mixed: u and f are binary32 arrays,
+    each local solve runs in binary64 and is rounded once to binary32."""
     m, n, l = u.shape
     A, Bb, Cc, bounds = _planes(u.shape)
     for s in range(m + n + l - 2):
@@ -65,22 +67,29 @@ def sweep3d(u, f, h, d):
         ux = nbmin(i, m, lambda t: u[t, j, k])
         uy = nbmin(j, n, lambda t: u[i, t, k])
         uz = nbmin(k, l, lambda t: u[i, j, t])
-        un = solution3d(ux, uy, uz, f[i, j, k], h)
+        if mixed:
+            un = solution3d(ux.astype(np.float64), uy.astype(np.float64), uz.astype(np.float64),
+                            f[i, j, k].astype(np.float64), h).astype(np.float32)
+        else:
+            un = solution3d(ux, uy, uz, f[i, j, k], h)
         cur = u[i, j, k]
         u[i, j, k] = np.where(cur < un, cur, un)
 
 
-def forward3d(u0, f, h, tol, max_iter=20):
-    u = np.array(u0, dtype=np.float64, copy=True)
-    f = np.asarray(f, dtype=np.float64)
+def forward3d(u0, f, h, tol, max_iter=20, mixed=False):
+    """mixed: binary32 storage and stop rule, binary64 local solves."""
+    dt = np.float32 if mixed else np.float64
+    u = np.array(u0, dtype=dt, copy=True)
+    f = np.asarray(f, dtype=dt)
+    tol = dt(tol)
     it = 0
-    err = 0.0
+    err = dt(0.0)
     for it in range(1, max_iter + 1):
         uold = u.copy()
         for d in DIRS3:
-            sweep3d(u, f, h, d)
+            sweep3d(u, f, h, d, mixed)
         diff = np.abs(u - uold)
-        err = 0.0
+        err = dt(0.0)
         for v in diff.reshape(-1):
             err = v if err < v else err
         if err < tol:

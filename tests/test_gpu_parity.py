@@ -423,3 +423,33 @@ def test_autotune_stores_a_valid_setting():
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
                    order=base["order"])
+
+
+# ------------------------------------------- FP32 storage, FP64 solves ---
+
+def test_fp32_storage_fp64_solve_bit_exact():
+    # both forward kernels (l <= 64 and l > 64), eager and graph, against the
+    # oracle's mixed twin; the adjoint then runs in FP32 on these times
+    import adtomo_hip as A
+    fs, u0s, gs = _mixed_batch(500)
+    f32 = [dev(a, np.float32) for a in fs]
+    u32 = [dev(a, np.float32) for a in u0s]
+    g32 = [dev(a, np.float32) for a in gs]
+    res = A.eikonal3d_batched(u32, f32, [0.5] * len(fs), [1e-4] * len(fs), grad_us=g32,
+                              compute_f64=True)
+    G = A.EikonalGraph(u32, f32, [0.5] * len(fs), [1e-4] * len(fs), compute_f64=True)
+    G.launch()
+    ginfo = G.infos()
+    worst = 0.0
+    for e in range(len(fs)):
+        um, it, _ = oracle.eikonal3d_m32(u0s[e], fs[e], 0.5, 1e-4)
+        assert res["info"][e]["iterations"] == it and ginfo[e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), um), e
+        assert bits_equal(host(G.us[e]), um), e
+        u64, _, _ = oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-4)
+        worst = max(worst, rel(um.astype(np.float64), u64))
+        _, gf32, _ = oracle.eikonal3d_grad(gs[e], um, u0s[e], fs[e], 0.5, dtype=np.float32)
+        assert rel(host(res["grad_f"][e]), gf32) <= 1e-5, e
+    G.close()
+    print(f"FP32 storage / FP64 solve vs FP64: max rel travel-time err {worst:.3e}", flush=True)
+    assert worst < 1e-6

@@ -38,6 +38,38 @@ def test_forward3d_matches_hyperplane_order(shape, seed):
         np.testing.assert_array_equal(u.view(np.uint64), uh.view(np.uint64))
 
 
+@pytest.mark.parametrize("shape,seed", [((7, 6, 5), 20), ((9, 5, 11), 21), ((2, 3, 4), 22)])
+def test_forward3d_mixed_matches_hyperplane_order(shape, seed):
+    # FP32 storage, FP64 local solves: the independent numpy formulation
+    # (binary32 arrays, binary64 solve rounded once) agrees bit for bit
+    r = synth.rng(seed)
+    f = r.uniform(0.5, 2.0, size=shape).astype(np.float32)
+    src = tuple(int(r.integers(0, s)) for s in shape)
+    u0 = synth.point_source_u0(shape, src).astype(np.float32)
+    for tol in (1e-3, 0.0):
+        u, it, err = oracle.eikonal3d_m32(u0, f, 0.7, tol)
+        uh, ith, errh = hp.forward3d(u0, f, 0.7, tol, mixed=True)
+        assert u.dtype == np.float32 and it == ith and err == float(errh)
+        np.testing.assert_array_equal(u.view(np.uint32), uh.view(np.uint32))
+
+
+def test_forward3d_mixed_is_fp64_rounded_within_storage_error():
+    # on a grid where the binary32 twin loses accuracy to cancellation in the
+    # 3D quadratic (travel times large against f*h), the mixed variant stays
+    # within a few binary32 ulps of FP64 while the binary32 twin does not
+    shape = (40, 36, 20)
+    f = synth.rng(23).uniform(0.5, 0.6, size=shape)
+    u0 = synth.point_source_u0(shape, (0, 0, 0))
+    h = 0.05
+    u64, _, _ = oracle.eikonal3d(u0, f, h, 0.0, max_iter=6)
+    um, _, _ = oracle.eikonal3d_m32(u0, f, h, 0.0, max_iter=6)
+    u32, _, _ = oracle.eikonal3d(u0, f, h, 0.0, dtype=np.float32, max_iter=6)
+    em = np.abs(um.astype(np.float64) - u64).max() / u64.max()
+    e32 = np.abs(u32.astype(np.float64) - u64).max() / u64.max()
+    assert em < 2e-6
+    assert e32 > 10 * em
+
+
 def test_forward3d_random_u0_matches_hyperplane():
     # gradtest.jl style: u0 = rand * 1000 (deps/CustomOps/Eikonal3D/gradtest.jl:43)
     r = synth.rng(11)

@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the forward kernel on this workload (256 events),
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy kernel of
 # known size: profiles/pmc_traffic_fwd3_diag_adj3_kahn_256x128x128x64.md
-FWD_TRAFFIC_BYTES_256 = 144.4e9 + 58.6e9
+FWD_TRAFFIC_BYTES_256 = 112.5e9 + 60.5e9
 
 
 def log(*a):
@@ -241,7 +241,7 @@ def main():
                    "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
                    "adj_sweeps_mean": float(sweeps.mean()),
                    "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order")}},
-        "roofline": {"bound": "hbm", "kernel": "fwd3_diag_kernel<double>",
+        "roofline": {"bound": "hbm", "kernel": "fwd3_diag_kernel<double, false>",
                      "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fwd_gbs / HBM_PEAK_GBS, 5),
                      "traffic": FWD_TRAFFIC_BYTES_256 if nev == 256 else None},

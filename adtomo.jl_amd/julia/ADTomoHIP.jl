@@ -8,16 +8,19 @@
 # (deps/CustomOps/Eikonal/Eikonal.cpp:48-63) and EikonalThreeDGrad
 # (deps/CustomOps/Eikonal3D/EikonalThreeD.cpp:47-65).
 #
-# Arrays are plain Julia `Array{Float64}` (host memory, staged by the library
-# with ADT_HOST_POINTERS).  The reference's TF tensors are row-major, so the
+# The single-event ops take plain Julia `Array{Float64}` (host memory, staged
+# by the library with ADT_HOST_POINTERS); the batched, graph and objective
+# calls below take device arrays.  The reference's TF tensors are row-major, so the
 # 3D op sees flat index i*n*l + j*l + k: a Julia (m,n,l) array is passed as
 # permutedims(A, (3,2,1)) and results are permuted back; the 2D (n+1, m+1)
 # matrix likewise goes through permutedims(f) (flat j*(m+1)+i).
 module ADTomoHIP
 
 using ChainRulesCore
+using Libdl
 
-export eikonal, eikonal3d
+export eikonal, eikonal3d, eikonal3d_batched!, EikonalGraph, launch!, close!, source_u0!,
+       misfit!
 
 const libadt = normpath(joinpath(@__DIR__, "..", "libadtomo_hip.so"))
 const ADT_HOST_POINTERS = Int32(0x1)
@@ -144,6 +147,173 @@ function ChainRulesCore.rrule(::typeof(eikonal3d), u0, f, h::Float64, m::Integer
                 NoTangent(), NoTangent(), NoTangent(), NoTangent(), NoTangent())
     end
     return u, eikonal3d_pullback
+end
+
+# ---------------------------------------------- batched, device arrays ---
+#
+# The per-station op loop of scripts/inversion.jl:47-62 as one launch over
+# device arrays (e.g. AMDGPU.jl `ROCArray`s; anything whose `pointer` is a
+# device address).  Device arrays carry the reference's flat layout: a grid
+# (m, n, l) is a column-major array of size (l, n, m), so its flat index is
+# i*n*l + j*l + k.  `stream` is a HIP stream handle (C_NULL: the null stream).
+
+const ADT_FORWARD = Int32(0x2)
+const ADT_ADJOINT = Int32(0x4)
+const ADT_F32_COMPUTE_F64 = Int32(0x8)
+
+# Mirrors `adt_event3d` in include/adtomo_hip.h.
+struct AdtEvent3d
+    m::Int64
+    n::Int64
+    l::Int64
+    h::Float64
+    tol::Float64
+    u0::Ptr{Cvoid}
+    f::Ptr{Cvoid}
+    grad_u::Ptr{Cvoid}
+    u::Ptr{Cvoid}
+    grad_u0::Ptr{Cvoid}
+    grad_f::Ptr{Cvoid}
+end
+
+devptr(A) = A === nothing ? C_NULL : Ptr{Cvoid}(UInt(pointer(A)))
+griddims(A) = (size(A, 3), size(A, 2), size(A, 1))   # (m, n, l) of an (l, n, m) array
+sfx(A) = eltype(A) == Float32 ? "_f32" : "_f64"
+
+function event_table(u0s, fs, us, hs, tols, grad_us, grad_u0s, grad_fs)
+    return [AdtEvent3d(griddims(fs[e])..., Float64(hs[e]), Float64(tols[e]), devptr(u0s[e]),
+                       devptr(fs[e]), grad_us === nothing ? C_NULL : devptr(grad_us[e]),
+                       devptr(us[e]), grad_u0s === nothing ? C_NULL : devptr(grad_u0s[e]),
+                       grad_fs === nothing ? C_NULL : devptr(grad_fs[e]))
+            for e in eachindex(fs)]
+end
+
+"""
+    eikonal3d_batched!(us, u0s, fs, hs, tols; grad_us, grad_u0s, grad_fs,
+                       forward=true, compute_f64=false, stream=C_NULL)
+
+Forward (and, with `grad_us`/`grad_fs`, adjoint) of every event in one
+launch; returns the per-event `AdtInfo`s.  Boxes may differ per event.
+"""
+function eikonal3d_batched!(us, u0s, fs, hs, tols; grad_us=nothing, grad_u0s=nothing,
+                            grad_fs=nothing, forward::Bool=true, compute_f64::Bool=false,
+                            stream::Ptr{Cvoid}=C_NULL)
+    tab = event_table(u0s, fs, us, hs, tols, grad_us, grad_u0s, grad_fs)
+    flags = (forward ? ADT_FORWARD : Int32(0)) | (grad_us === nothing ? Int32(0) : ADT_ADJOINT) |
+            (compute_f64 ? ADT_F32_COMPUTE_F64 : Int32(0))
+    infos = [AdtInfo() for _ in tab]
+    fn = Symbol("adt_eikonal3d_batched", sfx(fs[1]))
+    rc = ccall(Libdl.dlsym(Libdl.dlopen(libadt), fn), Cint,
+               (Ptr{AdtEvent3d}, Int32, Int32, Ptr{AdtInfo}, Ptr{Cvoid}),
+               tab, length(tab), flags, infos, stream)
+    check(rc, "eikonal3d_batched")
+    return infos
+end
+
+"""
+    EikonalGraph(us, u0s, fs, hs, tols; grad_us, grad_u0s, grad_fs, forward, compute_f64)
+
+hipGraph of the batched launch over fixed device buffers; `launch!(g)`
+replays it, `close!(g)` frees it.  The buffers must outlive the graph.
+"""
+mutable struct EikonalGraph
+    ptr::Ptr{Cvoid}
+end
+
+function EikonalGraph(us, u0s, fs, hs, tols; grad_us=nothing, grad_u0s=nothing, grad_fs=nothing,
+                      forward::Bool=true, compute_f64::Bool=false, stream::Ptr{Cvoid}=C_NULL)
+    tab = event_table(u0s, fs, us, hs, tols, grad_us, grad_u0s, grad_fs)
+    flags = (forward ? ADT_FORWARD : Int32(0)) | (grad_us === nothing ? Int32(0) : ADT_ADJOINT) |
+            (compute_f64 ? ADT_F32_COMPUTE_F64 : Int32(0))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:adt_graph_create, libadt), Cint,
+               (Ptr{AdtEvent3d}, Int32, Int32, Int32, Ref{Ptr{Cvoid}}, Ptr{Cvoid}),
+               tab, length(tab), flags, Int32(eltype(fs[1]) == Float32), out, stream)
+    check(rc, "adt_graph_create")
+    g = EikonalGraph(out[])
+    finalizer(close!, g)
+    return g
+end
+
+launch!(g::EikonalGraph; stream::Ptr{Cvoid}=C_NULL) =
+    check(ccall((:adt_graph_launch, libadt), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, stream),
+          "adt_graph_launch")
+
+function close!(g::EikonalGraph)
+    if g.ptr != C_NULL
+        ccall((:adt_graph_destroy, libadt), Cint, (Ptr{Cvoid},), g.ptr)
+        g.ptr = C_NULL
+    end
+    return nothing
+end
+
+# ------------------------------------------- objective, device arrays ---
+
+# Mirrors `adt_source3d` / `adt_misfit3d` in include/adtomo_hip.h.
+struct AdtSource3d
+    x::Float64
+    y::Float64
+    z::Float64
+    scale::Float64
+    u0::Ptr{Cvoid}
+end
+
+struct AdtMisfit3d
+    m::Int64
+    n::Int64
+    l::Int64
+    u::Ptr{Cvoid}
+    tobs::Ptr{Cvoid}
+    weight::Ptr{Cvoid}
+    tcal::Ptr{Cvoid}
+    grad_u::Ptr{Cvoid}
+    loss::Ptr{Cvoid}
+    huber_delta::Float64
+end
+
+"""
+    source_u0!(u0s, stations, vel0, h; scale=1.0, stream=C_NULL)
+
+Initial travel times of each station (x, y, z), 1-based reals, on the
+device (scripts/inversion.jl:48-60; `scale` = vp/vs for S).
+"""
+function source_u0!(u0s, stations, vel0, h; scale=1.0, stream::Ptr{Cvoid}=C_NULL)
+    tab = [AdtSource3d(Float64(s[1]), Float64(s[2]), Float64(s[3]), Float64(scale), devptr(u0s[e]))
+           for (e, s) in enumerate(stations)]
+    m, n, l = griddims(vel0)
+    fn = Symbol("adt_source_u0_3d_batched", sfx(vel0))
+    T = eltype(vel0)
+    rc = ccall(Libdl.dlsym(Libdl.dlopen(libadt), fn), Cint,
+               (Ptr{AdtSource3d}, Int32, Ptr{Cvoid}, Int64, Int64, Int64, T, Ptr{Cvoid}),
+               tab, length(tab), devptr(vel0), m, n, l, T(h), stream)
+    check(rc, "source_u0")
+    return u0s
+end
+
+"""
+    misfit!(loss, us, rx, ry, rz, tobs, weights; grad_us=nothing, tcals=nothing,
+            huber_delta=0.0, stream=C_NULL)
+
+Receiver gather + weighted misfit per station (scripts/inversion.jl:64-105):
+`loss` is a device vector (one entry per station), `rx, ry, rz` host event
+coordinates; `grad_us` receive dL/du for the adjoint.
+"""
+function misfit!(loss, us, rx, ry, rz, tobs, weights; grad_us=nothing, tcals=nothing,
+                 huber_delta=0.0, stream::Ptr{Cvoid}=C_NULL)
+    T = eltype(us[1])
+    lp = devptr(loss)
+    tab = [AdtMisfit3d(griddims(us[e])..., devptr(us[e]), devptr(tobs[e]), devptr(weights[e]),
+                       tcals === nothing ? C_NULL : devptr(tcals[e]),
+                       grad_us === nothing ? C_NULL : devptr(grad_us[e]),
+                       lp + (e - 1) * sizeof(T), Float64(huber_delta)) for e in eachindex(us)]
+    fn = Symbol("adt_misfit3d_batched", sfx(us[1]))
+    rc = ccall(Libdl.dlsym(Libdl.dlopen(libadt), fn), Cint,
+               (Ptr{AdtMisfit3d}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64,
+                Ptr{Cvoid}),
+               tab, length(tab), Vector{Float64}(rx), Vector{Float64}(ry), Vector{Float64}(rz),
+               length(rx), stream)
+    check(rc, "misfit")
+    return loss
 end
 
 end # module

@@ -93,3 +93,43 @@ def test_tune_get_set_validation_without_device():
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
                    order=base["order"])
+
+
+def _c_struct_fields(name):
+    import re
+    hdr = open(os.path.join(os.path.dirname(GOLD), "..", "include", "adtomo_hip.h")).read()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), hdr, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names = decl.split(None, 1)[1] if not decl.startswith("const ") else decl.split(None, 2)[2]
+        for nm in names.split(","):
+            nm = nm.strip().lstrip("*").split("[")[0].strip()
+            fields.append(nm)
+    return fields
+
+
+def _julia_struct_fields(name):
+    import re
+    jl = open(os.path.join(os.path.dirname(GOLD), "..", "adtomo.jl_amd", "julia",
+                           "ADTomoHIP.jl")).read()
+    body = re.search(r"\nstruct %s\n(.*?)\nend" % name, jl, re.S).group(1)
+    return [ln.split("::")[0].strip() for ln in body.splitlines() if "::" in ln]
+
+
+@pytest.mark.parametrize("c_name,py_name,jl_name", [
+    ("adt_info", "AdtInfo", "AdtInfo"), ("adt_event3d", "AdtEvent3d", "AdtEvent3d"),
+    ("adt_source3d", "AdtSource3d", "AdtSource3d"), ("adt_misfit3d", "AdtMisfit3d", "AdtMisfit3d"),
+    ("adt_tune", "AdtTune", None)])
+def test_struct_layouts_agree_across_bindings(c_name, py_name, jl_name):
+    # the C header, the ctypes twin and the Julia module declare the same
+    # fields in the same order
+    from adtomo_hip import _lib as L
+    c = _c_struct_fields(c_name)
+    py = [f[0] for f in getattr(L, py_name)._fields_]
+    assert py == c, (py, c)
+    if jl_name:
+        assert _julia_struct_fields(jl_name) == c

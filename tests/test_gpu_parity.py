@@ -453,3 +453,45 @@ def test_fp32_storage_fp64_solve_bit_exact():
     G.close()
     print(f"FP32 storage / FP64 solve vs FP64: max rel travel-time err {worst:.3e}", flush=True)
     assert worst < 1e-6
+
+
+# ------------------------------------------------------------ edge cases ---
+
+def test_iteration_cap_reports_not_converged():
+    # tol 0 never stops early: the reference runs its 20 iterations and
+    # prints nothing; the library reports converged = 0 with the same field
+    import adtomo_hip as A
+    shape = (9, 7, 6)
+    r = synth.rng(61)
+    f = r.uniform(0.3, 3.0, size=shape)
+    u0 = synth.point_source_u0(shape, (4, 3, 2))
+    u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, 0.0)
+    u = A.eikonal3d(dev(u0), dev(f), 0.5, *shape, 0.0, False)
+    assert it == 20 and A.ops.last_info["iterations"] == 20
+    assert A.ops.last_info["converged"] == 0
+    assert bits_equal(host(u), u_ref)
+    res = A.eikonal3d_batched([dev(u0)], [dev(f)], [0.5], [0.0])
+    assert res["info"][0]["iterations"] == 20 and res["info"][0]["converged"] == 0
+    assert bits_equal(host(res["u"][0]), u_ref)
+
+
+def test_no_source_field_is_all_z():
+    # u0 = 1000 everywhere: no node improves, every node is in Z (u == u0),
+    # so lambda = 0, grad_f = 0 and grad_u0 = grad_u
+    import adtomo_hip as A
+    shape = (8, 9, 10)
+    f = synth.rng(62).uniform(0.3, 3.0, size=shape)
+    u0 = np.full(shape, 1000.0)
+    g = synth.sparse_adjoint_source(shape, 12, seed=63, scale=1.0)
+    res = A.eikonal3d_batched([dev(u0)], [dev(f)], [0.5], [1e-3], grad_us=[dev(g)])
+    assert bits_equal(host(res["u"][0]), u0)
+    assert not host(res["grad_f"][0]).any()
+    assert bits_equal(host(res["grad_u0"][0]), g)
+    gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(g, u0, u0, f, 0.5)
+    assert bits_equal(host(res["grad_u0"][0]), gu0_ref) and not gf_ref.any()
+
+
+def test_empty_batch_is_a_no_op():
+    import adtomo_hip as A
+    res = A.eikonal3d_batched([], [], [], [])
+    assert res["u"] == [] and res["info"] == []

@@ -83,39 +83,64 @@ struct Walk {
 };
 
 constexpr uint32_t kCntShift = 13;
+// Nodes per thread and iteration of the prep passes (all their loads are
+// issued before any is used, so more requests are in flight per wave).
+constexpr int kPrepNodes = 2;
 constexpr uint32_t kZ = 0x40u;
 
 __device__ __forceinline__ uint32_t meta_shift(int64_t p) { return (p & 1) ? 16u : 0u; }
 
-// Upwind code of node p (Eikonal3D.cpp:105-171): kZ for Z, else the argmin
-// side of every active dimension.
+// Upwind code of a node (Eikonal3D.cpp:105-171) from its value up, its
+// initial value u0p and its neighbours nb[2d] (at -1) and nb[2d+1] (at +1)
+// along each dimension d: kZ for Z, else the argmin side of every active
+// dimension.  On a grid edge only the inner neighbour counts.
 template <typename T>
-__device__ __forceinline__ uint32_t code3(gptr<const T> u, gptr<const T> u0, int m, int n, int l,
-                                         int64_t p, const int (&co)[3]) {
-  const int64_t nl = (int64_t)n * l;
-  const int ex[3] = {m, n, l};
-  const int64_t st[3] = {nl, l, 1};
-  const T up = u[p];
-  if (up == u0[p]) return kZ;
+__device__ __forceinline__ uint32_t code3(T up, T u0p, const T (&nb)[6], const int (&co)[3],
+                                         const int (&ex)[3]) {
+  if (up == u0p) return kZ;
   uint32_t cd = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    const T vm = nb[2 * d], vp = nb[2 * d + 1];
     T umin;
     uint32_t side;
     if (co[d] == 0) {
-      umin = u[p + st[d]];
+      umin = vp;
       side = 2;
     } else if (co[d] == ex[d] - 1) {
-      umin = u[p - st[d]];
+      umin = vm;
       side = 1;
     } else {
-      const T vp = u[p + st[d]], vm = u[p - st[d]];
       umin = smin(vp, vm);
       side = (vp > vm) ? 1u : 2u;
     }
     if (up > umin) cd |= side << (2 * d);
   }
   return cd ? cd : kZ;
+}
+
+// kPrepNodes nodes p0 + k * blockDim.x of the prep passes' grid-stride
+// loop, with their coordinates; a node past N is replaced by p0 (its loads
+// stay in range, its results are not stored).
+struct PrepGroup {
+  int64_t q[kPrepNodes];
+  bool valid[kPrepNodes];
+  int co[kPrepNodes][3];
+};
+
+__device__ __forceinline__ PrepGroup prep_group(int64_t p0, int64_t N, Walk (&wk)[kPrepNodes]) {
+  PrepGroup G;
+#pragma unroll
+  for (int k = 0; k < kPrepNodes; ++k) {
+    const int64_t pk = p0 + (int64_t)k * blockDim.x;
+    G.valid[k] = pk < N;
+    G.q[k] = G.valid[k] ? pk : p0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) G.co[k][d] = G.valid[k] ? wk[k].co[d] : wk[0].co[d];
+  }
+#pragma unroll
+  for (int k = 0; k < kPrepNodes; ++k) wk[k].advance();
+  return G;
 }
 
 // Values node r reads to resolve itself, loaded for several nodes before
@@ -215,55 +240,101 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
     qtail = 0;
     nonz = 0;
   }
-  // pass 1: upwind codes; grad_u0 = g where u == u0 (Eikonal3D.cpp:105-110)
+  // pass 1: upwind codes; grad_u0 = g where u == u0 (Eikonal3D.cpp:105-110).
+  // Each thread takes kPrepNodes nodes per iteration and issues all their
+  // loads before using any, so that more requests are in flight per wave.
+  const int ex[3] = {m, n, l};
+  const int64_t st[3] = {nl, l, 1};
+  const int64_t stride = kPrepNodes * (int64_t)blockDim.x;
   {
-    Walk wk(threadIdx.x, blockDim.x, m, n, l);
-    for (int64_t p = threadIdx.x; p < N; p += blockDim.x, wk.advance()) {
-      if (gu0) gu0[p] = (u[p] == u0[p]) ? g[p] : T(0);
-      meta[p] = (uint16_t)code3<T>(u, u0, m, n, l, p, wk.co);
+    Walk wk[kPrepNodes] = {Walk(threadIdx.x, stride, m, n, l),
+                           Walk(threadIdx.x + blockDim.x, stride, m, n, l)};
+    for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
+      const PrepGroup G = prep_group(p0, N, wk);
+      T uc[kPrepNodes], u0c[kPrepNodes], gc[kPrepNodes], nb[kPrepNodes][6];
+#pragma unroll
+      for (int k = 0; k < kPrepNodes; ++k) {
+        const int64_t q = G.q[k];
+        uc[k] = u[q];
+        u0c[k] = u0[q];
+        gc[k] = g[q];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {  // clamped into the grid; edges use one side
+          nb[k][2 * d] = u[G.co[k][d] > 0 ? q - st[d] : q + st[d]];
+          nb[k][2 * d + 1] = u[G.co[k][d] < ex[d] - 1 ? q + st[d] : q - st[d]];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kPrepNodes; ++k) {
+        const uint32_t code = code3<T>(uc[k], u0c[k], nb[k], G.co[k], ex);
+        if (G.valid[k]) {
+          if (gu0) gu0[G.q[k]] = (uc[k] == u0c[k]) ? gc[k] : T(0);
+          meta[G.q[k]] = (uint16_t)code;
+        }
+      }
     }
   }
   __syncthreads();
   // pass 2: feeders, counters, coefficients; nodes without feeders start.
   // Other threads rewrite neighbouring meta entries meanwhile, but only
   // bits 7-15; the code bits read here never change.
-  const int ex[3] = {m, n, l};
-  const int64_t st[3] = {nl, l, 1};
   int my_nonz = 0;
-  Walk wk(threadIdx.x, blockDim.x, m, n, l);
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x, wk.advance()) {
-    const uint32_t cr = meta[p];
-    const int(&co)[3] = wk.co;
-    uint32_t fm = 0;
+  {
+    Walk wk[kPrepNodes] = {Walk(threadIdx.x, stride, m, n, l),
+                           Walk(threadIdx.x + blockDim.x, stride, m, n, l)};
+    for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
+      const PrepGroup G = prep_group(p0, N, wk);
+      uint32_t cr[kPrepNodes], nm[kPrepNodes][6];
+      T uc[kPrepNodes], gc[kPrepNodes], nu[kPrepNodes][6];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (co[d] > 0) {
-        const uint32_t cc = meta[p - st[d]];
-        fm |= (!(cc & kZ) && ((cc >> (2 * d)) & 3u) == 2u) ? (1u << (2 * d)) : 0u;
+      for (int k = 0; k < kPrepNodes; ++k) {
+        const int64_t q = G.q[k];
+        cr[k] = meta[q];
+        uc[k] = u[q];
+        gc[k] = g[q];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {  // a missing neighbour reads the node itself
+          const int64_t lo = G.co[k][d] > 0 ? q - st[d] : q;
+          const int64_t hi = G.co[k][d] < ex[d] - 1 ? q + st[d] : q;
+          nm[k][2 * d] = meta[lo];
+          nm[k][2 * d + 1] = meta[hi];
+          nu[k][2 * d] = u[lo];
+          nu[k][2 * d + 1] = u[hi];
+        }
       }
-      if (co[d] < ex[d] - 1) {
-        const uint32_t cc = meta[p + st[d]];
-        fm |= (!(cc & kZ) && ((cc >> (2 * d)) & 3u) == 1u) ? (2u << (2 * d)) : 0u;
+#pragma unroll
+      for (int k = 0; k < kPrepNodes; ++k) {
+        uint32_t fm = 0;  // neighbours whose upwind neighbour this node is
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const uint32_t cm = nm[k][2 * d], cp = nm[k][2 * d + 1];
+          if (G.co[k][d] > 0)
+            fm |= (!(cm & kZ) && ((cm >> (2 * d)) & 3u) == 2u) ? (1u << (2 * d)) : 0u;
+          if (G.co[k][d] < ex[d] - 1)
+            fm |= (!(cp & kZ) && ((cp >> (2 * d)) & 3u) == 1u) ? (2u << (2 * d)) : 0u;
+        }
+        Rec<T> rp;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const uint32_t sd = (cr[k] >> (2 * d)) & 3u;
+          // 2 (u_r - u_argmin): Eikonal3D.cpp:149-166
+          rp.w[d] = sd ? T(2.0) * (uc[k] - (sd == 1 ? nu[k][2 * d] : nu[k][2 * d + 1])) : T(0);
+        }
+        const bool z = cr[k] & kZ;
+        rp.lam = z ? T(0) : gc[k];
+        if (G.valid[k]) {
+          const int64_t p = G.q[k];
+          rec[p].lam = rp.lam;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) rec[p].w[d] = rp.w[d];
+          const uint32_t cnt = z ? 7u : (uint32_t)__builtin_popcount(fm);
+          meta[p] = (uint16_t)((cr[k] & 0x7fu) | (fm << 7) | (cnt << kCntShift));
+          if (!z) {
+            ++my_nonz;
+            if (cnt == 0) queue[atomicAdd(&qtail, 1)] = (int32_t)p;
+          }
+        }
       }
-    }
-    Rec<T> rp;
-    const T up = u[p];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const uint32_t sd = (cr >> (2 * d)) & 3u;
-      // 2 (u_r - u_argmin): Eikonal3D.cpp:149-166
-      rp.w[d] = sd ? T(2.0) * (up - u[sd == 1 ? p - st[d] : p + st[d]]) : T(0);
-    }
-    const bool z = cr & kZ;
-    rp.lam = z ? T(0) : g[p];
-    rec[p].lam = rp.lam;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) rec[p].w[d] = rp.w[d];
-    const uint32_t cnt = z ? 7u : (uint32_t)__builtin_popcount(fm);
-    meta[p] = (uint16_t)((cr & 0x7fu) | (fm << 7) | (cnt << kCntShift));
-    if (!z) {
-      ++my_nonz;
-      if (cnt == 0) queue[atomicAdd(&qtail, 1)] = (int32_t)p;
     }
   }
   atomicAdd(&nonz, my_nonz);

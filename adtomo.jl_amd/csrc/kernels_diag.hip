@@ -105,6 +105,8 @@ struct DSweep {
   double hd;    // h for the FP64 local solve of the FP32-storage variant
   rsrc_t u;     // travel times (read old / write new in place)
   rsrc_t f;
+  rsrc_t snap_src;  // last sweep of an iteration: snapshot at its start
+  rsrc_t snap_dst;  // ... and the snapshot for the next iteration
 };
 
 // cnt values of column `col` from canonical c0 on (element j <- c0 + j),
@@ -284,6 +286,67 @@ __device__ __forceinline__ void seg_store(lptr<const T> pl, rsrc_t r, int seg, i
   }
 }
 
+// seg_store for the last sweep of an iteration: also stores the values to
+// the next snapshot and folds max |new - snapshot| into `key` (the stop
+// rule's err, Eikonal3D.cpp:79-81), `snap` holding the snapshot's values at
+// the same offsets (loaded with seg_issue one step earlier).  The stores are
+// issued before the snapshot values are waited for.
+template <typename T, bool FULL>
+__device__ __forceinline__ void seg_store_err(lptr<const T> pl, rsrc_t r, rsrc_t sdst, int seg,
+                                              int cnt, bool fwd, int p, const Flight<T>& snap,
+                                              uint64_t& key) {
+  using P = Pieces<T>;
+  if (FULL) cnt = kLW;
+  T x[P::LPC][P::EP];
+  bool ok[P::LPC];
+#pragma unroll
+  for (int i = 0; i < P::LPC; ++i) {
+    const int q = i * P::CPI + p / P::LPC;
+    const int m0 = (p % P::LPC) * P::EP;
+    const int sq = __shfl(seg, q);
+    ok[i] = sq >= 0 && (FULL || m0 + P::EP <= cnt);
+#pragma unroll
+    for (int e = 0; e < P::EP; ++e) {
+      const int m = m0 + e;
+      const int j = (FULL || m < cnt) ? canon<FULL>(m, cnt, fwd) : kLW;
+      x[i][e] = pl[j * P::RS + q + 1];
+    }
+    const uint32_t off = ok[i] ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB;
+    const u32x4 v = piece_make<T>(x[i]);
+    bstore4(r, off, v);
+    bstore4(sdst, off, v);
+  }
+  T xt[P::EP - 1];
+  if (!FULL) {
+    const int nfull = (cnt / P::EP) * P::EP;
+#pragma unroll
+    for (int t = 0; t < P::EP - 1; ++t) {
+      const int m = nfull + t;
+      const int j = m < cnt ? canon<FULL>(m, cnt, fwd) : kLW;
+      xt[t] = pl[j * P::RS + p + 1];
+      const uint32_t off = boff<T>(seg >= 0 && m < cnt, (int64_t)seg + m);
+      bstore(r, off, xt[t]);
+      bstore(sdst, off, xt[t]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < P::LPC; ++i) {
+#pragma unroll
+    for (int e = 0; e < P::EP; ++e) {
+      const uint64_t k = maxkey(Real<T>::fabs_(x[i][e] - piece_get<T>(snap.pc[i], e)));
+      key = (ok[i] && k > key) ? k : key;
+    }
+  }
+  if (!FULL) {
+    const int nfull = (cnt / P::EP) * P::EP;
+#pragma unroll
+    for (int t = 0; t < P::EP - 1; ++t) {
+      const uint64_t k = maxkey(Real<T>::fabs_(xt[t] - snap.tail[t]));
+      key = (seg >= 0 && nfull + t < cnt && k > key) ? k : key;
+    }
+  }
+}
+
 // Publishes canonical node 0 of every lane's segment from a transfer in
 // flight (once it has arrived) to dst[lane]: the old value the wave
 // upstream along k needs one past its own segment.
@@ -353,9 +416,9 @@ __device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> p
 // values are stored at the start of the next step, before any load is
 // waited for.  So the compiler's wait counts stay exact on every path into
 // the loop body: landing a load never waits for a store issued after it.
-template <typename T, bool FULL, bool MIX>
-__device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
-                                                lptr<T> xch, lptr<T> xold, lptr<T> planes) {
+template <typename T, bool FULL, bool MIX, bool ERR>
+__device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
+                                                    lptr<T> xch, lptr<T> xold, lptr<T> planes) {
   using P = Pieces<T>;
   const DGeom& g = S.g;
   const int p = threadIdx.x & 63;
@@ -393,6 +456,8 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
   T OWN[kLW], LE[kLW];  // LE: band-edge values (general variant)
   u32x4 EF;             // band-edge pieces (full-length variant)
   Flight<T> FD, FF;
+  Flight<T> FS;         // ERR: snapshot values of the segment stored next step
+  uint64_t key = 0;     // ERR: max |new - snapshot| key of this lane
   // band-edge segment of lane 0 / lane H-1 for column `col` (full variant)
   auto eseg_of = [&](const DCol& col) -> int {
     const bool eu = lane_first && col.valid && col.a > 0;
@@ -419,6 +484,7 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
   seg_issue<T, FULL>(FD, S.u, seg_of(cv1), cnt, p);
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
   issue_edge(cv);
+  if constexpr (ERR) seg_issue<T, FULL>(FS, S.snap_src, -1, cnt, p);
   seg_publish<T, FULL>(FD, xold + kDW * 64 + w * 64, cnt, fwd, p);
   __syncthreads();
 
@@ -428,7 +494,13 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
     const int v = v0 + t;
     const bool active = v >= 0 && v < g.V;
     // new values of v-1 (plane n) to memory
-    seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
+    if constexpr (ERR) {
+      seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, cnt, fwd, p, FS, key);
+      // snapshot values of diagonal v, stored at the next step
+      seg_issue<T, FULL>(FS, S.snap_src, active ? seg_of(cv) : -1, cnt, p);
+    } else {
+      seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
+    }
     // rotate by one diagonal: OWN <- old(v) from plane x, which then
     // receives old(v+1); plane f receives f(v)
 #pragma unroll
@@ -519,25 +591,30 @@ __device__ __forceinline__ void diag_sweep_wave(const DSweep<T>& S, int w, int c
     cv2 = dcol(g, v + 3, p, S.base0, S.tA, S.tB);
     __syncthreads();
   }
-  seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
+  if constexpr (ERR)
+    seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, cnt, fwd, p, FS, key);
+  else
+    seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
   __syncthreads();  // the next sweep reads these values
+  return key;
 }
 
 // One sweep of one event by the whole workgroup.  Waves whose segment has
 // kLW nodes take the fixed-length variant; the others (and idle waves) the
 // general one.  Both run the same number of steps and barriers.
-template <typename T, bool MIX>
-__device__ __forceinline__ void diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> xold,
-                                           lptr<T> planes) {
+// ERR (last sweep of an iteration): returns this lane's max |u - snapshot|
+// key and writes the next snapshot.
+template <typename T, bool MIX, bool ERR>
+__device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> xold,
+                                               lptr<T> planes) {
   // the wave index is uniform across the wave: keep it (and everything
   // derived from it) in scalar registers
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
   if (cnt == kLW)
-    diag_sweep_wave<T, true, MIX>(S, w, cnt, xch, xold, planes);
-  else
-    diag_sweep_wave<T, false, MIX>(S, w, cnt, xch, xold, planes);
+    return diag_sweep_wave<T, true, MIX, ERR>(S, w, cnt, xch, xold, planes);
+  return diag_sweep_wave<T, false, MIX, ERR>(S, w, cnt, xch, xold, planes);
 }
 
 template <typename T, bool MIX>
@@ -564,10 +641,12 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   S.hd = ev.h;
   S.u = make_rsrc(ev.u, bytes);
   S.f = make_rsrc(ev.f, bytes);
+  S.snap_dst = make_rsrc(ev.uold, bytes);
   const T tol = (T)ev.tol;
   int it = 0;
   T err = T(0);
   for (it = 0; it < max_iter; ++it) {
+    uint64_t key = 0;
     for (int sw = 0; sw < 8; ++sw) {
       const int dir = kDir3[sw];
       const bool fi = dir & 1, fj = dir & 2, fk = dir & 4;
@@ -579,20 +658,14 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
       S.tC = S.flipC ? -1 : 1;
       S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
-      diag_sweep<T, MIX>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
-    }
-    // err = max |u - u_old| over the event (coalesced pass after the 8
-    // sweeps), which also leaves u_old = u for the next iteration
-    uint64_t key = 0;
-    {
-      const gptr<const T> ua = (gptr<const T>)ev.u;
-      const gptr<const T> sr = (gptr<const T>)((it == 0) ? ev.u0 : ev.uold);
-      const gptr<T> so = (gptr<T>)ev.uold;
-      for (int64_t q = threadIdx.x; q < N; q += blockDim.x) {
-        const T uq = ua[q];
-        const uint64_t kq = maxkey(Real<T>::fabs_(uq - sr[q]));
-        key = kq > key ? kq : key;
-        so[q] = uq;
+      if (sw < 7) {
+        diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
+      } else {
+        // err = max |u - u_old| (Eikonal3D.cpp:79-81), taken as the last
+        // sweep stores each node, which also writes u_old = u for the next
+        // iteration (u_old of the first iteration is u0 itself)
+        S.snap_src = make_rsrc(it == 0 ? ev.u0 : ev.uold, bytes);
+        key = diag_sweep<T, MIX, true>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
       }
     }
     for (int off = 32; off > 0; off >>= 1) {

@@ -103,8 +103,11 @@ struct DSweep {
   bool flipA, flipB, flipC;
   T h;
   double hd;    // h for the FP64 local solve of the FP32-storage variant
-  rsrc_t u;     // travel times (read old / write new in place)
+  rsrc_t u;     // travel times: new values written (and read back) here
+  rsrc_t src;   // old values: u, or u0 in the very first sweep (u = u0,
+                // Eikonal3D.cpp:72, folded into that sweep)
   rsrc_t f;
+  rsrc_t u0;
   rsrc_t snap_src;  // last sweep of an iteration: snapshot at its start
   rsrc_t snap_dst;  // ... and the snapshot for the next iteration
 };
@@ -383,14 +386,19 @@ __device__ __forceinline__ void seg_publish(const Flight<T>& fl, lptr<T> dst, in
 // lane H-1's (a+1, b) column, both kLW nodes, moved as 2 x LPC pieces by
 // lanes 0 .. 2*LPC-1 of one transfer.  `eseg` = memory index of the lowest
 // node of the lane's band-edge segment (lanes 0 and H-1; -1: none).
+// Lane 0's segment holds new values (read from r_new), lane H-1's old
+// values (read from r_old).
 template <typename T>
-__device__ __forceinline__ u32x4 edge_issue(rsrc_t r, int eseg, int hm1, int p) {
+__device__ __forceinline__ u32x4 edge_issue(rsrc_t r_new, rsrc_t r_old, int eseg, int hm1, int p) {
   using P = Pieces<T>;
   const int src = p < P::LPC ? 0 : hm1;
   const int sq = __shfl(eseg, src);
   const int m0 = (p % P::LPC) * P::EP;
   const bool ok = p < 2 * P::LPC && sq >= 0;
-  return bload4(r, ok ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB);
+  const uint32_t off = ok ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB;
+  const u32x4 a = bload4(r_new, p < P::LPC ? off : kOOB);
+  const u32x4 b = bload4(r_old, p < P::LPC ? kOOB : off);
+  return p < P::LPC ? a : b;
 }
 
 // Writes the band-edge pieces: lane 0's segment to plane n position 0,
@@ -445,7 +453,11 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   auto load_edge = [&](T(&buf)[kLW], const DCol& col) {
     const bool eu = lane_first && col.valid && col.a > 0;
     const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
-    load_seg(buf, S.u, col, eu ? -S.tA : S.tA, eu || ed, c0, cnt, L, S.tC);
+    T bo[kLW];
+    load_seg(buf, S.u, col, -S.tA, eu, c0, cnt, L, S.tC);
+    load_seg(bo, S.src, col, S.tA, ed, c0, cnt, L, S.tC);
+#pragma unroll
+    for (int j = 0; j < kLW; ++j) buf[j] = eu ? buf[j] : bo[j];
   };
 
   // Registers: OWN = old values of diagonal v in, new values out.  FD, FF,
@@ -468,7 +480,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   };
   auto issue_edge = [&](const DCol& col) {
     if constexpr (FULL)
-      EF = edge_issue<T>(S.u, eseg_of(col), g.H - 1, p);
+      EF = edge_issue<T>(S.u, S.src, eseg_of(col), g.H - 1, p);
     else
       load_edge(LE, col);
   };
@@ -479,9 +491,9 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   DCol cv2 = dcol(g, v0 + 2, p, S.base0, S.tA, S.tB);
   // prologue: plane x = old(v0); in flight: old(v0+1), f(v0), edges(v0);
   // node 0 of old(v0+1) published for the wave upstream (its step 0)
-  seg_issue<T, FULL>(FD, S.u, seg_of(cv), cnt, p);
+  seg_issue<T, FULL>(FD, S.src, seg_of(cv), cnt, p);
   seg_land<T, FULL>(FD, px, cnt, fwd, p);
-  seg_issue<T, FULL>(FD, S.u, seg_of(cv1), cnt, p);
+  seg_issue<T, FULL>(FD, S.src, seg_of(cv1), cnt, p);
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
   issue_edge(cv);
   if constexpr (ERR) seg_issue<T, FULL>(FS, S.snap_src, -1, cnt, p);
@@ -550,7 +562,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       const bool yf = g.a_is_i ? bf : af, yl = g.a_is_i ? bl : al;
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
-        if (e == 1) seg_issue<T, FULL>(FD, S.u, seg_of(cv2), cnt, p);
+        if (e == 1) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
         if (e == 3) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
         if (e == 4) issue_edge(cv1);
         if (FULL || e < cnt) {
@@ -629,17 +641,12 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   if (l > max_l || l > kDW * kLW) return;  // handled by the k-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
-  {  // u = u0 (Eikonal3D.cpp:72)
-    const gptr<const T> s = (gptr<const T>)ev.u0;
-    const gptr<T> d = (gptr<T>)ev.u;
-    for (int64_t q = threadIdx.x; q < N; q += blockDim.x) d[q] = s[q];
-  }
-  __syncthreads();
   DSweep<T> S;
   S.g = make_dgeom(m, n, l);
   S.h = (T)ev.h;
   S.hd = ev.h;
   S.u = make_rsrc(ev.u, bytes);
+  S.u0 = make_rsrc(ev.u0, bytes);
   S.f = make_rsrc(ev.f, bytes);
   S.snap_dst = make_rsrc(ev.uold, bytes);
   const T tol = (T)ev.tol;
@@ -658,6 +665,9 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
       S.tC = S.flipC ? -1 : 1;
       S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                 (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
+      // u = u0 (Eikonal3D.cpp:72): the first sweep reads its old values
+      // from u0; every node is written once per sweep
+      S.src = (it == 0 && sw == 0) ? S.u0 : S.u;
       if (sw < 7) {
         diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
       } else {

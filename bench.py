@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the forward kernel on this workload (256 events),
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy kernel of
 # known size: profiles/pmc_traffic_fwd3_diag_adj3_kahn_256x128x128x64.md
-FWD_TRAFFIC_BYTES_256 = 112.5e9 + 60.5e9
+FWD_TRAFFIC_BYTES_256 = 112.0e9 + 58.3e9
 
 
 def log(*a):

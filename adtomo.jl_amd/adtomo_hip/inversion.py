@@ -17,6 +17,8 @@ each chunk's forward, misfit and adjoint run back to back over one set of
 slot buffers, and the per-station dL/df are summed into one accumulator per
 phase, so device memory holds ``chunk`` x phases travel-time fields (plus
 their adjoint workspace), not one per station.
+By default ``chunk`` is every station when their slots fit in 80% of the
+free device memory, else the largest count that fits.
 Without chunking (``chunk`` = number of stations) the initial times u0_s,
 which depend on the fixed starting model only (inversion.jl:48-60), are
 built once; with chunking they are rebuilt on the device per chunk.
@@ -94,7 +96,9 @@ class TravelTimeInversion:
         # solve slots: `chunk` stations per batched launch, for every phase
         # at once (slots [q*C, (q+1)*C) belong to phase q), so that one
         # launch carries all phases' solves
-        self.chunk = ns if chunk is None else max(1, min(int(chunk), ns))
+        if chunk is None:
+            chunk = self.fitting_chunk(ns, len(self.phases), self.N, self.x.element_size())
+        self.chunk = max(1, min(int(chunk), ns))
         C = self.chunk
         P = len(self.phases)
         self.u0s = [torch.empty_like(self.x) for _ in range(P * C)]
@@ -110,6 +114,20 @@ class TravelTimeInversion:
                             out=self.u0s[q * C:(q + 1) * C])
         self.nevals = 0
         self.last_loss = {}
+
+    @staticmethod
+    def slot_bytes(N, elem):
+        """Device bytes of one solve slot: u0, u, dL/du, dL/du0 and dL/df
+        (N elements each) plus the batched workspace (forward snapshot,
+        adjoint records of 4 elements, 16-bit node states, int32 queue)."""
+        return N * (5 * elem + elem + 4 * elem + 2 + 4)
+
+    def fitting_chunk(self, ns, nphases, N, elem, budget=0.8):
+        """All stations if their slots fit in ``budget`` of the free device
+        memory, else the largest number of stations per launch that does."""
+        free, _ = self.torch.cuda.mem_get_info()
+        per_station = nphases * self.slot_bytes(N, elem)
+        return max(1, min(ns, int(budget * free // per_station)))
 
     def _u0_scale(self, phase):
         return 1.0 if phase == "P" else self.vpvs0

@@ -19,8 +19,8 @@ Other workloads (``--workload``), each printing its own JSON line:
           surface stations x {P, S} = 200 solves, 500 events, Huber residual
           (delta 0.1 s), picks with N(0, 0.05 s) noise; plus one L-BFGS step
   large   400x400x160 FP32 storage with FP64 local solves (h = 0.25 km):
-          150 stations in chunks, 2000 events; against FP64 (and pure FP32)
-          on a 16-station subset
+          150 stations (as many per launch as fit in memory, or --chunk),
+          2000 events; against FP64 (and pure FP32) on a 16-station subset
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nev 256]
                        [--workload headline|uneven|joint|large]
@@ -60,7 +60,8 @@ def main():
                     help="keep the default launch setting instead of timing the candidates")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "uneven", "joint", "large"])
-    ap.add_argument("--chunk", type=int, default=0, help="stations per batched launch (0: all)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="stations per batched launch (0: as many as fit in device memory)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -375,6 +376,7 @@ def run_joint(args):
     inv = TravelTimeInversion(vel0, h, stations, events, {"P": tp, "S": ts}, {"P": w, "S": w},
                               smooth=(3, 3), lam=0.1, phases=("P", "S"), vpvs=1.7,
                               huber_delta=0.1, chunk=args.chunk or None)
+    chunk = inv.chunk
     p0 = inv.initial_params()
     L0 = None
     for _ in range(args.warmup):
@@ -398,7 +400,7 @@ def run_joint(args):
           200 / ms * 1e3, "solves/s", ms, args, "f64",
           {"workload": "joint Vp/Vs gradient, 200x200x80, h=0.5 km, 100 stations x {P,S}, "
                        "500 events, Huber delta 0.1 s, noise N(0, 0.05 s)",
-           "solves": 200, "events": 500, "chunk": args.chunk or 100},
+           "solves": 200, "events": 500, "chunk": chunk},
           lbfgs_step={"evaluations": len(hist), "seconds": t, "loss_before": hist[0],
                       "loss_after": float(res.fun), "vpvs_after": float(res.x[-1])})
 
@@ -410,7 +412,6 @@ def run_large(args):
     from adtomo_hip import synth
     from adtomo_hip.inversion import TravelTimeInversion
     shape, h = (400, 400, 160), 0.25
-    chunk = args.chunk or 50
     t0 = time.time()
     vtrue = synth.layered_grf_model(shape, h, 3.5, 7.5, 0.03, 5.0, seed=51)
     vel0 = synth.layered_model(shape, 3.5, 7.5)
@@ -419,12 +420,14 @@ def run_large(args):
     stations = [(float(r.uniform(1, shape[0])), float(r.uniform(1, shape[1])), 1.0)
                 for _ in range(ns)]
     events = tuple(r.uniform(1, s, ne) for s in shape)
-    tobs = _picks(vtrue, h, stations, events, chunk) + r.normal(0, 0.05, (ns, ne))
+    tobs = _picks(vtrue, h, stations, events, 50) + r.normal(0, 0.05, (ns, ne))
     del vtrue
     w = np.ones((ns, ne))
     log(f"large workload built in {time.time() - t0:.1f}s")
+    # chunk: --chunk, else as many stations per launch as fit in device memory
     inv = TravelTimeInversion(vel0, h, stations, events, tobs, w, smooth=(3, 3), lam=0.1,
-                              dtype="float32", chunk=chunk, compute_f64=True)
+                              dtype="float32", chunk=args.chunk or None, compute_f64=True)
+    chunk = inv.chunk
     p0 = inv.initial_params()
     for _ in range(args.warmup):
         inv.objective(p0)

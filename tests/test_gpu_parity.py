@@ -526,6 +526,13 @@ def test_config2_gradient_central_differences_batched():
     out = A.eikonal3d_batched([u0d] * len(fs), fs, [1.0] * len(fs), [0.0] * len(fs))
     gd = dev(g)
     loss = [float((u * gd).sum()) for u in out["u"]]
+    # rounding of the loss differences: the loss sums terms of size up to
+    # S = sum |g u|, so a difference quotient carries ~ eps S / step of noise
+    S = float(np.abs(g * host(res["u"][0])).sum())
+    worst = 0.0
     for k, p in enumerate(picks):
         fd = (loss[2 * k] - loss[2 * k + 1]) / (2 * dl[k])
-        assert abs(fd - gf[p]) <= 1e-5 * abs(gf[p]) + 1e-9, (p, fd, gf[p])
+        noise = 8 * np.finfo(np.float64).eps * S / dl[k]
+        assert abs(fd - gf[p]) <= 1e-5 * abs(gf[p]) + noise, (p, fd, gf[p], noise)
+        worst = max(worst, abs(fd - gf[p]) / abs(gf[p]))
+    print(f"config2 central differences: worst rel diff {worst:.2e} over 20 nodes", flush=True)

@@ -67,7 +67,7 @@ struct DevBuf {
 // [1] FP32), read at each launch / graph creation and written by
 // adt_set_tune / adt_autotune under a mutex.
 std::mutex g_tune_mu;
-adt_tune g_tune[2] = {{adt::kDiagMaxL, 1024, 1, 0.f, 0.f}, {adt::kDiagMaxL, 1024, 1, 0.f, 0.f}};
+adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, 0.f, 0.f}, {adt::kDiagAll, 1024, 1, 0.f, 0.f}};
 
 template <typename T>
 constexpr int tune_index() {
@@ -79,7 +79,7 @@ adt_tune current_tune() {
   return g_tune[tune_index<T>()];
 }
 bool tune_valid(const adt_tune& t) {
-  return t.diag_max_l >= 0 && t.diag_max_l <= adt::kDiagMaxL &&
+  return t.diag_max_l >= 0 && t.diag_max_l <= adt::kDiagAll &&
          (t.adj_wg == 256 || t.adj_wg == 512 || t.adj_wg == 1024) && (t.order == 0 || t.order == 1);
 }
 
@@ -639,8 +639,12 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
   const adt_tune cur = current_tune<T>();
   int min_l = 1 << 30;
   for (int e = 0; e < nev; ++e) min_l = std::min<int>(min_l, (int)ev[e].l);
-  std::vector<int> dmls = {adt::kDiagMaxL};
-  if (min_l <= adt::kDiagMaxL) dmls.push_back(0);  // k-streaming for every event
+  int max_l_ev = 0;
+  for (int e = 0; e < nev; ++e) max_l_ev = std::max<int>(max_l_ev, (int)ev[e].l);
+  // diagonal kernel for every event, k-streaming for every event, and
+  // (mixed batches) diagonal up to one k chunk of 64, k-streaming above
+  std::vector<int> dmls = {adt::kDiagAll, 0};
+  if (min_l <= 64 && max_l_ev > 64) dmls.push_back(64);
   const int wgs[3] = {256, 512, 1024};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ADT_HIP(hipEventCreate(&e0));

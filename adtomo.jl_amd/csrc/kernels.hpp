@@ -87,8 +87,9 @@ __global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter, int min_l
 // coefficients).
 template <typename T>
 constexpr size_t kAdjRecBytes = 4 * sizeof(T);
-// Largest k extent handled by fwd3_diag_kernel (8 waves x 8 nodes).
-constexpr int kDiagMaxL = 64;
+// Launch setting diag_max_l meaning "every k extent on fwd3_diag_kernel"
+// (it sweeps the k axis in chunks of 64 nodes).
+constexpr int kDiagAll = 1 << 30;
 template <typename T, bool MIX>
 __global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
 template <typename T>

@@ -103,6 +103,8 @@ struct DSweep {
   bool flipA, flipB, flipC;
   T h;
   double hd;    // h for the FP64 local solve of the FP32-storage variant
+  int cs;       // canonical c of this pass's first node (k chunks of <= 64)
+  int Lfull;    // k extent of the event (g.L is the chunk's)
   rsrc_t u;     // travel times: new values written (and read back) here
   rsrc_t src;   // old values: u, or u0 in the very first sweep (u = u0,
                 // Eikonal3D.cpp:72, folded into that sweep)
@@ -424,7 +426,7 @@ __device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> p
 // values are stored at the start of the next step, before any load is
 // waited for.  So the compiler's wait counts stay exact on every path into
 // the loop body: landing a load never waits for a store issued after it.
-template <typename T, bool FULL, bool MIX, bool ERR>
+template <typename T, bool FULL, bool MIX, bool ERR, bool CH>
 __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
                                                     lptr<T> xch, lptr<T> xold, lptr<T> planes) {
   using P = Pieces<T>;
@@ -470,6 +472,21 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   Flight<T> FD, FF;
   Flight<T> FS;         // ERR: snapshot values of the segment stored next step
   uint64_t key = 0;     // ERR: max |new - snapshot| key of this lane
+  // CH (k in several chunks): wave 0's c-1 neighbour (new, end of the
+  // previous chunk) and the last segment's c+1 neighbour (old, start of the
+  // next chunk) come from memory, one value per lane and step, loaded one
+  // step ahead.  Without chunks both are grid edges.
+  const bool need_cm = CH && w == 0 && S.cs > 0;
+  const bool need_cp = CH && !has_x && cnt > 0 && S.cs + L < S.Lfull;
+  auto load_cm = [&](const DCol& col) -> T {
+    if constexpr (!CH) return T(0);
+    return bload(S.u, boff<T>(need_cm && col.valid, col.node0 - S.tC), T(0));
+  };
+  auto load_cp = [&](const DCol& col) -> T {
+    if constexpr (!CH) return T(0);
+    return bload(S.src, boff<T>(need_cp && col.valid, col.node0 + (int64_t)L * S.tC), T(0));
+  };
+  T XCM, XCP;  // values for the current diagonal, landed at step start
   // band-edge segment of lane 0 / lane H-1 for column `col` (full variant)
   auto eseg_of = [&](const DCol& col) -> int {
     const bool eu = lane_first && col.valid && col.a > 0;
@@ -496,6 +513,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   seg_issue<T, FULL>(FD, S.src, seg_of(cv1), cnt, p);
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
   issue_edge(cv);
+  T XCM_n = load_cm(cv), XCP_n = load_cp(cv);
   if constexpr (ERR) seg_issue<T, FULL>(FS, S.snap_src, -1, cnt, p);
   seg_publish<T, FULL>(FD, xold + kDW * 64 + w * 64, cnt, fwd, p);
   __syncthreads();
@@ -520,6 +538,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       if (FULL || j < cnt) OWN[j] = px[j * P::RS + p + 1];
     seg_land<T, FULL>(FD, px, cnt, fwd, p);
     seg_land<T, FULL>(FF, pf, cnt, fwd, p);
+    XCM = XCM_n;
+    XCP = XCP_n;
     // band-edge neighbours of v at lanes -1 (plane n) and H (plane x)
     if constexpr (FULL) {
       edge_land<T>(EF, pn, px, g.H, fwd, p);
@@ -542,8 +562,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     // nodes, every step, so the memory pipeline works alongside the math.
     T xo = T(0);
     {
-      const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : T(0);
-      const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : T(0);
+      const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : XCM;
+      const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : XCP;
       const int ra = S.flipA ? g.Ma - 1 - cv.a : cv.a;
       const int rb = S.flipB ? g.Nb - 1 - cv.b : cv.b;
       const bool af = ra == 0, al = ra == g.Ma - 1;
@@ -565,6 +585,10 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
         if (e == 1) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
         if (e == 3) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
         if (e == 4) issue_edge(cv1);
+        if (e == 5) {
+          XCM_n = load_cm(cv1);
+          XCP_n = load_cp(cv1);
+        }
         if (FULL || e < cnt) {
           const int c = c0 + e;
           const int rw = e * P::RS;
@@ -575,10 +599,11 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
           const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
           const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
           if (cv.valid) {
-            const int rc = S.flipC ? L - 1 - c : c;
+            const int cg = S.cs + c;  // canonical c in the whole event
+            const int rc = S.flipC ? S.Lfull - 1 - cg : cg;
             const T ux = xf ? xp : (xl ? xm : smin(xp, xm));
             const T uy = yf ? yp : (yl ? ym : smin(yp, ym));
-            const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == L - 1);
+            const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == S.Lfull - 1);
             T cand;
             if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
               cand = (T)solution3d<double>(ux, uy, mc, fe, S.hd);
@@ -624,9 +649,12 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
-  if (cnt == kLW)
-    return diag_sweep_wave<T, true, MIX, ERR>(S, w, cnt, xch, xold, planes);
-  return diag_sweep_wave<T, false, MIX, ERR>(S, w, cnt, xch, xold, planes);
+  if (S.Lfull > kDW * kLW) {
+    if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, true>(S, w, cnt, xch, xold, planes);
+    return diag_sweep_wave<T, false, MIX, ERR, true>(S, w, cnt, xch, xold, planes);
+  }
+  if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, false>(S, w, cnt, xch, xold, planes);
+  return diag_sweep_wave<T, false, MIX, ERR, false>(S, w, cnt, xch, xold, planes);
 }
 
 template <typename T, bool MIX>
@@ -638,11 +666,17 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   __shared__ uint64_t red[kDW];
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
-  if (l > max_l || l > kDW * kLW) return;  // handled by the k-streaming kernel
+  if (l > max_l) return;  // handled by the k-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
   DSweep<T> S;
   S.g = make_dgeom(m, n, l);
+  S.Lfull = l;
+  // the k axis is swept in chunks of kDW * kLW nodes (the last one shorter),
+  // in sweep order: a chunk's nodes read the previous chunk's as new values
+  // and the next chunk's as old ones, as the lexicographic sweep does
+  constexpr int kChunk = kDW * kLW;
+  const int nchunk = (l + kChunk - 1) / kChunk;
   S.h = (T)ev.h;
   S.hd = ev.h;
   S.u = make_rsrc(ev.u, bytes);
@@ -663,19 +697,28 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
       S.tA = S.flipA ? -S.g.sA : S.g.sA;
       S.tB = S.flipB ? -S.g.sB : S.g.sB;
       S.tC = S.flipC ? -1 : 1;
-      S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
-                (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) + (S.flipC ? (int64_t)(l - 1) : 0);
       // u = u0 (Eikonal3D.cpp:72): the first sweep reads its old values
       // from u0; every node is written once per sweep
       S.src = (it == 0 && sw == 0) ? S.u0 : S.u;
-      if (sw < 7) {
-        diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
-      } else {
-        // err = max |u - u_old| (Eikonal3D.cpp:79-81), taken as the last
-        // sweep stores each node, which also writes u_old = u for the next
-        // iteration (u_old of the first iteration is u0 itself)
-        S.snap_src = make_rsrc(it == 0 ? ev.u0 : ev.uold, bytes);
-        key = diag_sweep<T, MIX, true>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
+      if (sw == 7) S.snap_src = make_rsrc(it == 0 ? ev.u0 : ev.uold, bytes);
+      for (int ch = 0; ch < nchunk; ++ch) {
+        S.cs = ch * kChunk;
+        const int Lc = l - S.cs < kChunk ? l - S.cs : kChunk;
+        S.g.L = Lc;
+        S.g.Lw = (Lc + kDW - 1) / kDW;
+        S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
+                  (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) +
+                  (S.flipC ? (int64_t)(l - 1 - S.cs) : (int64_t)S.cs);
+        if (sw < 7) {
+          diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
+        } else {
+          // err = max |u - u_old| (Eikonal3D.cpp:79-81), taken as the last
+          // sweep stores each node, which also writes u_old = u for the
+          // next iteration (u_old of the first iteration is u0 itself)
+          const uint64_t kc =
+              diag_sweep<T, MIX, true>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
+          key = kc > key ? kc : key;
+        }
       }
     }
     for (int off = 32; off > 0; off >>= 1) {

@@ -158,8 +158,9 @@ int adt_graph_destroy(adt_graph *g);
  * every setting gives the same travel times (bit for bit) and gradients
  * (to rounding). */
 typedef struct adt_tune {
-  int32_t diag_max_l; /* forward: events with l <= diag_max_l (0..64) use the
-                         diagonal-streaming kernel, the others k-streaming */
+  int32_t diag_max_l; /* forward: events with l <= diag_max_l use the
+                         diagonal-streaming kernel (any l, in k chunks of
+                         64; default 2^30 = all), the others k-streaming */
   int32_t adj_wg;     /* adjoint workgroup size: 256, 512 or 1024 */
   int32_t order;      /* 0: events in the given order; 1: largest first */
   float fwd_ms;       /* adt_autotune: measured forward / adjoint time per */

@@ -83,11 +83,12 @@ def test_tune_get_set_validation_without_device():
     import adtomo_hip as A
     base = A.get_tune("float64")
     try:
-        assert base["diag_max_l"] == 64 and base["adj_wg"] in (256, 512, 1024)
+        assert base["diag_max_l"] == 1 << 30 and base["adj_wg"] in (256, 512, 1024)
         t = A.set_tune("float64", diag_max_l=0, adj_wg=256, order=0)
         assert A.get_tune("float64")["adj_wg"] == 256 and t["diag_max_l"] == 0
         assert A.get_tune("float32")["adj_wg"] == base["adj_wg"]  # per precision
-        for bad in ({"diag_max_l": 65}, {"diag_max_l": -1}, {"adj_wg": 300}, {"order": 2}):
+        for bad in ({"diag_max_l": (1 << 30) + 1}, {"diag_max_l": -1}, {"adj_wg": 300},
+                    {"order": 2}):
             with pytest.raises(A.AdtError):
                 A.set_tune("float64", **bad)
     finally:

@@ -372,6 +372,7 @@ def _mixed_batch(seed=300):
 
 
 @pytest.mark.parametrize("setting", [
+    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1},
     {"diag_max_l": 64, "adj_wg": 1024, "order": 1},
     {"diag_max_l": 0, "adj_wg": 256, "order": 0},
     {"diag_max_l": 32, "adj_wg": 512, "order": 1},
@@ -412,7 +413,7 @@ def test_autotune_stores_a_valid_setting():
         best = A.autotune([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
                           [1e-3] * len(fs), grad_us=[dev(a) for a in gs], reps=2)
         print("autotune:", best, flush=True)
-        assert best["diag_max_l"] in (0, 64) and best["adj_wg"] in (256, 512, 1024)
+        assert best["diag_max_l"] in (0, 64, 1 << 30) and best["adj_wg"] in (256, 512, 1024)
         assert best["order"] in (0, 1) and best["fwd_ms"] > 0 and best["adj_ms"] > 0
         cur = A.get_tune("float64")
         assert all(cur[k] == best[k] for k in ("diag_max_l", "adj_wg", "order"))

@@ -134,3 +134,34 @@ def test_struct_layouts_agree_across_bindings(c_name, py_name, jl_name):
     assert py == c, (py, c)
     if jl_name:
         assert _julia_struct_fields(jl_name) == c
+
+
+def test_objective_and_tuning_argument_errors_without_device():
+    # every new entry point validates its arguments before touching a device
+    from adtomo_hip import _lib as L
+    lib = L.lib()
+    buf = np.zeros(64)
+    p = buf.ctypes.data
+    st = (L.AdtSource3d * 1)(L.AdtSource3d(0.5, 2.0, 2.0, 1.0, p))  # x below 1
+    assert lib.adt_source_u0_3d_batched_f64(st, 1, p, 4, 4, 4, 1.0, None) == -1
+    assert b"outside" in lib.adt_last_error()
+    mis = (L.AdtMisfit3d * 1)(L.AdtMisfit3d(4, 4, 4, p, p, p, None, None, p, 0.0))
+    coords = np.array([1.0, 2.0, 4.5])  # z beyond l = 4
+    ptr = coords.ctypes.data
+    assert lib.adt_misfit3d_batched_f64(mis, 1, ptr, ptr, ptr + 16, 1, None) == -1
+    assert b"outside" in lib.adt_last_error()
+    assert lib.adt_misfit3d_batched_f64(mis, -1, ptr, ptr, ptr, 1, None) == -1
+    assert lib.adt_smooth3d_l1_f64(p, 4, 4, 4, 2, 3, 1.0, p, p, None) == -1
+    assert lib.adt_smooth3d_l1_f64(p, 4, 4, 4, 5, 3, 1.0, p, p, None) == -1
+    assert lib.adt_sum_arrays_f64(None, 0, 8, p, None) == -1
+    assert lib.adt_slowness3d_f64(p, -1, 1.7, p, None) == -1
+    assert lib.adt_slowness3d_grad_f64(None, 8, 1.7, None, p, p, None) == -1
+    assert lib.adt_model3d_f64(None, p, 8, p, p, None) == -1
+    assert lib.adt_model3d_grad_f64(p, p, None, None, 8, None, None) == -1
+    ev = (L.AdtEvent3d * 1)()
+    best = L.AdtTune()
+    assert lib.adt_autotune(ev, 0, L.ADT_FORWARD, 0, 1, ctypes.byref(best), None) == -1
+    assert lib.adt_autotune(ev, 1, 0, 0, 1, ctypes.byref(best), None) == -1
+    assert lib.adt_autotune(ev, 1, L.ADT_FORWARD, 2, 1, ctypes.byref(best), None) == -1
+    g = ctypes.c_void_p()
+    assert lib.adt_graph_create(ev, 1, 0, 0, ctypes.byref(g), None) == -1

@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the forward kernel on this workload (256 events),
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy kernel of
 # known size: profiles/pmc_traffic_fwd3_diag_adj3_kahn_256x128x128x64.md
-FWD_TRAFFIC_BYTES_256 = 112.0e9 + 58.3e9
+FWD_TRAFFIC_BYTES_256 = 110.8e9 + 58.4e9
 
 
 def log(*a):
@@ -165,36 +165,45 @@ def main():
 
     # FP32 variant of the same step, with its error against FP64 (the FP64
     # device results are bit-exact with the oracle's travel times)
-    fp32 = None
+    fp32 = mixed = None
     if not args.no_fp32:
         ff = fd.float()
         u0f = [a.float() for a in u0d]
         gdf = [a.float() for a in gd]
-        if not args.no_autotune:
-            A.autotune(u0f, [ff] * nev, [h] * nev, [tol] * nev, grad_us=gdf, reps=1)
-        G32 = A.EikonalGraph(u0f, [ff] * nev, [h] * nev, [tol] * nev, grad_us=gdf)
-        ms32 = time_graph(G32, max(2, args.steps))
-        K32 = np.array([i["iterations"] for i in G32.infos()])
-        eu_max = eu_ss = u_max = eg_max = eg_ss = g_max = 0.0
-        for e in range(nev):
-            du = G32.us[e].double() - G.us[e]
-            dg = G32.grad_fs[e].double() - G.grad_fs[e]
-            eu_max = max(eu_max, float(du.abs().max()))
-            eg_max = max(eg_max, float(dg.abs().max()))
-            eu_ss += float((du * du).sum())
-            eg_ss += float((dg * dg).sum())
-            u_max = max(u_max, float(G.us[e].abs().max()))
-            g_max = max(g_max, float(G.grad_fs[e].abs().max()))
-        fp32 = {"event_solves_per_s": nev / (ms32 / 1e3), "ms_per_step": ms32,
-                "K_mean": float(K32.mean()),
-                "u_err_max_rel": eu_max / u_max, "u_err_rms_s": (eu_ss / (nev * N)) ** 0.5,
-                "grad_f_err_max_rel": eg_max / g_max,
-                "grad_f_err_rms": (eg_ss / (nev * N)) ** 0.5}
-        G32.close()
+
+        def variant(mix):
+            if not args.no_autotune:
+                A.autotune(u0f, [ff] * nev, [h] * nev, [tol] * nev, grad_us=gdf, reps=1,
+                           compute_f64=mix)
+            G32 = A.EikonalGraph(u0f, [ff] * nev, [h] * nev, [tol] * nev, grad_us=gdf,
+                                 compute_f64=mix)
+            ms32 = time_graph(G32, max(2, args.steps))
+            K32 = np.array([i["iterations"] for i in G32.infos()])
+            eu_max = eu_ss = u_max = eg_max = eg_ss = g_max = 0.0
+            for e in range(nev):
+                du = G32.us[e].double() - G.us[e]
+                dg = G32.grad_fs[e].double() - G.grad_fs[e]
+                eu_max = max(eu_max, float(du.abs().max()))
+                eg_max = max(eg_max, float(dg.abs().max()))
+                eu_ss += float((du * du).sum())
+                eg_ss += float((dg * dg).sum())
+                u_max = max(u_max, float(G.us[e].abs().max()))
+                g_max = max(g_max, float(G.grad_fs[e].abs().max()))
+            G32.close()
+            out = {"event_solves_per_s": nev / (ms32 / 1e3), "ms_per_step": ms32,
+                   "K_mean": float(K32.mean()),
+                   "u_err_max_rel": eu_max / u_max, "u_err_rms_s": (eu_ss / (nev * N)) ** 0.5,
+                   "grad_f_err_max_rel": eg_max / g_max,
+                   "grad_f_err_rms": (eg_ss / (nev * N)) ** 0.5}
+            log(f"{'fp32 storage, fp64 solves' if mix else 'fp32'}: {ms32:.2f} ms/step -> "
+                f"{out['event_solves_per_s']:.1f} event-solves/s; travel time max rel err "
+                f"{out['u_err_max_rel']:.2e}, rms {out['u_err_rms_s']:.2e} s; "
+                f"grad_f max rel err {out['grad_f_err_max_rel']:.2e}")
+            return out
+
+        fp32 = variant(False)
+        mixed = variant(True)
         del u0f, gdf, ff
-        log(f"fp32: {ms32:.2f} ms/step -> {fp32['event_solves_per_s']:.1f} event-solves/s; "
-            f"travel time max rel err {fp32['u_err_max_rel']:.2e}, rms {fp32['u_err_rms_s']:.2e} s; "
-            f"grad_f max rel err {fp32['grad_f_err_max_rel']:.2e}")
 
     plain = None
     if not args.no_plain and args.plain_events > 0:
@@ -248,6 +257,7 @@ def main():
                      "traffic": FWD_TRAFFIC_BYTES_256 if nev == 256 else None},
         "cpu_baseline": cpu,
         "fp32": fp32,
+        "fp32_storage_fp64_solve": mixed,
         "plain_event_solves_per_s": plain,
         "batched_vs_plain": (value / plain) if plain else None,
         "node_updates_per_s": node_updates / (fwd_ms / 1e3),

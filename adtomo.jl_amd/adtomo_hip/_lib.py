@@ -34,7 +34,7 @@ class AdtInfo(ctypes.Structure):
         ("ambiguous", _i32),
         ("adj_sweeps", _i32),
         ("adj_converged", _i32),
-        ("reserved", _i32),
+        ("fault", _i32),
         ("n_singular", _i64),
         ("err_log", _d * MAX_LOGGED_ITERS),
     ]
@@ -48,6 +48,7 @@ class AdtInfo(ctypes.Structure):
             "ambiguous": bool(self.ambiguous),
             "adj_sweeps": self.adj_sweeps,
             "adj_converged": bool(self.adj_converged),
+            "fault": self.fault,
             "n_singular": self.n_singular,
             "err_log": list(self.err_log[:k]),
         }

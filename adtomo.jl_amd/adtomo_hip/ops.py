@@ -43,6 +43,14 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _checked(info: dict) -> dict:
+    """Raises if a kernel reported an internal fault for this event (a
+    bounded wait between its waves timed out; its results are invalid)."""
+    if info.get("fault"):
+        raise L.AdtError("kernel fault reported for an event (wave synchronisation timed out)")
+    return info
+
+
 def _sfx(dtype) -> str:
     name = str(dtype)
     return "_f32" if name.endswith("float32") else "_f64"
@@ -54,7 +62,7 @@ def _np_dtype(x):
 
 def _set_info(info: L.AdtInfo):
     global last_info
-    last_info = info.as_dict()
+    last_info = _checked(info.as_dict())
     return last_info
 
 
@@ -284,7 +292,7 @@ def eikonal3d_batched(u0s, fs, hs, tols, grad_us=None, forward=True, us=None,
         _stream())
     L.check(rc, "eikonal3d_batched")
     return {"u": us, "grad_u0": gu0s, "grad_f": gfs,
-            "info": [infos[e].as_dict() for e in range(nev)] if infos is not None else None}
+            "info": [_checked(infos[e].as_dict()) for e in range(nev)] if infos is not None else None}
 
 
 def _dtype_code(dtype):
@@ -357,7 +365,7 @@ class EikonalGraph:
     def infos(self):
         arr = (L.AdtInfo * max(self.nev, 1))()
         L.check(L.lib().adt_graph_infos(self._g, ctypes.byref(arr), _stream()), "adt_graph_infos")
-        return [arr[e].as_dict() for e in range(self.nev)]
+        return [_checked(arr[e].as_dict()) for e in range(self.nev)]
 
     def close(self):
         if self._g:

@@ -193,6 +193,7 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   d.uold = base;
   d.info = (adt_info*)(base + off_info);
   ADT_HIP(hipMemcpyAsync(base + off_desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
   const adt_tune tn = current_tune<T>();
   const bool mix = flags & ADT_F32_COMPUTE_F64;
   if (l <= tn.diag_max_l)
@@ -206,6 +207,7 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   ADT_HIP(hipStreamSynchronize(s));
   if (verbose) print_verbose(hinf);
   if (info) *info = hinf;
+  if (hinf.fault) return fail(ADT_EHIP, "eikonal3d: wave synchronisation timed out in the kernel");
   return ADT_OK;
 }
 
@@ -252,6 +254,7 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   const size_t od = a + c + q;
   d.info = (adt_info*)(base + od + 256);
   ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
   adt::adj3_kahn_kernel<T><<<1, current_tune<T>().adj_wg, 0, s>>>((const adt::EventDesc*)(base + od));
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
@@ -300,6 +303,7 @@ int fwd2_impl(const T* f, int64_t m, int64_t n, int64_t srcx, int64_t srcy, T h,
   d.uold = base;
   d.info = (adt_info*)(base + a + 256);
   ADT_HIP(hipMemcpyAsync(base + a, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
   adt::fwd2_event_kernel<T><<<1, kWG, 0, s>>>((const adt::Event2Desc*)(base + a), kMaxIter2);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
@@ -350,6 +354,7 @@ int bwd2_impl(const T* g, const T* u, const T* f, int64_t m, int64_t n, int64_t 
   const size_t od = a + 2 * c + q;
   d.info = (adt_info*)(base + od + 256);
   ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
   adt::adj2_kahn_kernel<T><<<1, kWG, 0, s>>>((const adt::Event2Desc*)(base + od));
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
@@ -453,8 +458,10 @@ struct BatchPlan {
 
 template <typename T>
 int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, const adt_tune& tn,
-                 hipStream_t s) {
+                 adt_info* infos, hipStream_t s) {
   if (nev == 0) return ADT_OK;
+  adt::info_clear_kernel<<<1, 256, 0, s>>>(infos, nev);
+  ADT_HIP(hipGetLastError());
   if (flags & ADT_FORWARD) {
     // events with l <= diag_max_l: diagonal streaming; longer ones: k-streaming
     const bool mix = flags & ADT_F32_COMPUTE_F64;
@@ -492,12 +499,15 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   ADT_HIP(hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
                          hipMemcpyHostToDevice, s));
   if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags,
-                            plan.max_l, tn, s)))
+                            plan.max_l, tn, (adt_info*)(base + plan.off_info), s)))
     return rc;
   if (infos) {
     ADT_HIP(hipMemcpyAsync(infos, base + plan.off_info, sizeof(adt_info) * nev,
                            hipMemcpyDeviceToHost, s));
     ADT_HIP(hipStreamSynchronize(s));
+    for (int e = 0; e < nev; ++e)
+      if (infos[e].fault)
+        return fail(ADT_EHIP, "batched: wave synchronisation timed out in event " + std::to_string(e));
   }
   return ADT_OK;
 }
@@ -673,11 +683,12 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     }
     // time `reps` launches of one setting after one untimed launch
     auto time_it = [&](int fl, const adt_tune& t, float* ms) -> int {
-      int r = launch_batch<T>(dd, nev, fl, plan.max_l, t, s);
+      int r = launch_batch<T>(dd, nev, fl, plan.max_l, t, (adt_info*)(base + plan.off_info), s);
       if (r) return r;
       ADT_HIP(hipEventRecord(e0, s));
       for (int k = 0; k < reps; ++k)
-        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, t, s))) return r;
+        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, t, (adt_info*)(base + plan.off_info), s)))
+          return r;
       ADT_HIP(hipEventRecord(e1, s));
       ADT_HIP(hipEventSynchronize(e1));
       ADT_HIP(hipEventElapsedTime(ms, e0, e1));
@@ -702,7 +713,7 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
       if (rc) break;
       // leave this order's best forward in the outputs for the adjoint
       if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), plan.max_l, t,
-                                s)))
+                                (adt_info*)(base + plan.off_info), s)))
         break;
     }
     if (flags & ADT_ADJOINT) {
@@ -777,7 +788,8 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, tn, cs);
+    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, tn,
+                         (adt_info*)(base + plan.off_info), cs);
     hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
     if (rc == ADT_OK && e2 != hipSuccess) e = e2;
   }
@@ -907,6 +919,9 @@ int adt_graph_infos(adt_graph* g, adt_info* infos, void* stream) {
   ADT_HIP(hipMemcpyAsync(infos, (char*)g->g->ws + g->g->off_info, sizeof(adt_info) * g->g->nev,
                          hipMemcpyDeviceToHost, s));
   ADT_HIP(hipStreamSynchronize(s));
+  for (int e = 0; e < g->g->nev; ++e)
+    if (infos[e].fault)
+      return fail(ADT_EHIP, "graph: wave synchronisation timed out in event " + std::to_string(e));
   return ADT_OK;
 }
 int adt_graph_destroy(adt_graph* g) {

@@ -92,6 +92,7 @@ constexpr size_t kAdjRecBytes = 4 * sizeof(T);
 constexpr int kDiagAll = 1 << 30;
 template <typename T, bool MIX>
 __global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
+__global__ void info_clear_kernel(adt_info* info, int n);
 template <typename T>
 __global__ void copy_kernel(T* dst, const T* src, int64_t N);
 template <typename T>

@@ -419,6 +419,42 @@ __device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> p
 
 }  // namespace
 
+// Point-to-point step synchronisation of the waves of a sweep.  Wave w's
+// step t reads the slots its neighbours w-1 (xch) and w+1 (xold) wrote at
+// their step t-1, and at its end overwrites the slots they read at their
+// step t-1: so before step t a wave waits until both neighbours have
+// completed t steps (prog[] counts completed steps), and after its own
+// writes it publishes t+1.  Waves drift at most one step apart instead of
+// meeting at a workgroup barrier every step.  Every wait is bounded: past
+// kSpinCap polls it sets *abort, after which no wave waits any more, the
+// kernel runs to its end and reports a fault.
+constexpr int kSpinCap = 1 << 16;
+
+__device__ __forceinline__ void wait_neighbours(int* prog, int* abort_flag, int w, int t) {
+  if (t == 0) return;
+  for (int spin = 0;; ++spin) {
+    const int a = w > 0 ? __hip_atomic_load(&prog[w - 1], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP)
+                        : t;
+    const int b = w + 1 < kDW ? __hip_atomic_load(&prog[w + 1], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
+                              : t;
+    if (a >= t && b >= t) break;
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+    if (spin >= kSpinCap) {
+      __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ void publish_progress(int* prog, int w, int done) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __hip_atomic_store(&prog[w], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // One sweep of one event by the whole workgroup (the steps of wave w).
 //
 // Every global memory instruction of the step loop is issued on every step
@@ -428,7 +464,8 @@ __device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> p
 // the loop body: landing a load never waits for a store issued after it.
 template <typename T, bool FULL, bool MIX, bool ERR, bool CH>
 __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
-                                                    lptr<T> xch, lptr<T> xold, lptr<T> planes) {
+                                                    lptr<T> xch, lptr<T> xold, lptr<T> planes,
+                                                    int* prog, int* abort_flag) {
   using P = Pieces<T>;
   const DGeom& g = S.g;
   const int p = threadIdx.x & 63;
@@ -516,6 +553,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   T XCM_n = load_cm(cv), XCP_n = load_cp(cv);
   if constexpr (ERR) seg_issue<T, FULL>(FS, S.snap_src, -1, cnt, p);
   seg_publish<T, FULL>(FD, xold + kDW * 64 + w * 64, cnt, fwd, p);
+  if (p == 0) __hip_atomic_store(&prog[w], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   __syncthreads();
 
   const int steps = g.V + kDW - 1;
@@ -561,6 +599,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     // v+2, slowness and band edges of v+1) are issued between the first
     // nodes, every step, so the memory pipeline works alongside the math.
     T xo = T(0);
+    wait_neighbours(prog, abort_flag, w, t);
     {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : XCM;
       const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : XCP;
@@ -626,7 +665,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     cv = cv1;
     cv1 = cv2;
     cv2 = dcol(g, v + 3, p, S.base0, S.tA, S.tB);
-    __syncthreads();
+    if (p == 0) publish_progress(prog, w, t + 1);
   }
   if constexpr (ERR)
     seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, cnt, fwd, p, FS, key);
@@ -643,18 +682,18 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
 // key and writes the next snapshot.
 template <typename T, bool MIX, bool ERR>
 __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> xold,
-                                               lptr<T> planes) {
+                                               lptr<T> planes, int* prog, int* abort_flag) {
   // the wave index is uniform across the wave: keep it (and everything
   // derived from it) in scalar registers
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
   if (S.Lfull > kDW * kLW) {
-    if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, true>(S, w, cnt, xch, xold, planes);
-    return diag_sweep_wave<T, false, MIX, ERR, true>(S, w, cnt, xch, xold, planes);
+    if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, true>(S, w, cnt, xch, xold, planes, prog, abort_flag);
+    return diag_sweep_wave<T, false, MIX, ERR, true>(S, w, cnt, xch, xold, planes, prog, abort_flag);
   }
-  if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, false>(S, w, cnt, xch, xold, planes);
-  return diag_sweep_wave<T, false, MIX, ERR, false>(S, w, cnt, xch, xold, planes);
+  if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, false>(S, w, cnt, xch, xold, planes, prog, abort_flag);
+  return diag_sweep_wave<T, false, MIX, ERR, false>(S, w, cnt, xch, xold, planes, prog, abort_flag);
 }
 
 template <typename T, bool MIX>
@@ -664,11 +703,15 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   __shared__ T xold[2 * kDW * 64];
   __shared__ T planes[kDW * 3 * Pieces<T>::PLANE];
   __shared__ uint64_t red[kDW];
+  __shared__ int prog[kDW];
+  __shared__ int abort_flag;
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
   if (l > max_l) return;  // handled by the k-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
+  if (threadIdx.x == 0) abort_flag = 0;
+  __syncthreads();
   DSweep<T> S;
   S.g = make_dgeom(m, n, l);
   S.Lfull = l;
@@ -710,13 +753,15 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
                   (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) +
                   (S.flipC ? (int64_t)(l - 1 - S.cs) : (int64_t)S.cs);
         if (sw < 7) {
-          diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
+          diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes, prog,
+                                    &abort_flag);
         } else {
           // err = max |u - u_old| (Eikonal3D.cpp:79-81), taken as the last
           // sweep stores each node, which also writes u_old = u for the
           // next iteration (u_old of the first iteration is u0 itself)
           const uint64_t kc =
-              diag_sweep<T, MIX, true>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes);
+              diag_sweep<T, MIX, true>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes, prog,
+                                       &abort_flag);
           key = kc > key ? kc : key;
         }
       }
@@ -741,6 +786,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
     ev.info->iterations = it > max_iter ? max_iter : it;
     ev.info->converged = (err < tol) ? 1 : 0;
     ev.info->err = (double)err;
+    if (abort_flag) ev.info->fault = 1;
   }
 }
 

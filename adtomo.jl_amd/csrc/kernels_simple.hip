@@ -14,6 +14,14 @@
 
 namespace adt {
 
+// Clears the per-event diagnostics before a launch writes its fields (a
+// kernel, not a memset, so that it can sit in a captured graph).
+__global__ void info_clear_kernel(adt_info* __restrict__ info, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    info[i] = adt_info{};
+}
+
+
 // ------------------------------------------------------------------ 3D ---
 
 template <typename T>

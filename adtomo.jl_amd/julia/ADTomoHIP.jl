@@ -33,7 +33,7 @@ struct AdtInfo
     ambiguous::Int32
     adj_sweeps::Int32
     adj_converged::Int32
-    reserved::Int32
+    fault::Int32
     n_singular::Int64
     err_log::NTuple{100,Float64}
 end

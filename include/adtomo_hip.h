@@ -67,7 +67,8 @@ typedef struct adt_info {
   int32_t ambiguous;     /* 2D: err within 1e-12 (rel) of the 1e-8 threshold */
   int32_t adj_sweeps;    /* adjoint Gauss-Seidel sweeps run */
   int32_t adj_converged; /* 1 if the adjoint reached its exact fixed point */
-  int32_t reserved;
+  int32_t fault;         /* nonzero: a kernel's bounded wait for another wave
+                            timed out, so this event's results are invalid */
   int64_t n_singular;    /* 2D adjoint: non-source nodes with no upwind
                             neighbour (singular rows in the reference) */
   double err_log[ADT_MAX_LOGGED_ITERS];

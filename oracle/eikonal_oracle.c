@@ -14,7 +14,10 @@
  *   adt_oracle_eikonal2d_bwd_f64  deps/CustomOps/Eikonal/Eikonal.h:95-199
  *   adt_oracle_eikonal3d_fwd_f64  deps/CustomOps/Eikonal3D/Eikonal3D.cpp:11-94
  *   adt_oracle_eikonal3d_bwd_f64  deps/CustomOps/Eikonal3D/Eikonal3D.cpp:96-198
- * The _f32 twins evaluate the same expressions in binary32.
+ * The _f32 twins evaluate the same expressions in binary32;
+ * adt_oracle_eikonal3d_fwd_m32 keeps binary32 storage and evaluates each
+ * local solve in binary64, rounded once (the library's ADT_F32_COMPUTE_F64
+ * variant; pinned against an independent numpy formulation in the tests).
  */
 #include <math.h>
 #include <stdint.h>

@@ -596,8 +596,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
 
     // Node updates of diagonal v (lanes of inactive steps hold no valid
     // column and skip them).  The loads for the next step (old values of
-    // v+2, slowness and band edges of v+1) are issued between the first
-    // nodes, every step, so the memory pipeline works alongside the math.
+    // v+2, slowness and band edges of v+1) are issued with the first
+    // nodes, every step, so they have nearly a whole step to arrive.
     T xo = T(0);
     wait_neighbours(prog, abort_flag, w, t);
     {
@@ -621,10 +621,10 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       const bool yf = g.a_is_i ? bf : af, yl = g.a_is_i ? bl : al;
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
-        if (e == 1) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
-        if (e == 3) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
-        if (e == 4) issue_edge(cv1);
-        if (e == 5) {
+        if (e == 0) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
+        if (e == 1) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
+        if (e == 2) issue_edge(cv1);
+        if (e == 2) {
           XCM_n = load_cm(cv1);
           XCP_n = load_cp(cv1);
         }

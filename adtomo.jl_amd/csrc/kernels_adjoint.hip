@@ -45,6 +45,12 @@ __device__ __forceinline__ unsigned dec_byte(uint8_t* base, int64_t idx) {
 //            bits 13-15 number of feeders not yet final.  Z nodes start
 //            with 7, so releases (at most 6) never make them ready.
 // Releasing a neighbour is one 32-bit atomic on the word holding its meta.
+//
+// Queue entries: for events of at most 2^20 nodes an entry carries, beside
+// the node index (bits 12-31), the node's upwind code (bits 0-5) and feeder
+// mask (bits 6-11), both final once the prep passes end, so that resolving
+// a node does not read its meta word (one scattered read less per node).
+// Larger events store the plain index and read the meta word.
 template <typename T>
 struct alignas(kAdjRecBytes<T>) Rec {
   T lam;
@@ -87,6 +93,7 @@ constexpr uint32_t kCntShift = 13;
 // issued before any is used, so more requests are in flight per wave).
 constexpr int kPrepNodes = 2;
 constexpr uint32_t kZ = 0x40u;
+constexpr int64_t kPackedMaxNodes = int64_t(1) << 20;
 
 __device__ __forceinline__ uint32_t meta_shift(int64_t p) { return (p & 1) ? 16u : 0u; }
 
@@ -156,13 +163,13 @@ struct Gathered {
 
 template <typename T>
 __device__ __forceinline__ void gather3(gptr<const Rec<T>> rec, gptr<const uint16_t> meta, int n,
-                                        int l, int64_t r, bool valid, Gathered<T>& G) {
+                                        int l, uint32_t e, bool packed, bool valid, Gathered<T>& G) {
   const int64_t nl = (int64_t)n * l;
   const int64_t st[3] = {nl, l, 1};
-  G.r = valid ? r : 0;
+  const int64_t r = !valid ? 0 : packed ? (int64_t)(e >> 12) : (int64_t)e;
+  G.r = r;
   G.valid = valid;
-  r = G.r;
-  G.mr = valid ? (uint32_t)meta[r] : 0u;
+  G.mr = !valid ? 0u : packed ? ((e & 63u) | (((e >> 6) & 63u) << 7)) : (uint32_t)meta[r];
   G.own.lam = rec[r].lam;
 #pragma unroll
   for (int d = 0; d < 3; ++d) G.own.w[d] = rec[r].w[d];
@@ -212,12 +219,18 @@ __device__ __forceinline__ void resolve3(gptr<Rec<T>> rec, gptr<uint32_t> meta_w
   }
 }
 
+// Queue entry of node p whose meta is mt (the packed form for small events).
+__device__ __forceinline__ uint32_t qentry(int64_t p, uint32_t mt, bool packed) {
+  return packed ? ((uint32_t)p << 12) | (mt & 63u) | (((mt >> 7) & 63u) << 6) : (uint32_t)p;
+}
+
 __device__ __forceinline__ void push_ready(const uint32_t (&old)[3], const int64_t (&tgt)[3],
-                                           gptr<int32_t> queue, int* qtail) {
+                                           gptr<uint32_t> queue, int* qtail, bool packed) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
-    if (((old[d] >> (kCntShift + meta_shift(tgt[d]))) & 7u) == 1u)
-      queue[atomicAdd(qtail, 1)] = (int32_t)tgt[d];
+  for (int d = 0; d < 3; ++d) {
+    const uint32_t mt = old[d] >> meta_shift(tgt[d]);
+    if (((mt >> kCntShift) & 7u) == 1u) queue[atomicAdd(qtail, 1)] = qentry(tgt[d], mt, packed);
+  }
 }
 
 template <typename T>
@@ -228,11 +241,12 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
   const int m = ev.m, n = ev.n, l = ev.l;
   const int64_t nl = (int64_t)n * l;
   const int64_t N = (int64_t)m * nl;
+  const bool packed = N <= kPackedMaxNodes;
   const gptr<const T> u = (gptr<const T>)ev.u;
   const gptr<const T> u0 = (gptr<const T>)ev.u0;
   const gptr<const T> g = (gptr<const T>)ev.grad_u;
   const gptr<T> gu0 = (gptr<T>)ev.grad_u0;
-  const gptr<int32_t> queue = (gptr<int32_t>)ev.queue;
+  const gptr<uint32_t> queue = (gptr<uint32_t>)ev.queue;
   const gptr<Rec<T>> rec = (gptr<Rec<T>>)ev.lam;
   const gptr<uint16_t> meta = (gptr<uint16_t>)ev.code;
   const gptr<uint32_t> meta_w = (gptr<uint32_t>)ev.code;
@@ -328,10 +342,11 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
 #pragma unroll
           for (int d = 0; d < 3; ++d) rec[p].w[d] = rp.w[d];
           const uint32_t cnt = z ? 7u : (uint32_t)__builtin_popcount(fm);
-          meta[p] = (uint16_t)((cr[k] & 0x7fu) | (fm << 7) | (cnt << kCntShift));
+          const uint32_t mt = (cr[k] & 0x7fu) | (fm << 7) | (cnt << kCntShift);
+          meta[p] = (uint16_t)mt;
           if (!z) {
             ++my_nonz;
-            if (cnt == 0) queue[atomicAdd(&qtail, 1)] = (int32_t)p;
+            if (cnt == 0) queue[atomicAdd(&qtail, 1)] = qentry(p, mt, packed);
           }
         }
       }
@@ -346,16 +361,16 @@ __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __rest
       // two queue entries per thread, resolved with their loads overlapped
       const int i0 = base + threadIdx.x, i1 = base + bs + threadIdx.x;
       const bool v0 = i0 < tail, v1 = i1 < tail;
-      const int64_t r0 = queue[v0 ? i0 : head], r1 = queue[v1 ? i1 : head];
+      const uint32_t e0 = queue[v0 ? i0 : head], e1 = queue[v1 ? i1 : head];
       Gathered<T> G0, G1;
-      gather3((gptr<const Rec<T>>)rec, (gptr<const uint16_t>)meta, n, l, r0, v0, G0);
-      gather3((gptr<const Rec<T>>)rec, (gptr<const uint16_t>)meta, n, l, r1, v1, G1);
+      gather3((gptr<const Rec<T>>)rec, (gptr<const uint16_t>)meta, n, l, e0, packed, v0, G0);
+      gather3((gptr<const Rec<T>>)rec, (gptr<const uint16_t>)meta, n, l, e1, packed, v1, G1);
       uint32_t o0[3], o1[3];
       int64_t t0[3], t1[3];
       resolve3(rec, meta_w, n, l, G0, o0, t0);
       resolve3(rec, meta_w, n, l, G1, o1, t1);
-      push_ready(o0, t0, queue, &qtail);
-      push_ready(o1, t1, queue, &qtail);
+      push_ready(o0, t0, queue, &qtail, packed);
+      push_ready(o1, t1, queue, &qtail, packed);
     }
     __syncthreads();
     head = tail;

@@ -195,6 +195,27 @@ def test_config2_forward_adjoint(tol):
     assert e <= GRAD_RTOL
 
 
+@pytest.mark.parametrize("shape", [(128, 128, 64), (129, 128, 64)])
+def test_adjoint_at_queue_entry_packing_limit(shape):
+    # 128x128x64 is exactly 2^20 nodes, the largest event whose queue entries
+    # carry the node's code and feeder mask beside its index (the last index
+    # fills all 20 index bits); 129x128x64 is past it and stores plain indices
+    import adtomo_hip as A
+    f = 1.0 / synth.layered_grf_model(shape, 0.5, seed=shape[0])
+    u0 = synth.point_source_u0(shape, synth.interior_source(shape, seed=shape[0] + 1))
+    u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, 1e-3)
+    u = A.eikonal3d(dev(u0), dev(f), 0.5, *shape, 1e-3, False)
+    assert bits_equal(host(u), u_ref)
+    g = synth.sparse_adjoint_source(shape, 60, seed=shape[0] + 2, scale=1.0)
+    gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(g, u_ref, u0, f, 0.5)
+    gu0, gf = A.eikonal3d_grad(dev(g), u, dev(u0), dev(f), 0.5, *shape)
+    assert A.ops.last_info["adj_converged"]
+    assert bits_equal(host(gu0), gu0_ref)
+    e = rel(host(gf), gf_ref)
+    print(shape, "grad rel err", e, "levels", A.ops.last_info["adj_sweeps"], flush=True)
+    assert e <= GRAD_RTOL
+
+
 def test_headline_single_event_bit_exact():
     import adtomo_hip as A
     f, u0s, gs, srcs = synth.headline_event_set(1)

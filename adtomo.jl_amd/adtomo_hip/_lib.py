@@ -96,6 +96,11 @@ _SIGS = {
     "adt_eikonal2d_bwd": [_p, _p, _p, _i64, _i64, _i64, _i64, "R", _p, _p, _i32, _p],
     "adt_eikonal3d_fwd": [_p, _p, "R", _i64, _i64, _i64, "R", _i32, _p, _p, _i32, _p],
     "adt_eikonal3d_bwd": [_p, _p, _p, _p, "R", _i64, _i64, _i64, _p, _p, _p, _i32, _p],
+    "adt_eikonal2d_fwd_strided": [_p, _i64, _i64, _i64, _i64, "R", _p, _p, _p, _i32, _p],
+    "adt_eikonal2d_bwd_strided": [_p, _p, _p, _i64, _i64, _i64, _i64, "R", _p, _p, _p, _i32, _p],
+    "adt_eikonal3d_fwd_strided": [_p, _p, "R", _i64, _i64, _i64, "R", _i32, _p, _p, _p, _i32, _p],
+    "adt_eikonal3d_bwd_strided": [_p, _p, _p, _p, "R", _i64, _i64, _i64, _p, _p, _p, _p, _i32,
+                                  _p],
     "adt_eikonal3d_batched": [_p, _i32, _i32, _p, _p],
     "adt_selftest_arith": [_i32, _p, _p, _p, _i64, _p],
     "adt_source_u0_3d_batched": [_p, _i32, _p, _i64, _i64, _i64, "R", _p],

@@ -238,6 +238,116 @@ def eikonal3d(u0, f, h, m, n, l, tol, verbose=False):
     return _eik3d_fwd_raw(np.asarray(u0), np.asarray(f), float(h), m, n, l, float(tol), verbose)
 
 
+# ------------------------------------------------------ strided layouts ---
+#
+# The _strided entry points take arrays of any common layout: a Fortran-
+# ordered numpy array, a permuted or sliced torch view.  All arrays of a
+# call share one shape and one set of strides; outputs get the same strides.
+
+def _layout(arrays, ndim):
+    """(strides in elements, device flag) shared by ``arrays``; raises
+    unless they agree and every stride of an axis longer than one is
+    positive."""
+    ref = arrays[0]
+    if ref.ndim != ndim:
+        raise ValueError(f"strided calls take {ndim}-D arrays")
+    dev = _is_torch(ref)
+    st = None
+    for a in arrays:
+        if _is_torch(a) != dev or tuple(a.shape) != tuple(ref.shape) or a.dtype != ref.dtype:
+            raise ValueError("arrays of a strided call must share kind, shape and dtype")
+        if dev:
+            if not a.is_cuda:
+                raise ValueError("torch tensors must be on the HIP device")
+            s = tuple(int(x) for x in a.stride())
+        else:
+            if any(x % a.itemsize for x in a.strides):
+                raise ValueError("numpy strides must be whole elements")
+            s = tuple(int(x) // a.itemsize for x in a.strides)
+        if st is None:
+            st = s
+        elif s != st:
+            raise ValueError("arrays of a strided call must share their strides")
+    if any(x <= 0 for x, d in zip(st, ref.shape) if d > 1):
+        raise ValueError("strides must be positive")
+    return st, dev
+
+
+def _empty_like_strided(a, st):
+    """Uninitialised array with a's shape and element strides ``st``."""
+    if _is_torch(a):
+        return _torch().empty_strided(tuple(a.shape), st, dtype=a.dtype, device=a.device)
+    span = 1 + sum((d - 1) * x for d, x in zip(a.shape, st) if d > 1)
+    base = np.empty(span, dtype=a.dtype)
+    return np.lib.stride_tricks.as_strided(base, a.shape, tuple(x * a.itemsize for x in st))
+
+
+def _ptr(a):
+    return a.data_ptr() if _is_torch(a) else a.ctypes.data
+
+
+def _call_strided(name, dev, dtype, args, info):
+    fn = getattr(L.lib(), name + _sfx(dtype))
+    rc = fn(*args, ctypes.byref(info), 0 if dev else L.ADT_HOST_POINTERS,
+            _stream() if dev else None)
+    L.check(rc, name)
+    _set_info(info)
+
+
+def eikonal3d_strided(u0, f, h, tol, verbose=False):
+    """EikonalThreeD (EikonalThreeD.cpp:14-45) on (m, n, l) arrays of any
+    common layout (adt_eikonal3d_fwd_strided_*); returns u with the same
+    strides."""
+    st, dev = _layout([u0, f], 3)
+    m, n, l = (int(x) for x in f.shape)
+    u = _empty_like_strided(f, st)
+    info = L.AdtInfo()
+    _call_strided("adt_eikonal3d_fwd_strided", dev, f.dtype,
+                  [_ptr(u0), _ptr(f), float(h), m, n, l, float(tol), int(verbose),
+                   (ctypes.c_int64 * 3)(*st), _ptr(u)], info)
+    return u
+
+
+def eikonal3d_grad_strided(grad_u, u, u0, f, h):
+    """EikonalThreeDGrad (EikonalThreeD.cpp:154-250) on (m, n, l) arrays of
+    any common layout; returns (grad_u0, grad_f) with the same strides."""
+    st, dev = _layout([grad_u, u, u0, f], 3)
+    m, n, l = (int(x) for x in f.shape)
+    gu0, gf = _empty_like_strided(f, st), _empty_like_strided(f, st)
+    info = L.AdtInfo()
+    _call_strided("adt_eikonal3d_bwd_strided", dev, f.dtype,
+                  [_ptr(grad_u), _ptr(u), _ptr(u0), _ptr(f), float(h), m, n, l,
+                   (ctypes.c_int64 * 3)(*st), _ptr(gu0), _ptr(gf)], info)
+    return gu0, gf
+
+
+def eikonal_strided(f, srcx, srcy, h):
+    """Eikonal (Eikonal.cpp:20-46) on an (n+1, m+1) array (rows = depth j,
+    columns = width i, as src/eikonal_op.jl:3-21 takes it) of any layout;
+    returns u with the same strides."""
+    st, dev = _layout([f], 2)
+    n, m = int(f.shape[0]) - 1, int(f.shape[1]) - 1
+    u = _empty_like_strided(f, st)
+    info = L.AdtInfo()
+    _call_strided("adt_eikonal2d_fwd_strided", dev, f.dtype,
+                  [_ptr(f), m, n, int(srcx), int(srcy), float(h),
+                   (ctypes.c_int64 * 2)(st[1], st[0]), _ptr(u)], info)
+    return u
+
+
+def eikonal_grad_strided(grad_u, u, f, srcx, srcy, h):
+    """EikonalGrad (Eikonal.cpp:48-63) on (n+1, m+1) arrays of any common
+    layout; returns grad_f with the same strides."""
+    st, dev = _layout([grad_u, u, f], 2)
+    n, m = int(f.shape[0]) - 1, int(f.shape[1]) - 1
+    gf = _empty_like_strided(f, st)
+    info = L.AdtInfo()
+    _call_strided("adt_eikonal2d_bwd_strided", dev, f.dtype,
+                  [_ptr(grad_u), _ptr(u), _ptr(f), m, n, int(srcx), int(srcy), float(h),
+                   (ctypes.c_int64 * 2)(st[1], st[0]), _ptr(gf)], info)
+    return gf
+
+
 # -------------------------------------------------------------- batched ---
 
 def _event_table(u0s, fs, hs, tols, grad_us, us, grad_u0s, grad_fs):

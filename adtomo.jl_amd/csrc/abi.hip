@@ -369,6 +369,209 @@ int bwd2_impl(const T* g, const T* u, const T* f, int64_t m, int64_t n, int64_t 
   return ADT_OK;
 }
 
+// -------------------------------------------------------------- strided ---
+//
+// The _strided entry points: every array of a call is gathered on the
+// device into a row-major scratch twin, the row-major path runs on the
+// twins, and the outputs are scattered back.  Host arrays (ADT_HOST_POINTERS)
+// travel as their whole span, so positions between elements keep their
+// values.
+
+// A strided layout as three row-major axes (2D: a unit axis, then j, i).
+struct Layout {
+  int64_t dim[3], st[3];
+  int64_t N, span;  // elements, and the span of addresses they cover
+};
+
+int make_layout(const int64_t (&dim)[3], const int64_t (&st)[3], size_t elem, Layout* L) {
+  int idx[3], na = 0;
+  int64_t span = 1;
+  for (int a = 0; a < 3; ++a) {
+    L->dim[a] = dim[a];
+    L->st[a] = dim[a] > 1 ? st[a] : 0;  // an axis of one element has no stride
+    if (dim[a] <= 1) continue;
+    if (st[a] <= 0) return fail(ADT_EINVAL, "strides must be positive");
+    int64_t ext;
+    if (__builtin_mul_overflow(dim[a] - 1, st[a], &ext) || __builtin_add_overflow(span, ext, &span))
+      return fail(ADT_EINVAL, "strides: span overflows");
+    idx[na++] = a;
+  }
+  int64_t bytes;
+  if (__builtin_mul_overflow(span, (int64_t)elem, &bytes) || bytes > (int64_t(1) << 46))
+    return fail(ADT_EINVAL, "strides: span too large");
+  // no two elements share an address: with the axes in increasing stride,
+  // each stride must reach past the whole extent of the axes below it
+  // (st * dim <= span + st stays far from overflow after the check above)
+  std::sort(idx, idx + na, [&](int x, int y) { return L->st[x] < L->st[y]; });
+  int64_t reach = 1;
+  for (int t = 0; t < na; ++t) {
+    const int a = idx[t];
+    if (L->st[a] < reach) return fail(ADT_EINVAL, "strides map two elements to one address");
+    reach = L->st[a] * L->dim[a];
+  }
+  L->N = dim[0] * dim[1] * dim[2];
+  L->span = span;
+  return ADT_OK;
+}
+
+template <typename T>
+struct StridedIO {
+  const Layout& L;
+  bool hp;
+  hipStream_t s;
+  std::list<DevBuf> bufs;
+  StridedIO(const Layout& lay, bool host_ptrs, hipStream_t st) : L(lay), hp(host_ptrs), s(st) {}
+
+  int scratch(size_t n, T** out) {
+    bufs.emplace_back();
+    int rc = bufs.back().alloc(n * sizeof(T), s);
+    *out = (T*)bufs.back().p;
+    return rc;
+  }
+  void copy(const T* src, T* dst, bool to_rowmajor) {
+    const int64_t b = (L.N + 255) / 256;
+    const int grid = (int)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+    adt::strided_copy_kernel<T><<<grid, 256, 0, s>>>(src, dst, L.N, L.dim[1], L.dim[2], L.st[0],
+                                                       L.st[1], L.st[2], to_rowmajor ? 1 : 0);
+  }
+  // row-major device twin of a strided input
+  int in(const T* a, const T** out) {
+    int rc;
+    const T* src = a;
+    if (hp) {
+      T* span;
+      if ((rc = scratch(L.span, &span))) return rc;
+      ADT_HIP(hipMemcpyAsync(span, a, L.span * sizeof(T), hipMemcpyHostToDevice, s));
+      src = span;
+    }
+    T* twin;
+    if ((rc = scratch(L.N, &twin))) return rc;
+    copy(src, twin, true);
+    ADT_HIP(hipGetLastError());
+    *out = twin;
+    return ADT_OK;
+  }
+  // row-major device buffer for an output (NULL stays NULL)
+  int out_twin(const T* a, T** out) {
+    if (!a) {
+      *out = nullptr;
+      return ADT_OK;
+    }
+    return scratch(L.N, out);
+  }
+  // scatters a row-major result into the caller's strided output
+  int out(const T* twin, T* a) {
+    if (!a) return ADT_OK;
+    int rc;
+    T* dst = a;
+    if (hp) {
+      if ((rc = scratch(L.span, &dst))) return rc;
+      ADT_HIP(hipMemcpyAsync(dst, a, L.span * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    copy(twin, dst, false);
+    ADT_HIP(hipGetLastError());
+    if (hp) ADT_HIP(hipMemcpyAsync(a, dst, L.span * sizeof(T), hipMemcpyDeviceToHost, s));
+    return ADT_OK;
+  }
+  int finish() {
+    ADT_HIP(hipStreamSynchronize(s));
+    return ADT_OK;
+  }
+};
+
+int layout3(int64_t m, int64_t n, int64_t l, const int64_t* strides, size_t elem, Layout* L) {
+  if (!strides) return fail(ADT_EINVAL, "strides: NULL");
+  return make_layout({m, n, l}, {strides[0], strides[1], strides[2]}, elem, L);
+}
+// 2D: flat j*(m+1) + i is row-major over (j, i)
+int layout2(int64_t m, int64_t n, const int64_t* strides, size_t elem, Layout* L) {
+  if (!strides) return fail(ADT_EINVAL, "strides: NULL");
+  return make_layout({1, n + 1, m + 1}, {0, strides[1], strides[0]}, elem, L);
+}
+
+template <typename T>
+int fwd3_strided_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T tol,
+                      int verbose, const int64_t* strides, T* u, adt_info* info, int flags,
+                      void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!u0 || !f || !u) return fail(ADT_EINVAL, "eikonal3d: NULL buffer");
+  Layout L;
+  if ((rc = layout3(m, n, l, strides, sizeof(T), &L))) return rc;
+  StridedIO<T> io(L, flags & ADT_HOST_POINTERS, (hipStream_t)stream);
+  const T *r_u0, *r_f;
+  T* r_u;
+  if ((rc = io.in(u0, &r_u0)) || (rc = io.in(f, &r_f)) || (rc = io.out_twin(u, &r_u))) return rc;
+  rc = fwd3_impl<T>(r_u0, r_f, h, m, n, l, tol, verbose, r_u, info, flags & ~ADT_HOST_POINTERS,
+                    stream);
+  if (rc) return rc;
+  if ((rc = io.out(r_u, u))) return rc;
+  return io.finish();
+}
+
+template <typename T>
+int bwd3_strided_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m,
+                      int64_t n, int64_t l, const int64_t* strides, T* grad_u0, T* grad_f,
+                      adt_info* info, int flags, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!g || !u || !u0 || !f || !grad_f) return fail(ADT_EINVAL, "eikonal3d grad: NULL buffer");
+  Layout L;
+  if ((rc = layout3(m, n, l, strides, sizeof(T), &L))) return rc;
+  StridedIO<T> io(L, flags & ADT_HOST_POINTERS, (hipStream_t)stream);
+  const T *r_g, *r_u, *r_u0, *r_f;
+  T *r_gu0, *r_gf;
+  if ((rc = io.in(g, &r_g)) || (rc = io.in(u, &r_u)) || (rc = io.in(u0, &r_u0)) ||
+      (rc = io.in(f, &r_f)) || (rc = io.out_twin(grad_u0, &r_gu0)) ||
+      (rc = io.out_twin(grad_f, &r_gf)))
+    return rc;
+  rc = bwd3_impl<T>(r_g, r_u, r_u0, r_f, h, m, n, l, r_gu0, r_gf, info,
+                    flags & ~ADT_HOST_POINTERS, stream);
+  if (rc) return rc;
+  if ((rc = io.out(r_gu0, grad_u0)) || (rc = io.out(r_gf, grad_f))) return rc;
+  return io.finish();
+}
+
+template <typename T>
+int fwd2_strided_impl(const T* f, int64_t m, int64_t n, int64_t srcx, int64_t srcy, T h,
+                      const int64_t* strides, T* u, adt_info* info, int flags, void* stream) {
+  int rc = check_dims2(m, n, srcx, srcy);
+  if (rc) return rc;
+  if (!f || !u) return fail(ADT_EINVAL, "eikonal2d: NULL buffer");
+  Layout L;
+  if ((rc = layout2(m, n, strides, sizeof(T), &L))) return rc;
+  StridedIO<T> io(L, flags & ADT_HOST_POINTERS, (hipStream_t)stream);
+  const T* r_f;
+  T* r_u;
+  if ((rc = io.in(f, &r_f)) || (rc = io.out_twin(u, &r_u))) return rc;
+  rc = fwd2_impl<T>(r_f, m, n, srcx, srcy, h, r_u, info, flags & ~ADT_HOST_POINTERS, stream);
+  if (rc) return rc;
+  if ((rc = io.out(r_u, u))) return rc;
+  return io.finish();
+}
+
+template <typename T>
+int bwd2_strided_impl(const T* g, const T* u, const T* f, int64_t m, int64_t n, int64_t srcx,
+                      int64_t srcy, T h, const int64_t* strides, T* grad_f, adt_info* info,
+                      int flags, void* stream) {
+  int rc = check_dims2(m, n, srcx, srcy);
+  if (rc) return rc;
+  if (!g || !u || !f || !grad_f) return fail(ADT_EINVAL, "eikonal2d grad: NULL buffer");
+  Layout L;
+  if ((rc = layout2(m, n, strides, sizeof(T), &L))) return rc;
+  StridedIO<T> io(L, flags & ADT_HOST_POINTERS, (hipStream_t)stream);
+  const T *r_g, *r_u, *r_f;
+  T* r_gf;
+  if ((rc = io.in(g, &r_g)) || (rc = io.in(u, &r_u)) || (rc = io.in(f, &r_f)) ||
+      (rc = io.out_twin(grad_f, &r_gf)))
+    return rc;
+  rc = bwd2_impl<T>(r_g, r_u, r_f, m, n, srcx, srcy, h, r_gf, info, flags & ~ADT_HOST_POINTERS,
+                    stream);
+  if (rc) return rc;
+  if ((rc = io.out(r_gf, grad_f))) return rc;
+  return io.finish();
+}
+
 // -------------------------------------------------------------- batched ---
 
 // Builds the device descriptor table and workspace for `nev` events.
@@ -877,6 +1080,58 @@ int adt_eikonal3d_bwd_f32(const float* grad_u, const float* u, const float* u0, 
                           float h, int64_t m, int64_t n, int64_t l, float* grad_u0,
                           float* grad_f, adt_info* info, int32_t flags, void* stream) {
   return bwd3_impl<float>(grad_u, u, u0, f, h, m, n, l, grad_u0, grad_f, info, flags, stream);
+}
+int adt_eikonal2d_fwd_strided_f64(const double* f, int64_t m, int64_t n, int64_t srcx,
+                                  int64_t srcy, double h, const int64_t strides[2], double* u,
+                                  adt_info* info, int32_t flags, void* stream) {
+  return fwd2_strided_impl<double>(f, m, n, srcx, srcy, h, strides, u, info, flags, stream);
+}
+int adt_eikonal2d_fwd_strided_f32(const float* f, int64_t m, int64_t n, int64_t srcx,
+                                  int64_t srcy, float h, const int64_t strides[2], float* u,
+                                  adt_info* info, int32_t flags, void* stream) {
+  return fwd2_strided_impl<float>(f, m, n, srcx, srcy, h, strides, u, info, flags, stream);
+}
+int adt_eikonal2d_bwd_strided_f64(const double* grad_u, const double* u, const double* f,
+                                  int64_t m, int64_t n, int64_t srcx, int64_t srcy, double h,
+                                  const int64_t strides[2], double* grad_f, adt_info* info,
+                                  int32_t flags, void* stream) {
+  return bwd2_strided_impl<double>(grad_u, u, f, m, n, srcx, srcy, h, strides, grad_f, info,
+                                   flags, stream);
+}
+int adt_eikonal2d_bwd_strided_f32(const float* grad_u, const float* u, const float* f,
+                                  int64_t m, int64_t n, int64_t srcx, int64_t srcy, float h,
+                                  const int64_t strides[2], float* grad_f, adt_info* info,
+                                  int32_t flags, void* stream) {
+  return bwd2_strided_impl<float>(grad_u, u, f, m, n, srcx, srcy, h, strides, grad_f, info,
+                                  flags, stream);
+}
+int adt_eikonal3d_fwd_strided_f64(const double* u0, const double* f, double h, int64_t m,
+                                  int64_t n, int64_t l, double tol, int32_t verbose,
+                                  const int64_t strides[3], double* u, adt_info* info,
+                                  int32_t flags, void* stream) {
+  return fwd3_strided_impl<double>(u0, f, h, m, n, l, tol, verbose, strides, u, info, flags,
+                                   stream);
+}
+int adt_eikonal3d_fwd_strided_f32(const float* u0, const float* f, float h, int64_t m,
+                                  int64_t n, int64_t l, float tol, int32_t verbose,
+                                  const int64_t strides[3], float* u, adt_info* info,
+                                  int32_t flags, void* stream) {
+  return fwd3_strided_impl<float>(u0, f, h, m, n, l, tol, verbose, strides, u, info, flags,
+                                  stream);
+}
+int adt_eikonal3d_bwd_strided_f64(const double* grad_u, const double* u, const double* u0,
+                                  const double* f, double h, int64_t m, int64_t n, int64_t l,
+                                  const int64_t strides[3], double* grad_u0, double* grad_f,
+                                  adt_info* info, int32_t flags, void* stream) {
+  return bwd3_strided_impl<double>(grad_u, u, u0, f, h, m, n, l, strides, grad_u0, grad_f, info,
+                                   flags, stream);
+}
+int adt_eikonal3d_bwd_strided_f32(const float* grad_u, const float* u, const float* u0,
+                                  const float* f, float h, int64_t m, int64_t n, int64_t l,
+                                  const int64_t strides[3], float* grad_u0, float* grad_f,
+                                  adt_info* info, int32_t flags, void* stream) {
+  return bwd3_strided_impl<float>(grad_u, u, u0, f, h, m, n, l, strides, grad_u0, grad_f, info,
+                                  flags, stream);
 }
 int adt_eikonal3d_batched_f64(const adt_event3d* events, int32_t nev, int32_t flags,
                               adt_info* infos, void* stream) {

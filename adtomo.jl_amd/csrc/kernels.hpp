@@ -94,6 +94,9 @@ template <typename T, bool MIX>
 __global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
 __global__ void info_clear_kernel(adt_info* info, int n);
 template <typename T>
+__global__ void strided_copy_kernel(const T* src, T* dst, int64_t N, int64_t d1, int64_t d2,
+                                    int64_t s0, int64_t s1, int64_t s2, int to_rowmajor);
+template <typename T>
 __global__ void copy_kernel(T* dst, const T* src, int64_t N);
 template <typename T>
 __global__ void maxdiff_kernel(const T* a, const T* b, int64_t N, unsigned long long* out);

@@ -22,6 +22,31 @@ __global__ void info_clear_kernel(adt_info* __restrict__ info, int n) {
 }
 
 
+// Copies between a strided array and its row-major twin: element p of the
+// row-major (d0, d1, d2) array is (i, j, k) = (p / (d1 d2), p / d2 mod d1,
+// p mod d2) and sits at i*s0 + j*s1 + k*s2 of the strided one.  to_rowmajor
+// gathers (dst[p] = src[offset]), else it scatters (dst[offset] = src[p]).
+template <typename T>
+__global__ void strided_copy_kernel(const T* __restrict__ src, T* __restrict__ dst, int64_t N,
+                                    int64_t d1, int64_t d2, int64_t s0, int64_t s1, int64_t s2,
+                                    int to_rowmajor) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = p / d2, k = p - q * d2;
+    const int64_t i = q / d1, j = q - i * d1;
+    const int64_t off = i * s0 + j * s1 + k * s2;
+    if (to_rowmajor)
+      dst[p] = src[off];
+    else
+      dst[off] = src[p];
+  }
+}
+template __global__ void strided_copy_kernel<double>(const double*, double*, int64_t, int64_t,
+                                                     int64_t, int64_t, int64_t, int64_t, int);
+template __global__ void strided_copy_kernel<float>(const float*, float*, int64_t, int64_t,
+                                                    int64_t, int64_t, int64_t, int64_t, int);
+
+
 // ------------------------------------------------------------------ 3D ---
 
 template <typename T>

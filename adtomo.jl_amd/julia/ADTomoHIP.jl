@@ -9,11 +9,12 @@
 # (deps/CustomOps/Eikonal3D/EikonalThreeD.cpp:47-65).
 #
 # The single-event ops take plain Julia `Array{Float64}` (host memory, staged
-# by the library with ADT_HOST_POINTERS); the batched, graph and objective
-# calls below take device arrays.  The reference's TF tensors are row-major, so the
-# 3D op sees flat index i*n*l + j*l + k: a Julia (m,n,l) array is passed as
-# permutedims(A, (3,2,1)) and results are permuted back; the 2D (n+1, m+1)
-# matrix likewise goes through permutedims(f) (flat j*(m+1)+i).
+# by the library with ADT_HOST_POINTERS) through the library's _strided entry
+# points: element (i, j, k) of a column-major (m, n, l) Array sits at
+# strides (1, m, m*n), and (row j, column i) of an (n+1, m+1) Matrix at
+# (i: n+1, j: 1), so arrays pass as they are, without the transposed copies
+# of the reference's row-major tensors.  The batched, graph and objective
+# calls below take device arrays in the reference's flat layout.
 module ADTomoHIP
 
 using ChainRulesCore
@@ -46,11 +47,12 @@ function check(rc::Integer, what::AbstractString)
     end
 end
 
-# Julia column-major <-> reference row-major flat layout.
-rowmajor(A::AbstractArray{Float64,3}) = permutedims(A, (3, 2, 1))
-rowmajor(A::AbstractMatrix{Float64}) = permutedims(A, (2, 1))
-colmajor3(B::Array{Float64,3}) = permutedims(B, (3, 2, 1))
-colmajor2(B::Matrix{Float64}) = permutedims(B, (2, 1))
+# Element strides of column-major arrays, in the library's axis order.
+strides3(m, n) = Int64[1, m, m * n]                 # (i, j, k) of an (m, n, l) Array
+strides2(n) = Int64[n + 1, 1]                       # (i, j) of an (n+1, m+1) Matrix
+dense3(A, m, n, l) = A isa Array{Float64,3} && size(A) == (m, n, l) ? A :
+                     Array{Float64,3}(reshape(A, m, n, l))
+dense2(A) = A isa Matrix{Float64} ? A : Matrix{Float64}(A)
 
 # ------------------------------------------------------------------ 2D ---
 
@@ -63,31 +65,30 @@ column `srcx` and row `srcy`, spacing `h` (src/eikonal_op.jl:3-21).
 function eikonal(f::Matrix{Float64}, srcx::Int64, srcy::Int64, h::Float64)
     n_, m_ = size(f)
     m, n = m_ - 1, n_ - 1
-    fr = rowmajor(f)                      # flat j*(m+1)+i
-    ur = similar(fr)
+    u = similar(f)
     info = Ref(AdtInfo())
-    rc = ccall((:adt_eikonal2d_fwd_f64, libadt), Cint,
-               (Ptr{Float64}, Int64, Int64, Int64, Int64, Float64, Ptr{Float64},
+    rc = ccall((:adt_eikonal2d_fwd_strided_f64, libadt), Cint,
+               (Ptr{Float64}, Int64, Int64, Int64, Int64, Float64, Ptr{Int64}, Ptr{Float64},
                 Ref{AdtInfo}, Int32, Ptr{Cvoid}),
-               fr, m, n, srcx, srcy, h, ur, info, ADT_HOST_POINTERS, C_NULL)
+               f, m, n, srcx, srcy, h, strides2(n), u, info, ADT_HOST_POINTERS, C_NULL)
     check(rc, "eikonal")
     info[].converged == 0 && @warn "Eikonal does not converge!"
-    return colmajor2(ur)
+    return u
 end
 
 function eikonal_grad(grad_u::Matrix{Float64}, u::Matrix{Float64}, f::Matrix{Float64},
                       srcx::Int64, srcy::Int64, h::Float64)
     n_, m_ = size(f)
     m, n = m_ - 1, n_ - 1
-    gr, ur, fr = rowmajor(grad_u), rowmajor(u), rowmajor(f)
-    gfr = similar(fr)
+    g = dense2(grad_u)
+    gf = similar(f)
     info = Ref(AdtInfo())
-    rc = ccall((:adt_eikonal2d_bwd_f64, libadt), Cint,
+    rc = ccall((:adt_eikonal2d_bwd_strided_f64, libadt), Cint,
                (Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Int64, Int64, Float64,
-                Ptr{Float64}, Ref{AdtInfo}, Int32, Ptr{Cvoid}),
-               gr, ur, fr, m, n, srcx, srcy, h, gfr, info, ADT_HOST_POINTERS, C_NULL)
+                Ptr{Int64}, Ptr{Float64}, Ref{AdtInfo}, Int32, Ptr{Cvoid}),
+               g, u, f, m, n, srcx, srcy, h, strides2(n), gf, info, ADT_HOST_POINTERS, C_NULL)
     check(rc, "eikonal_grad")
-    return colmajor2(gfr)
+    return gf
 end
 
 function ChainRulesCore.rrule(::typeof(eikonal), f::Matrix{Float64}, srcx::Int64, srcy::Int64,
@@ -111,29 +112,30 @@ m x n x l (src/eikonal_op.jl:24-32).  Returns an m x n x l array.
 """
 function eikonal3d(u0::AbstractArray{Float64}, f::AbstractArray{Float64}, h::Float64,
                    m::Integer, n::Integer, l::Integer, tol::Float64, verbose::Bool)
-    u0r = rowmajor(reshape(u0, m, n, l))
-    fr = rowmajor(reshape(f, m, n, l))
-    ur = similar(fr)
+    u0a, fa = dense3(u0, m, n, l), dense3(f, m, n, l)
+    u = similar(fa)
     info = Ref(AdtInfo())
-    rc = ccall((:adt_eikonal3d_fwd_f64, libadt), Cint,
+    rc = ccall((:adt_eikonal3d_fwd_strided_f64, libadt), Cint,
                (Ptr{Float64}, Ptr{Float64}, Float64, Int64, Int64, Int64, Float64, Int32,
-                Ptr{Float64}, Ref{AdtInfo}, Int32, Ptr{Cvoid}),
-               u0r, fr, h, m, n, l, tol, Int32(verbose), ur, info, ADT_HOST_POINTERS, C_NULL)
+                Ptr{Int64}, Ptr{Float64}, Ref{AdtInfo}, Int32, Ptr{Cvoid}),
+               u0a, fa, h, m, n, l, tol, Int32(verbose), strides3(m, n), u, info,
+               ADT_HOST_POINTERS, C_NULL)
     check(rc, "eikonal3d")
-    return colmajor3(ur)
+    return u
 end
 
 function eikonal3d_grad(grad_u, u, u0, f, h::Float64, m::Integer, n::Integer, l::Integer)
-    gr = rowmajor(reshape(Array{Float64}(grad_u), m, n, l))
-    ur, u0r, fr = rowmajor(u), rowmajor(reshape(u0, m, n, l)), rowmajor(reshape(f, m, n, l))
-    gu0r, gfr = similar(fr), similar(fr)
+    g, ua = dense3(grad_u, m, n, l), dense3(u, m, n, l)
+    u0a, fa = dense3(u0, m, n, l), dense3(f, m, n, l)
+    gu0, gf = similar(fa), similar(fa)
     info = Ref(AdtInfo())
-    rc = ccall((:adt_eikonal3d_bwd_f64, libadt), Cint,
+    rc = ccall((:adt_eikonal3d_bwd_strided_f64, libadt), Cint,
                (Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Int64, Int64,
-                Int64, Ptr{Float64}, Ptr{Float64}, Ref{AdtInfo}, Int32, Ptr{Cvoid}),
-               gr, ur, u0r, fr, h, m, n, l, gu0r, gfr, info, ADT_HOST_POINTERS, C_NULL)
+                Int64, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ref{AdtInfo}, Int32, Ptr{Cvoid}),
+               g, ua, u0a, fa, h, m, n, l, strides3(m, n), gu0, gf, info, ADT_HOST_POINTERS,
+               C_NULL)
     check(rc, "eikonal3d_grad")
-    return colmajor3(gu0r), colmajor3(gfr)
+    return gu0, gf
 end
 
 function ChainRulesCore.rrule(::typeof(eikonal3d), u0, f, h::Float64, m::Integer, n::Integer,

@@ -14,9 +14,8 @@
  *   - Layout is the reference's: 3D fields are row-major (m, n, l) with
  *     flat index i*n*l + j*l + k (Eikonal3D.cpp:6-8); 2D fields have
  *     (n+1) rows (depth j) of (m+1) columns (width i), flat j*(m+1) + i
- *     (Eikonal.h:22-52).  A Julia column-major (m,n,l) Array must be passed
- *     as permutedims(A, (3,2,1)), which is what ADCME's tensor conversion
- *     does for the reference op.
+ *     (Eikonal.h:22-52).  The _strided twins of the single-event ops take
+ *     any per-axis strides instead (a Julia column-major Array as it is).
  *   - Sources are 1-based (srcx = column i+1, srcy = row j+1), like the
  *     reference op's inputs (Eikonal.cpp:125 subtracts 1).
  *   - Return value: ADT_OK (0) or a negative ADT_E* code; adt_last_error()
@@ -124,6 +123,50 @@ int adt_eikonal3d_bwd_f64(const double *grad_u, const double *u, const double *u
 int adt_eikonal3d_bwd_f32(const float *grad_u, const float *u, const float *u0, const float *f,
                           float h, int64_t m, int64_t n, int64_t l, float *grad_u0,
                           float *grad_f, adt_info *info, int32_t flags, void *stream);
+
+/* ---- Strided layouts of the four single-event ops above (the per-axis
+ * strides of the boundary that src/eikonal_op.jl:3-32's Julia callers need
+ * to pass arrays without a transposed copy).  Every array of a call holds
+ * element (i, j, k) at offset i*strides[0] + j*strides[1] + k*strides[2]
+ * elements; 2D arrays hold (i, j) (column i, row j) at i*strides[0] +
+ * j*strides[1].  A Julia column-major (m, n, l) Array passes
+ * strides {1, m, m*n}; a Julia (n+1, m+1) Matrix passes {n+1, 1}; the
+ * row-major layout of the calls above is {n*l, l, 1} / {1, m+1}.  Strides
+ * of axes longer than one must be positive and must not map two elements
+ * to one address (ADT_EINVAL otherwise).  Positions between elements (a
+ * padded layout) are neither read nor written.  The library gathers the
+ * arrays to the row-major layout on the device and scatters the results
+ * back, so the results equal those of the row-major calls bit for bit. */
+int adt_eikonal2d_fwd_strided_f64(const double *f, int64_t m, int64_t n, int64_t srcx,
+                                  int64_t srcy, double h, const int64_t strides[2], double *u,
+                                  adt_info *info, int32_t flags, void *stream);
+int adt_eikonal2d_fwd_strided_f32(const float *f, int64_t m, int64_t n, int64_t srcx,
+                                  int64_t srcy, float h, const int64_t strides[2], float *u,
+                                  adt_info *info, int32_t flags, void *stream);
+int adt_eikonal2d_bwd_strided_f64(const double *grad_u, const double *u, const double *f,
+                                  int64_t m, int64_t n, int64_t srcx, int64_t srcy, double h,
+                                  const int64_t strides[2], double *grad_f, adt_info *info,
+                                  int32_t flags, void *stream);
+int adt_eikonal2d_bwd_strided_f32(const float *grad_u, const float *u, const float *f,
+                                  int64_t m, int64_t n, int64_t srcx, int64_t srcy, float h,
+                                  const int64_t strides[2], float *grad_f, adt_info *info,
+                                  int32_t flags, void *stream);
+int adt_eikonal3d_fwd_strided_f64(const double *u0, const double *f, double h, int64_t m,
+                                  int64_t n, int64_t l, double tol, int32_t verbose,
+                                  const int64_t strides[3], double *u, adt_info *info,
+                                  int32_t flags, void *stream);
+int adt_eikonal3d_fwd_strided_f32(const float *u0, const float *f, float h, int64_t m,
+                                  int64_t n, int64_t l, float tol, int32_t verbose,
+                                  const int64_t strides[3], float *u, adt_info *info,
+                                  int32_t flags, void *stream);
+int adt_eikonal3d_bwd_strided_f64(const double *grad_u, const double *u, const double *u0,
+                                  const double *f, double h, int64_t m, int64_t n, int64_t l,
+                                  const int64_t strides[3], double *grad_u0, double *grad_f,
+                                  adt_info *info, int32_t flags, void *stream);
+int adt_eikonal3d_bwd_strided_f32(const float *grad_u, const float *u, const float *u0,
+                                  const float *f, float h, int64_t m, int64_t n, int64_t l,
+                                  const int64_t strides[3], float *grad_u0, float *grad_f,
+                                  adt_info *info, int32_t flags, void *stream);
 
 /* ---- Batched 3D: many independent events (the per-station op loop of
  * scripts/inversion.jl:47-62, sharded over MPI ranks by

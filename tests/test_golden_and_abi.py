@@ -70,6 +70,55 @@ def test_abi_argument_errors_without_device():
     assert rc == -1 and b"outside" in lib.adt_last_error()
 
 
+def test_strided_argument_errors_without_device():
+    # Stride validation happens before any device call: non-positive strides,
+    # strides that map two elements to one address, and NULL strides.
+    from adtomo_hip import _lib as L
+    lib = L.lib()
+    info = L.AdtInfo()
+    buf = np.zeros(64)
+    p = buf.ctypes.data
+    S3, S2 = ctypes.c_int64 * 3, ctypes.c_int64 * 2
+    for st, msg in (((0, 4, 1), b"positive"), ((-8, 4, 1), b"positive"),
+                    ((4, 4, 1), b"one address"), ((8, 4, 2), b"one address")):
+        rc = lib.adt_eikonal3d_fwd_strided_f64(p, p, 1.0, 2, 2, 4, 1e-3, 0, S3(*st), p,
+                                               ctypes.byref(info), 0, None)
+        assert rc == -1 and msg in lib.adt_last_error(), (st, lib.adt_last_error())
+    # the unit axis of a 2D call and an axis of extent 1 carry no stride
+    rc = lib.adt_eikonal3d_fwd_strided_f64(p, p, 1.0, 2, 2, 4, 1e-3, 0, None, p,
+                                           ctypes.byref(info), 0, None)
+    assert rc == -1 and b"NULL" in lib.adt_last_error()
+    rc = lib.adt_eikonal3d_bwd_strided_f64(p, p, p, p, 1.0, 2, 2, 4, S3(1, 1, 4), p, p,
+                                           ctypes.byref(info), 0, None)
+    assert rc == -1 and b"one address" in lib.adt_last_error()
+    rc = lib.adt_eikonal2d_fwd_strided_f64(p, 3, 3, 1, 1, 1.0, S2(1, 2), p, ctypes.byref(info),
+                                           0, None)
+    assert rc == -1 and b"one address" in lib.adt_last_error()
+    rc = lib.adt_eikonal2d_bwd_strided_f32(p, p, p, 3, 3, 9, 1, 1.0, S2(1, 4), p,
+                                           ctypes.byref(info), 0, None)
+    assert rc == -1 and b"outside" in lib.adt_last_error()
+
+
+def test_strided_layout_checks_in_python():
+    from adtomo_hip import ops
+    a = np.zeros((4, 5, 6))
+    assert ops._layout([a, a.copy()], 3) == ((30, 6, 1), False)
+    fa = np.asfortranarray(a)
+    assert ops._layout([fa], 3) == ((1, 4, 20), False)
+    with pytest.raises(ValueError, match="share their strides"):
+        ops._layout([a, fa], 3)
+    with pytest.raises(ValueError, match="positive"):
+        ops._layout([a[::-1]], 3)
+    with pytest.raises(ValueError, match="3-D"):
+        ops._layout([a[0]], 3)
+    b = ops._empty_like_strided(fa, (1, 4, 20))
+    assert b.shape == fa.shape and b.flags.f_contiguous
+    big = np.zeros((7, 8, 9))
+    v = big[:4, :5, :6]
+    st, _ = ops._layout([v], 3)
+    assert st == (72, 9, 1) and ops._empty_like_strided(v, st).strides == v.strides
+
+
 def test_product_package_does_not_import_oracle():
     import glob
     pkg = os.path.join(os.path.dirname(GOLD), "..", "adtomo.jl_amd")

@@ -278,6 +278,128 @@ def test_host_pointer_mode_numpy():
     assert rel(gf, d["grad_f"]) <= GRAD_RTOL
 
 
+# ------------------------------------------------------ strided layouts ---
+
+def _perm(a, dtype=None):
+    """Device tensor with the Julia column-major layout: strides (1, m, m*n)."""
+    return dev(np.ascontiguousarray(a.transpose(2, 1, 0)), dtype).permute(2, 1, 0)
+
+
+def test_strided_3d_layouts_match_rowmajor():
+    import adtomo_hip as A
+    shape = (20, 24, 18)
+    r = synth.rng(31)
+    f = r.uniform(0.3, 3.0, size=shape)
+    u0 = synth.point_source_u0(shape, (5, 7, 9))
+    g = r.normal(size=shape)
+    u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, 1e-6)
+    ud = A.eikonal3d(dev(u0), dev(f), 0.5, *shape, 1e-6, False)
+    assert bits_equal(host(ud), u_ref)
+    gu0d, gfd = (host(a) for a in A.eikonal3d_grad(dev(g), ud, dev(u0), dev(f), 0.5, *shape))
+    # host arrays in the Julia column-major layout
+    F = np.asfortranarray
+    u = A.eikonal3d_strided(F(u0), F(f), 0.5, 1e-6)
+    assert u.flags.f_contiguous and bits_equal(u, u_ref)
+    assert A.ops.last_info["iterations"] == it
+    gu0, gf = A.eikonal3d_grad_strided(F(g), F(u_ref), F(u0), F(f), 0.5)
+    assert gf.flags.f_contiguous and bits_equal(gu0, gu0d) and bits_equal(gf, gfd)
+    # device tensors with the same layout (permuted views)
+    ut = A.eikonal3d_strided(_perm(u0), _perm(f), 0.5, 1e-6)
+    assert ut.stride() == (1, shape[0], shape[0] * shape[1])
+    assert bits_equal(host(ut), u_ref)
+    gu0t, gft = A.eikonal3d_grad_strided(_perm(g), ut, _perm(u0), _perm(f), 0.5)
+    assert bits_equal(host(gu0t), gu0d) and bits_equal(host(gft), gfd)
+
+
+@pytest.mark.parametrize("host_ptrs", [False, True])
+def test_strided_padded_layout_leaves_gaps(host_ptrs):
+    # every array of the call is a (9, 11, 13) corner of a (12, 13, 18)
+    # buffer: results land in the corner, the rest of the output buffer keeps
+    # its values
+    import ctypes
+    import adtomo_hip as A
+    from adtomo_hip import _lib as L
+    torch = _torch()
+    shape, big = (9, 11, 13), (12, 13, 18)
+    r = synth.rng(32)
+    f = r.uniform(0.5, 2.0, size=shape)
+    u0 = synth.point_source_u0(shape, (4, 5, 6))
+    g = r.normal(size=shape)
+    u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, 1e-6)
+    gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(g, u_ref, u0, f, 0.5)
+    corner = (slice(0, shape[0]), slice(0, shape[1]), slice(0, shape[2]))
+
+    def embed(a):
+        b = np.full(big, -7.0)
+        b[corner] = a
+        return b
+
+    bufs = [embed(u0), embed(f), np.full(big, -7.0), embed(g), embed(u_ref),
+            np.full(big, -7.0), np.full(big, -7.0)]
+    st = (ctypes.c_int64 * 3)(big[1] * big[2], big[2], 1)
+    lib, info = L.lib(), L.AdtInfo()
+    if host_ptrs:
+        ptr = [b.ctypes.data for b in bufs]
+        flags, stream = L.ADT_HOST_POINTERS, None
+    else:
+        tb = [dev(b) for b in bufs]
+        ptr = [t.data_ptr() for t in tb]
+        flags, stream = 0, A.ops._stream()
+    rc = lib.adt_eikonal3d_fwd_strided_f64(ptr[0], ptr[1], 0.5, *shape, 1e-6, 0, st, ptr[2],
+                                           ctypes.byref(info), flags, stream)
+    L.check(rc, "eikonal3d strided")
+    assert info.iterations == it
+    rc = lib.adt_eikonal3d_bwd_strided_f64(ptr[3], ptr[4], ptr[0], ptr[1], 0.5, *shape, st,
+                                           ptr[5], ptr[6], ctypes.byref(info), flags, stream)
+    L.check(rc, "eikonal3d grad strided")
+    if not host_ptrs:
+        torch.cuda.synchronize()
+        bufs = [host(t) for t in tb]
+    gap = np.ones(big, bool)
+    gap[corner] = False
+    for k in (2, 5, 6):
+        assert (bufs[k][gap] == -7.0).all()
+    assert bits_equal(np.ascontiguousarray(bufs[2][corner]), u_ref)
+    assert bits_equal(np.ascontiguousarray(bufs[5][corner]), gu0_ref)
+    assert rel(bufs[6][corner], gf_ref) <= GRAD_RTOL
+
+
+def test_strided_2d_column_major_matches_rowmajor():
+    import adtomo_hip as A
+    r = synth.rng(33)
+    m, n = 30, 20
+    f = r.uniform(0.5, 2.0, size=(n + 1, m + 1))
+    srcx, srcy = 7, 5
+    u_ref, it, err, conv = oracle.eikonal2d(f, srcx, srcy, 0.3)
+    F = np.asfortranarray
+    u = A.eikonal_strided(F(f), srcx, srcy, 0.3)
+    assert u.flags.f_contiguous and bits_equal(u, u_ref)
+    g = r.normal(size=f.shape)
+    gf_row = A.eikonal_grad(g, u_ref, f, srcx, srcy, 0.3)
+    gf = A.eikonal_grad_strided(F(g), F(u_ref), F(f), srcx, srcy, 0.3)
+    assert bits_equal(gf, gf_row)
+    # device: the transposed view of an (m+1, n+1) tensor
+    ut = A.eikonal_strided(dev(np.ascontiguousarray(f.T)).t(), srcx, srcy, 0.3)
+    assert ut.stride() == (1, n + 1) and bits_equal(host(ut), u_ref)
+
+
+def test_strided_f32_matches_rowmajor_f32():
+    import adtomo_hip as A
+    shape = (16, 12, 10)
+    r = synth.rng(34)
+    f = r.uniform(0.5, 2.0, size=shape)
+    u0 = synth.point_source_u0(shape, (3, 4, 5))
+    g = r.normal(size=shape)
+    u32 = A.eikonal3d(dev(u0, np.float32), dev(f, np.float32), 0.5, *shape, 1e-4, False)
+    us = A.eikonal3d_strided(_perm(u0, np.float32), _perm(f, np.float32), 0.5, 1e-4)
+    assert bits_equal(host(us), host(u32))
+    ga, gb = A.eikonal3d_grad(dev(g, np.float32), u32, dev(u0, np.float32), dev(f, np.float32),
+                              0.5, *shape)
+    gc, gd = A.eikonal3d_grad_strided(_perm(g, np.float32), us, _perm(u0, np.float32),
+                                      _perm(f, np.float32), 0.5)
+    assert bits_equal(host(gc), host(ga)) and bits_equal(host(gd), host(gb))
+
+
 # --------------------------------------------------------------- 2D ---
 
 @pytest.mark.parametrize("m,n,seed", [(1, 1, 0), (12, 9, 1), (60, 30, 2), (100, 100, 3)])

@@ -11,7 +11,9 @@ achieved bandwidth of both kernels against the HBM roofline (algorithmic
 bytes: forward 8*K*N*(s_f + 2*s_u) per event, adjoint
 N*(s_u+s_u0+s_f+s_g+s_gradf+s_gradu0)), "roofline" being the kernel with the
 longer launch, and the CPU oracle timed on a bounded sample of the same
-events on the host cores.
+events on the host cores, with, on one 64x64x32 event, the oracle's
+triangular adjoint solve against the reference's sparse-LU route (SciPy
+SuperLU standing in for Eigen's SparseLU).
 
 Other workloads (``--workload``), each printing its own JSON line:
   uneven  256 events on boxes drawn from 48x48x24 .. 128x128x64 (FP64 and
@@ -48,6 +50,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _cpu_adjoint_solvers(oracle, synth, shape=(64, 64, 32)):
+    """The two CPU adjoint routes, timed single-threaded on one event of the
+    64x64x32 grid (the largest where a sparse LU finishes in seconds): the
+    oracle's exact triangular solve, and the reference's route (assemble the
+    upwind system, Eikonal3D.cpp:118-187, and solve it with a sparse LU,
+    :186-193), here SciPy's SuperLU standing in for Eigen's SparseLU."""
+    import numpy as np
+    f = 1.0 / synth.layered_grf_model(shape, 0.5, seed=1)
+    u0 = synth.point_source_u0(shape, synth.interior_source(shape, seed=2))
+    u, _, _ = oracle.eikonal3d(u0, f, 0.5, 1e-3)
+    g = synth.sparse_adjoint_source(shape, 60, seed=3)
+    t = time.perf_counter()
+    _, gf_tri, _ = oracle.eikonal3d_grad(g, u, u0, f, 0.5)
+    t_tri = time.perf_counter() - t
+    t = time.perf_counter()
+    _, gf_lu, _ = oracle.lu_grad_3d(g, u, u0, f, 0.5)
+    t_lu = time.perf_counter() - t
+    return {"grid": list(shape), "triangular_s": t_tri, "sparse_lu_s": t_lu,
+            "sparse_lu": "scipy SuperLU (stand-in for Eigen SparseLU)",
+            "max_rel_diff": float(np.abs(gf_lu - gf_tri).max() / np.abs(gf_tri).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +93,8 @@ def main():
     ap.add_argument("--cpu-events", type=int, default=0, help="0 = 20 per host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-superlu", action="store_true",
+                    help="skip timing the sparse-LU adjoint route on the host")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 variant")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch setting instead of timing the candidates")
@@ -233,8 +270,14 @@ def main():
         tc = time.perf_counter() - tc
         cpu = {"value": ce / tc, "unit": "event-solves/s", "cores": threads, "kind": "port",
                "sample": f"{ce} events of the same workload (128x128x64, FP64 fwd+adjoint), "
-                         f"one event per thread, {tc:.1f}s"}
+                         f"one event per thread, {tc:.1f}s",
+               "host_cpus": os.cpu_count(), "cpu_model": _cpu_model()}
         log(f"cpu oracle: {ce} events in {tc:.1f}s on {threads} threads -> {cpu['value']:.3f}/s")
+        if not args.no_superlu:
+            cpu["adjoint_solvers"] = _cpu_adjoint_solvers(oracle, synth)
+            log(f"cpu adjoint on {cpu['adjoint_solvers']['grid']}: "
+                f"{cpu['adjoint_solvers']['triangular_s']:.3f}s triangular solve, "
+                f"{cpu['adjoint_solvers']['sparse_lu_s']:.2f}s sparse LU")
 
     def roof(kernel, gbs, kms):
         return {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
@@ -258,6 +301,7 @@ def main():
         "config": {"workload": "256 point-source events, 128x128x64, h=0.5 km, layered "
                                "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
                    "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
+                   "serial_depth_per_sweep": int(sum(shape) - 2),
                    "adj_sweeps_mean": float(sweeps.mean()),
                    "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order")}},
         "roofline": roofs[0] if fwd_ms >= adj_ms else roofs[1],

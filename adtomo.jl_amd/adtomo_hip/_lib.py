@@ -77,7 +77,7 @@ class AdtMisfit3d(ctypes.Structure):
 
 
 class AdtTune(ctypes.Structure):
-    _fields_ = [("diag_max_l", _i32), ("adj_wg", _i32), ("order", _i32),
+    _fields_ = [("diag_max_l", _i32), ("adj_wg", _i32), ("order", _i32), ("fuse", _i32),
                 ("fwd_ms", ctypes.c_float), ("adj_ms", ctypes.c_float)]
 
     def as_dict(self):

@@ -67,7 +67,7 @@ struct DevBuf {
 // [1] FP32), read at each launch / graph creation and written by
 // adt_set_tune / adt_autotune under a mutex.
 std::mutex g_tune_mu;
-adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, 0.f, 0.f}, {adt::kDiagAll, 1024, 1, 0.f, 0.f}};
+adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, 1, 0.f, 0.f}, {adt::kDiagAll, 1024, 1, 1, 0.f, 0.f}};
 
 template <typename T>
 constexpr int tune_index() {
@@ -80,7 +80,8 @@ adt_tune current_tune() {
 }
 bool tune_valid(const adt_tune& t) {
   return t.diag_max_l >= 0 && t.diag_max_l <= adt::kDiagAll &&
-         (t.adj_wg == 256 || t.adj_wg == 512 || t.adj_wg == 1024) && (t.order == 0 || t.order == 1);
+         (t.adj_wg == 256 || t.adj_wg == 512 || t.adj_wg == 1024) && (t.order == 0 || t.order == 1) &&
+         (t.fuse == 0 || t.fuse == 1);
 }
 
 int check_dims3(int64_t m, int64_t n, int64_t l) {
@@ -665,6 +666,20 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, con
   if (nev == 0) return ADT_OK;
   adt::info_clear_kernel<<<1, 256, 0, s>>>(infos, nev);
   ADT_HIP(hipGetLastError());
+  // forward and adjoint of each event in one kernel (every event diagonal)
+  if (tn.fuse && (flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && max_l <= tn.diag_max_l) {
+    const bool mix = flags & ADT_F32_COMPUTE_F64;
+    if constexpr (std::is_same<T, float>::value) {
+      if (mix) {
+        adt::fwd_adj3_diag_kernel<float, true><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
+        ADT_HIP(hipGetLastError());
+        return ADT_OK;
+      }
+    }
+    adt::fwd_adj3_diag_kernel<T, false><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
+    ADT_HIP(hipGetLastError());
+    return ADT_OK;
+  }
   if (flags & ADT_FORWARD) {
     // events with l <= diag_max_l: diagonal streaming; longer ones: k-streaming
     const bool mix = flags & ADT_F32_COMPUTE_F64;
@@ -901,6 +916,7 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     adt_tune t = cur;
     t.order = order;
     float fbest = 0.f, abest = 0.f;
+    t.fuse = 0;  // the two kernels are timed apart first
     if (flags & ADT_FORWARD) {
       fbest = 3.4e38f;
       for (int dml : dmls) {
@@ -933,8 +949,21 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
       }
       if (rc) break;
     }
-    if (fbest + abest < win_total) {
-      win_total = fbest + abest;
+    // both kernels requested and every event diagonal: the fused kernel
+    // against the best pair of separate ones
+    float total = fbest + abest;
+    if ((flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && plan.max_l <= t.diag_max_l) {
+      adt_tune c = t;
+      c.fuse = 1;
+      float ms = 0.f;
+      if ((rc = time_it(flags & (ADT_FORWARD | ADT_ADJOINT | ADT_F32_COMPUTE_F64), c, &ms))) break;
+      if (ms < total) {
+        total = ms;
+        t.fuse = 1;
+      }
+    }
+    if (total < win_total) {
+      win_total = total;
       win = t;
       win.fwd_ms = fbest;
       win.adj_ms = abest;
@@ -1030,7 +1059,7 @@ int adt_get_tune(int32_t dtype, adt_tune* out) {
 int adt_set_tune(int32_t dtype, const adt_tune* t) {
   if ((dtype != 0 && dtype != 1) || !t) return fail(ADT_EINVAL, "set_tune: bad arguments");
   if (!tune_valid(*t))
-    return fail(ADT_EINVAL, "set_tune: diag_max_l must be in [0, 64], adj_wg 256/512/1024, order 0/1");
+    return fail(ADT_EINVAL, "set_tune: diag_max_l must be in [0, 2^30], adj_wg 256/512/1024, order 0/1, fuse 0/1");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[dtype] = *t;
   return ADT_OK;

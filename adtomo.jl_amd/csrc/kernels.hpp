@@ -92,6 +92,10 @@ constexpr size_t kAdjRecBytes = 4 * sizeof(T);
 constexpr int kDiagAll = 1 << 30;
 template <typename T, bool MIX>
 __global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
+// Forward then adjoint of each event in one workgroup (every event on the
+// diagonal forward).
+template <typename T, bool MIX>
+__global__ void fwd_adj3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
 __global__ void info_clear_kernel(adt_info* info, int n);
 template <typename T>
 __global__ void strided_copy_kernel(const T* src, T* dst, int64_t N, int64_t d1, int64_t d2,

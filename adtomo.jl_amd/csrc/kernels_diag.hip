@@ -41,6 +41,7 @@
 // err = max|u - u_old| is a coalesced pass after the eighth sweep against
 // the snapshot (u0 for the first iteration), which also writes the next
 // snapshot u_old = u.
+#include "adjoint3.hpp"
 #include "buffer_ops.hpp"
 #include "eikonal_device.hpp"
 #include "kernels.hpp"
@@ -696,16 +697,15 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
   return diag_sweep_wave<T, false, MIX, ERR, false>(S, w, cnt, xch, xold, planes, prog, abort_flag);
 }
 
+// The whole forward of one event, run by one workgroup of kDW waves.
 template <typename T, bool MIX>
-__global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __restrict__ evs,
-                                                             int max_iter, int max_l) {
+__device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_iter, int max_l) {
   __shared__ T xch[2 * kDW * 64];
   __shared__ T xold[2 * kDW * 64];
   __shared__ T planes[kDW * 3 * Pieces<T>::PLANE];
   __shared__ uint64_t red[kDW];
   __shared__ int prog[kDW];
   __shared__ int abort_flag;
-  const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
   if (l > max_l) return;  // handled by the k-streaming kernel
   const int64_t N = (int64_t)m * n * l;
@@ -790,8 +790,34 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   }
 }
 
+template <typename T, bool MIX>
+__global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __restrict__ evs,
+                                                             int max_iter, int max_l) {
+  fwd3_diag_event<T, MIX>(evs[blockIdx.x], max_iter, max_l);
+}
+
+// Forward and then adjoint of one event per workgroup, for batches whose
+// every event takes the diagonal forward: an event whose forward converges
+// in fewer iterations starts its adjoint while others still sweep, so the
+// memory-bound traversal overlaps the latency-bound sweeps.  The adjoint
+// runs with kDW waves and four queue entries per thread.
+template <typename T, bool MIX>
+__global__ void __launch_bounds__(kDW * 64) fwd_adj3_diag_kernel(const EventDesc* __restrict__ evs,
+                                                                 int max_iter, int max_l) {
+  const EventDesc& ev = evs[blockIdx.x];
+  fwd3_diag_event<T, MIX>(ev, max_iter, max_l);
+  // the adjoint reads the travel times the waves of this workgroup stored
+  __threadfence();
+  __syncthreads();
+  __threadfence();
+  adj3_kahn_event<T, 4>(ev);
+}
+
 template __global__ void fwd3_diag_kernel<double, false>(const EventDesc*, int, int);
 template __global__ void fwd3_diag_kernel<float, false>(const EventDesc*, int, int);
 template __global__ void fwd3_diag_kernel<float, true>(const EventDesc*, int, int);
+template __global__ void fwd_adj3_diag_kernel<double, false>(const EventDesc*, int, int);
+template __global__ void fwd_adj3_diag_kernel<float, false>(const EventDesc*, int, int);
+template __global__ void fwd_adj3_diag_kernel<float, true>(const EventDesc*, int, int);
 
 }  // namespace adt

@@ -7,10 +7,12 @@ sources, u0 = 1000 with 0 at the source, tol 1e-3, sparse adjoint sources at
 hipGraph launch.  Inputs are resident in HBM before timing starts.
 
 Also reported: the plain per-event launch loop on the same events, the
-achieved bandwidth of both kernels against the HBM roofline (algorithmic
+achieved bandwidth of the kernels against the HBM roofline (algorithmic
 bytes: forward 8*K*N*(s_f + 2*s_u) per event, adjoint
-N*(s_u+s_u0+s_f+s_g+s_gradf+s_gradu0)), "roofline" being the kernel with the
-longer launch, and the CPU oracle timed on a bounded sample of the same
+N*(s_u+s_u0+s_f+s_g+s_gradf+s_gradu0)): "roofline" is the step's kernel when
+forward and adjoint run fused in one kernel (the default setting), else the
+longer of the two; the separate forward and adjoint kernels are timed on
+their own as well; and the CPU oracle timed on a bounded sample of the same
 events on the host cores, with, on one 64x64x32 event, the oracle's
 triangular adjoint solve against the reference's sparse-LU route (SciPy
 SuperLU standing in for Eigen's SparseLU).
@@ -43,7 +45,9 @@ HBM_PEAK_GBS = 8000.0
 # kernel of known size: profiles/pmc_traffic_fwd3_diag_adj3_kahn_256x128x128x64.md
 FWD_KERNEL = "fwd3_diag_kernel<double, false>"
 ADJ_KERNEL = "adj3_kahn_kernel<double>"
-TRAFFIC_BYTES_256 = {FWD_KERNEL: 117.7e9 + 57.2e9, ADJ_KERNEL: 129.2e9 + 49.3e9}
+FUSED_KERNEL = "fwd_adj3_diag_kernel<double, false>"
+TRAFFIC_BYTES_256 = {FWD_KERNEL: 116.6e9 + 57.1e9, ADJ_KERNEL: 129.4e9 + 49.3e9,
+                     FUSED_KERNEL: 241.4e9 + 105.1e9}
 
 
 def log(*a):
@@ -282,9 +286,15 @@ def main():
     def roof(kernel, gbs, kms):
         return {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "launch_ms": round(kms, 3),
-                "traffic": TRAFFIC_BYTES_256[kernel] if nev == 256 else None}
+                "traffic": TRAFFIC_BYTES_256.get(kernel) if nev == 256 else None}
 
     roofs = [roof(FWD_KERNEL, fwd_gbs, fwd_ms), roof(ADJ_KERNEL, adj_gbs, adj_ms)]
+    # with the fused setting the step is one kernel doing each event's
+    # forward and then its adjoint: it is the step's dominant kernel
+    fused = bool(tune["fuse"]) and tune["diag_max_l"] >= shape[2]
+    if fused:
+        roofs.insert(0, roof(FUSED_KERNEL, total_gbs, ms))
+    dominant = roofs[0] if fused else max(roofs, key=lambda r: r["launch_ms"])
     out = {
         "metric": "event-solves/s fwd+adjoint at 128x128x64 (batched), HBM GB/s",
         "value": round(value, 3),
@@ -303,8 +313,8 @@ def main():
                    "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
                    "serial_depth_per_sweep": int(sum(shape) - 2),
                    "adj_sweeps_mean": float(sweeps.mean()),
-                   "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order")}},
-        "roofline": roofs[0] if fwd_ms >= adj_ms else roofs[1],
+                   "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order", "fuse")}},
+        "roofline": dominant,
         "roofline_kernels": roofs,
         "cpu_baseline": cpu,
         "fp32": fp32,
@@ -388,7 +398,7 @@ def run_uneven(args):
           {"workload": "256 events, boxes uniform in 48x48x24..128x128x64, h=0.5 km, layered "
                        "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
            "events": nev, "nodes_total": nodes, "K_mean": float(K64.mean()),
-           "launch_setting": {k: tune64[k] for k in ("diag_max_l", "adj_wg", "order")}},
+           "launch_setting": {k: tune64[k] for k in ("diag_max_l", "adj_wg", "order", "fuse")}},
           fp32=variant("f32"), fp32_storage_fp64_solve=variant("mixed"))
 
 

@@ -207,6 +207,13 @@ typedef struct adt_tune {
                          64; default 2^30 = all), the others k-streaming */
   int32_t adj_wg;     /* adjoint workgroup size: 256, 512 or 1024 */
   int32_t order;      /* 0: events in the given order; 1: largest first */
+  int32_t fuse;       /* 1: a call or graph with both ADT_FORWARD and
+                         ADT_ADJOINT whose every event takes the diagonal
+                         forward runs one kernel that does each event's
+                         forward and then its adjoint (an event's adjoint
+                         starts as soon as its forward converges; adj_wg
+                         does not apply); 0: separate forward and adjoint
+                         kernels */
   float fwd_ms;       /* adt_autotune: measured forward / adjoint time per */
   float adj_ms;       /* launch of the chosen setting (ms) */
 } adt_tune;

@@ -133,16 +133,18 @@ def test_tune_get_set_validation_without_device():
     base = A.get_tune("float64")
     try:
         assert base["diag_max_l"] == 1 << 30 and base["adj_wg"] in (256, 512, 1024)
-        t = A.set_tune("float64", diag_max_l=0, adj_wg=256, order=0)
-        assert A.get_tune("float64")["adj_wg"] == 256 and t["diag_max_l"] == 0
+        assert base["fuse"] == 1
+        t = A.set_tune("float64", diag_max_l=0, adj_wg=256, order=0, fuse=0)
+        assert A.get_tune("float64")["adj_wg"] == 256 and t["diag_max_l"] == 0 and t["fuse"] == 0
         assert A.get_tune("float32")["adj_wg"] == base["adj_wg"]  # per precision
+        assert A.get_tune("float32")["fuse"] == base["fuse"]
         for bad in ({"diag_max_l": (1 << 30) + 1}, {"diag_max_l": -1}, {"adj_wg": 300},
-                    {"order": 2}):
+                    {"order": 2}, {"fuse": 2}, {"fuse": -1}):
             with pytest.raises(A.AdtError):
                 A.set_tune("float64", **bad)
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"])
+                   order=base["order"], fuse=base["fuse"])
 
 
 def _c_struct_fields(name):

@@ -515,11 +515,12 @@ def _mixed_batch(seed=300):
 
 
 @pytest.mark.parametrize("setting", [
-    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1},
-    {"diag_max_l": 64, "adj_wg": 1024, "order": 1},
-    {"diag_max_l": 0, "adj_wg": 256, "order": 0},
-    {"diag_max_l": 32, "adj_wg": 512, "order": 1},
-    {"diag_max_l": 7, "adj_wg": 256, "order": 1},
+    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1, "fuse": 1},
+    {"diag_max_l": 1 << 30, "adj_wg": 512, "order": 0, "fuse": 0},
+    {"diag_max_l": 64, "adj_wg": 1024, "order": 1, "fuse": 1},
+    {"diag_max_l": 0, "adj_wg": 256, "order": 0, "fuse": 1},
+    {"diag_max_l": 32, "adj_wg": 512, "order": 1, "fuse": 0},
+    {"diag_max_l": 7, "adj_wg": 256, "order": 1, "fuse": 1},
 ])
 def test_every_tune_setting_matches_oracle(setting):
     import adtomo_hip as A
@@ -545,7 +546,7 @@ def test_every_tune_setting_matches_oracle(setting):
         G.close()
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"])
+                   order=base["order"], fuse=base["fuse"])
 
 
 def test_autotune_stores_a_valid_setting():
@@ -558,15 +559,16 @@ def test_autotune_stores_a_valid_setting():
         print("autotune:", best, flush=True)
         assert best["diag_max_l"] in (0, 64, 1 << 30) and best["adj_wg"] in (256, 512, 1024)
         assert best["order"] in (0, 1) and best["fwd_ms"] > 0 and best["adj_ms"] > 0
+        assert best["fuse"] in (0, 1)
         cur = A.get_tune("float64")
-        assert all(cur[k] == best[k] for k in ("diag_max_l", "adj_wg", "order"))
+        assert all(cur[k] == best[k] for k in ("diag_max_l", "adj_wg", "order", "fuse"))
         res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
                                   [1e-3] * len(fs))
         for e in range(len(fs)):
             assert bits_equal(host(res["u"][e]), oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-3)[0])
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"])
+                   order=base["order"], fuse=base["fuse"])
 
 
 # ------------------------------------------- FP32 storage, FP64 solves ---

@@ -549,6 +549,37 @@ def test_every_tune_setting_matches_oracle(setting):
                    order=base["order"], fuse=base["fuse"])
 
 
+@pytest.mark.parametrize("dtype,compute_f64", [("float64", False), ("float32", False),
+                                                ("float32", True)])
+def test_fused_forward_adjoint_matches_separate_kernels(dtype, compute_f64):
+    # one kernel per event (forward, then adjoint) against the two kernels:
+    # travel times, iteration counts and both gradients bit for bit
+    import adtomo_hip as A
+    torch = _torch()
+    base = A.get_tune(dtype)
+    fs, u0s, gs = _mixed_batch(600)
+    npd = np.float64 if dtype == "float64" else np.float32
+    args = ([dev(a, npd) for a in u0s], [dev(a, npd) for a in fs], [0.5] * len(fs),
+            [1e-4] * len(fs))
+    out = {}
+    try:
+        for fuse in (1, 0):
+            A.set_tune(dtype, diag_max_l=1 << 30, fuse=fuse)
+            G = A.EikonalGraph(*args, grad_us=[dev(a, npd) for a in gs], compute_f64=compute_f64)
+            G.launch()
+            G.launch()
+            torch.cuda.synchronize()
+            out[fuse] = ([host(u) for u in G.us], [host(g) for g in G.grad_fs],
+                         [host(g) for g in G.grad_u0s], [i["iterations"] for i in G.infos()])
+            G.close()
+    finally:
+        A.set_tune(dtype, diag_max_l=base["diag_max_l"], fuse=base["fuse"])
+    for k in range(3):
+        for a, b in zip(out[1][k], out[0][k]):
+            assert bits_equal(a, b), (dtype, compute_f64, k)
+    assert out[1][3] == out[0][3]
+
+
 def test_autotune_stores_a_valid_setting():
     import adtomo_hip as A
     base = A.get_tune("float64")

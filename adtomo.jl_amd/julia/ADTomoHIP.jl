@@ -21,7 +21,8 @@ using ChainRulesCore
 using Libdl
 
 export eikonal, eikonal3d, eikonal3d_batched!, EikonalGraph, launch!, close!, source_u0!,
-       misfit!
+       misfit!, model!, model_grad!, slowness!, slowness_grad!, smooth_l1!, sum_arrays!,
+       AdtTune, get_tune, set_tune!, autotune!
 
 const libadt = normpath(joinpath(@__DIR__, "..", "libadtomo_hip.so"))
 const ADT_HOST_POINTERS = Int32(0x1)
@@ -316,6 +317,160 @@ function misfit!(loss, us, rx, ry, rz, tobs, weights; grad_us=nothing, tcals=not
                length(rx), stream)
     check(rc, "misfit")
     return loss
+end
+
+# ------------------------------------- model, regulariser, sum, tuning ---
+
+objfn(base, A) = Libdl.dlsym(Libdl.dlopen(libadt), Symbol(base, sfx(A)))
+
+"""
+    model!(fvar, f, x, vel0; stream=C_NULL)
+
+fvar = 2 sigmoid(x) - 1 + vel0 and f = 1 ./ fvar (scripts/inversion.jl:42-43,61)
+on device arrays of one length (`f` may be `nothing`).
+"""
+function model!(fvar, f, x, vel0; stream::Ptr{Cvoid}=C_NULL)
+    rc = ccall(objfn("adt_model3d", x), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+               devptr(x), devptr(vel0), length(x), devptr(fvar), devptr(f), stream)
+    check(rc, "model")
+    return fvar
+end
+
+"""
+    model_grad!(grad_x, x, fvar; grad_f=nothing, grad_fvar=nothing, stream=C_NULL)
+
+Chain rule of `model!`: grad_x = (grad_fvar - grad_f ./ fvar.^2) .* 2 sigmoid'(x).
+"""
+function model_grad!(grad_x, x, fvar; grad_f=nothing, grad_fvar=nothing,
+                     stream::Ptr{Cvoid}=C_NULL)
+    rc = ccall(objfn("adt_model3d_grad", x), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}),
+               devptr(x), devptr(fvar), devptr(grad_f), devptr(grad_fvar), length(x),
+               devptr(grad_x), stream)
+    check(rc, "model_grad")
+    return grad_x
+end
+
+"""
+    slowness!(f_s, fvar, scale; stream=C_NULL)
+
+S slowness f_s = scale ./ fvar (scripts/inversion_joint.jl:79, scale = vp/vs).
+"""
+function slowness!(f_s, fvar, scale; stream::Ptr{Cvoid}=C_NULL)
+    T = eltype(fvar)
+    rc = ccall(objfn("adt_slowness3d", fvar), Cint, (Ptr{Cvoid}, Int64, T, Ptr{Cvoid}, Ptr{Cvoid}),
+               devptr(fvar), length(fvar), T(scale), devptr(f_s), stream)
+    check(rc, "slowness")
+    return f_s
+end
+
+"""
+    slowness_grad!(grad_fvar, fvar, scale, grad_fs; grad_scale=nothing, stream=C_NULL)
+
+Adjoint of `slowness!`: grad_fvar .+= -scale .* grad_fs ./ fvar.^2 (in place);
+`grad_scale` (a one-element device array) receives sum(grad_fs ./ fvar).
+"""
+function slowness_grad!(grad_fvar, fvar, scale, grad_fs; grad_scale=nothing,
+                        stream::Ptr{Cvoid}=C_NULL)
+    T = eltype(fvar)
+    rc = ccall(objfn("adt_slowness3d_grad", fvar), Cint,
+               (Ptr{Cvoid}, Int64, T, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+               devptr(fvar), length(fvar), T(scale), devptr(grad_fs), devptr(grad_fvar),
+               devptr(grad_scale), stream)
+    check(rc, "slowness_grad")
+    return grad_fvar
+end
+
+"""
+    smooth_l1!(grad_fvar, value, fvar, sh, sv, lambda; stream=C_NULL)
+
+Box-smoothing L1 regulariser (scripts/inversion.jl:107-121) of an (l, n, m)
+device array: `value` (one-element device array) = lambda * sum |fvar - box(fvar)|,
+`grad_fvar` its gradient (may be `nothing`).
+"""
+function smooth_l1!(grad_fvar, value, fvar, sh::Integer, sv::Integer, lambda;
+                    stream::Ptr{Cvoid}=C_NULL)
+    T = eltype(fvar)
+    m, n, l = griddims(fvar)
+    rc = ccall(objfn("adt_smooth3d_l1", fvar), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Int64, Int32, Int32, T, Ptr{Cvoid}, Ptr{Cvoid},
+                Ptr{Cvoid}),
+               devptr(fvar), m, n, l, Int32(sh), Int32(sv), T(lambda), devptr(grad_fvar),
+               devptr(value), stream)
+    check(rc, "smooth_l1")
+    return value
+end
+
+"""
+    sum_arrays!(out, arrays; stream=C_NULL)
+
+out = sum of the device arrays in order (the per-station dL/df sum).
+"""
+function sum_arrays!(out, arrays; stream::Ptr{Cvoid}=C_NULL)
+    ptrs = Ptr{Cvoid}[devptr(a) for a in arrays]
+    rc = ccall(objfn("adt_sum_arrays", out), Cint,
+               (Ptr{Ptr{Cvoid}}, Int32, Int64, Ptr{Cvoid}, Ptr{Cvoid}),
+               ptrs, length(ptrs), length(out), devptr(out), stream)
+    check(rc, "sum_arrays")
+    return out
+end
+
+# Mirrors `adt_tune` in include/adtomo_hip.h.
+struct AdtTune
+    diag_max_l::Int32
+    adj_wg::Int32
+    order::Int32
+    fuse::Int32
+    fwd_ms::Float32
+    adj_ms::Float32
+end
+
+dtype_code(::Type{Float32}) = Int32(1)
+dtype_code(::Type{Float64}) = Int32(0)
+
+"""
+    get_tune(T=Float64)
+
+Launch setting of the 3D kernels for element type `T`.
+"""
+function get_tune(::Type{T}=Float64) where {T}
+    t = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0))
+    check(ccall((:adt_get_tune, libadt), Cint, (Int32, Ref{AdtTune}), dtype_code(T), t),
+          "get_tune")
+    return t[]
+end
+
+"""
+    set_tune!(t::AdtTune, T=Float64)
+
+Stores a launch setting (validated by the library) for element type `T`.
+"""
+function set_tune!(t::AdtTune, ::Type{T}=Float64) where {T}
+    check(ccall((:adt_set_tune, libadt), Cint, (Int32, Ref{AdtTune}), dtype_code(T), Ref(t)),
+          "set_tune")
+    return t
+end
+
+"""
+    autotune!(us, u0s, fs, hs, tols; grad_us, grad_u0s, grad_fs, reps=3,
+              compute_f64=false, stream=C_NULL)
+
+Times the launch settings on these events (device arrays, as for
+`eikonal3d_batched!`), keeps the fastest for their element type and
+returns it; the events' outputs hold a valid solve afterwards.
+"""
+function autotune!(us, u0s, fs, hs, tols; grad_us=nothing, grad_u0s=nothing, grad_fs=nothing,
+                   reps::Integer=3, compute_f64::Bool=false, stream::Ptr{Cvoid}=C_NULL)
+    tab = event_table(u0s, fs, us, hs, tols, grad_us, grad_u0s, grad_fs)
+    flags = ADT_FORWARD | (grad_us === nothing ? Int32(0) : ADT_ADJOINT) |
+            (compute_f64 ? ADT_F32_COMPUTE_F64 : Int32(0))
+    best = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0))
+    rc = ccall((:adt_autotune, libadt), Cint,
+               (Ptr{AdtEvent3d}, Int32, Int32, Int32, Int32, Ref{AdtTune}, Ptr{Cvoid}),
+               tab, length(tab), flags, dtype_code(eltype(fs[1])), Int32(reps), best, stream)
+    check(rc, "autotune")
+    return best[]
 end
 
 end # module

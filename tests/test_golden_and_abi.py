@@ -175,7 +175,7 @@ def _julia_struct_fields(name):
 @pytest.mark.parametrize("c_name,py_name,jl_name", [
     ("adt_info", "AdtInfo", "AdtInfo"), ("adt_event3d", "AdtEvent3d", "AdtEvent3d"),
     ("adt_source3d", "AdtSource3d", "AdtSource3d"), ("adt_misfit3d", "AdtMisfit3d", "AdtMisfit3d"),
-    ("adt_tune", "AdtTune", None)])
+    ("adt_tune", "AdtTune", "AdtTune")])
 def test_struct_layouts_agree_across_bindings(c_name, py_name, jl_name):
     # the C header, the ctypes twin and the Julia module declare the same
     # fields in the same order

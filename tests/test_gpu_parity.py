@@ -226,6 +226,36 @@ def test_headline_single_event_bit_exact():
     assert bits_equal(host(u), u_ref)
 
 
+def test_headline_batch_full_size():
+    # the full headline batch, 256 events of 128x128x64, forward + adjoint in
+    # one call with the default launch setting (fused kernel): every event
+    # converged (forward and adjoint) with the op's invariants (0 <= u <= u0,
+    # u = 0 at the source, grad_u0 = grad_u exactly on the set u == u0 and 0
+    # elsewhere), and two sampled events against the oracle
+    import adtomo_hip as A
+    torch = _torch()
+    nev = 256
+    f, u0s, gs, srcs = synth.headline_event_set(nev)
+    fd = dev(f)
+    u0d = [dev(a) for a in u0s]
+    gd = [dev(a) for a in gs]
+    res = A.eikonal3d_batched(u0d, [fd] * nev, [0.5] * nev, [1e-3] * nev, grad_us=gd)
+    infos = res["info"]
+    assert all(i["converged"] and i["adj_converged"] and not i["fault"] for i in infos)
+    for e in range(nev):
+        u = res["u"][e]
+        assert bool((u <= u0d[e]).all()) and bool((u >= 0).all())
+        assert float(u[srcs[e]]) == 0.0
+        z = torch.where(u == u0d[e], gd[e], torch.zeros_like(gd[e]))
+        assert torch.equal(res["grad_u0"][e], z)
+    for e in (0, 173):
+        u_ref, it, _ = oracle.eikonal3d(u0s[e], f, 0.5, 1e-3)
+        assert infos[e]["iterations"] == it and bits_equal(host(res["u"][e]), u_ref)
+        _, gf_ref, _ = oracle.eikonal3d_grad(gs[e], u_ref, u0s[e], f, 0.5)
+        assert rel(host(res["grad_f"][e]), gf_ref) <= GRAD_RTOL
+    print("headline batch K:", sorted(set(i["iterations"] for i in infos)), flush=True)
+
+
 def test_plain_path_matches_oracle():
     import adtomo_hip as A
     shape = (24, 20, 16)

@@ -57,6 +57,7 @@ struct Walk {
   int co[3];
   int step[3];
   int ex[3];
+  Walk() = default;
   __device__ Walk(int64_t p, int64_t stride, int m, int n, int l) {
     const int64_t nl = (int64_t)n * l;
     co[0] = (int)(p / nl);
@@ -81,9 +82,6 @@ struct Walk {
 };
 
 constexpr uint32_t kCntShift = 13;
-// Nodes per thread and iteration of the prep passes (all their loads are
-// issued before any is used, so more requests are in flight per wave).
-constexpr int kPrepNodes = 2;
 constexpr uint32_t kZ = 0x40u;
 constexpr int64_t kPackedMaxNodes = int64_t(1) << 20;
 
@@ -118,19 +116,21 @@ __device__ __forceinline__ uint32_t code3(T up, T u0p, const T (&nb)[6], const i
   return cd ? cd : kZ;
 }
 
-// kPrepNodes nodes p0 + k * blockDim.x of the prep passes' grid-stride
-// loop, with their coordinates; a node past N is replaced by p0 (its loads
-// stay in range, its results are not stored).
+// P nodes p0 + k * blockDim.x of the prep passes' grid-stride loop, with
+// their coordinates; a node past N is replaced by p0 (its loads stay in
+// range, its results are not stored).
+template <int P>
 struct PrepGroup {
-  int64_t q[kPrepNodes];
-  bool valid[kPrepNodes];
-  int co[kPrepNodes][3];
+  int64_t q[P];
+  bool valid[P];
+  int co[P][3];
 };
 
-__device__ __forceinline__ PrepGroup prep_group(int64_t p0, int64_t N, Walk (&wk)[kPrepNodes]) {
-  PrepGroup G;
+template <int P>
+__device__ __forceinline__ PrepGroup<P> prep_group(int64_t p0, int64_t N, Walk (&wk)[P]) {
+  PrepGroup<P> G;
 #pragma unroll
-  for (int k = 0; k < kPrepNodes; ++k) {
+  for (int k = 0; k < P; ++k) {
     const int64_t pk = p0 + (int64_t)k * blockDim.x;
     G.valid[k] = pk < N;
     G.q[k] = G.valid[k] ? pk : p0;
@@ -138,7 +138,7 @@ __device__ __forceinline__ PrepGroup prep_group(int64_t p0, int64_t N, Walk (&wk
     for (int d = 0; d < 3; ++d) G.co[k][d] = G.valid[k] ? wk[k].co[d] : wk[0].co[d];
   }
 #pragma unroll
-  for (int k = 0; k < kPrepNodes; ++k) wk[k].advance();
+  for (int k = 0; k < P; ++k) wk[k].advance();
   return G;
 }
 
@@ -227,8 +227,10 @@ __device__ __forceinline__ void push_ready(const uint32_t (&old)[3], const int64
 
 // The whole adjoint of one event, run by one workgroup of any size.  kIlp
 // queue entries per thread and round are resolved with their loads in
-// flight together (more registers per entry, fewer rounds per level).
-template <typename T, int kIlp>
+// flight together (more registers per entry, fewer rounds per level); the
+// prep passes take kPrep nodes per thread and iteration, issuing all their
+// loads before using any (more requests in flight per wave).
+template <typename T, int kIlp, int kPrep>
 __device__ __forceinline__ void adj3_kahn_event(const EventDesc& ev) {
   __shared__ int qtail;
   __shared__ int nonz;
@@ -249,19 +251,20 @@ __device__ __forceinline__ void adj3_kahn_event(const EventDesc& ev) {
     nonz = 0;
   }
   // pass 1: upwind codes; grad_u0 = g where u == u0 (Eikonal3D.cpp:105-110).
-  // Each thread takes kPrepNodes nodes per iteration and issues all their
+  // Each thread takes kPrep nodes per iteration and issues all their
   // loads before using any, so that more requests are in flight per wave.
   const int ex[3] = {m, n, l};
   const int64_t st[3] = {nl, l, 1};
-  const int64_t stride = kPrepNodes * (int64_t)blockDim.x;
+  const int64_t stride = kPrep * (int64_t)blockDim.x;
   {
-    Walk wk[kPrepNodes] = {Walk(threadIdx.x, stride, m, n, l),
-                           Walk(threadIdx.x + blockDim.x, stride, m, n, l)};
-    for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
-      const PrepGroup G = prep_group(p0, N, wk);
-      T uc[kPrepNodes], u0c[kPrepNodes], gc[kPrepNodes], nb[kPrepNodes][6];
+    Walk wk[kPrep];
 #pragma unroll
-      for (int k = 0; k < kPrepNodes; ++k) {
+    for (int k = 0; k < kPrep; ++k) wk[k] = Walk(threadIdx.x + k * blockDim.x, stride, m, n, l);
+    for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
+      const PrepGroup<kPrep> G = prep_group<kPrep>(p0, N, wk);
+      T uc[kPrep], u0c[kPrep], gc[kPrep], nb[kPrep][6];
+#pragma unroll
+      for (int k = 0; k < kPrep; ++k) {
         const int64_t q = G.q[k];
         uc[k] = u[q];
         u0c[k] = u0[q];
@@ -273,7 +276,7 @@ __device__ __forceinline__ void adj3_kahn_event(const EventDesc& ev) {
         }
       }
 #pragma unroll
-      for (int k = 0; k < kPrepNodes; ++k) {
+      for (int k = 0; k < kPrep; ++k) {
         const uint32_t code = code3<T>(uc[k], u0c[k], nb[k], G.co[k], ex);
         if (G.valid[k]) {
           if (gu0) gu0[G.q[k]] = (uc[k] == u0c[k]) ? gc[k] : T(0);
@@ -288,14 +291,15 @@ __device__ __forceinline__ void adj3_kahn_event(const EventDesc& ev) {
   // bits 7-15; the code bits read here never change.
   int my_nonz = 0;
   {
-    Walk wk[kPrepNodes] = {Walk(threadIdx.x, stride, m, n, l),
-                           Walk(threadIdx.x + blockDim.x, stride, m, n, l)};
-    for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
-      const PrepGroup G = prep_group(p0, N, wk);
-      uint32_t cr[kPrepNodes], nm[kPrepNodes][6];
-      T uc[kPrepNodes], gc[kPrepNodes], nu[kPrepNodes][6];
+    Walk wk[kPrep];
 #pragma unroll
-      for (int k = 0; k < kPrepNodes; ++k) {
+    for (int k = 0; k < kPrep; ++k) wk[k] = Walk(threadIdx.x + k * blockDim.x, stride, m, n, l);
+    for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
+      const PrepGroup<kPrep> G = prep_group<kPrep>(p0, N, wk);
+      uint32_t cr[kPrep], nm[kPrep][6];
+      T uc[kPrep], gc[kPrep], nu[kPrep][6];
+#pragma unroll
+      for (int k = 0; k < kPrep; ++k) {
         const int64_t q = G.q[k];
         cr[k] = meta[q];
         uc[k] = u[q];
@@ -311,7 +315,7 @@ __device__ __forceinline__ void adj3_kahn_event(const EventDesc& ev) {
         }
       }
 #pragma unroll
-      for (int k = 0; k < kPrepNodes; ++k) {
+      for (int k = 0; k < kPrep; ++k) {
         uint32_t fm = 0;  // neighbours whose upwind neighbour this node is
 #pragma unroll
         for (int d = 0; d < 3; ++d) {

@@ -25,7 +25,7 @@ __device__ __forceinline__ unsigned dec_byte(uint8_t* base, int64_t idx) {
 
 template <typename T>
 __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __restrict__ evs) {
-  adj3_kahn_event<T, 2>(evs[blockIdx.x]);
+  adj3_kahn_event<T, 2, 2>(evs[blockIdx.x]);
 }
 
 // ----------------------------------------------------------------- 2D ---

@@ -800,7 +800,8 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
 // every event takes the diagonal forward: an event whose forward converges
 // in fewer iterations starts its adjoint while others still sweep, so the
 // memory-bound traversal overlaps the latency-bound sweeps.  The adjoint
-// runs with kDW waves and four queue entries per thread.
+// runs with kDW waves, four queue entries and four prep-pass nodes per
+// thread (the forward's register budget leaves room for them).
 template <typename T, bool MIX>
 __global__ void __launch_bounds__(kDW * 64) fwd_adj3_diag_kernel(const EventDesc* __restrict__ evs,
                                                                  int max_iter, int max_l) {
@@ -810,7 +811,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd_adj3_diag_kernel(const EventDesc
   __threadfence();
   __syncthreads();
   __threadfence();
-  adj3_kahn_event<T, 4>(ev);
+  adj3_kahn_event<T, 4, 4>(ev);
 }
 
 template __global__ void fwd3_diag_kernel<double, false>(const EventDesc*, int, int);

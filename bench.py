@@ -46,8 +46,8 @@ HBM_PEAK_GBS = 8000.0
 FWD_KERNEL = "fwd3_diag_kernel<double, false>"
 ADJ_KERNEL = "adj3_kahn_kernel<double>"
 FUSED_KERNEL = "fwd_adj3_diag_kernel<double, false>"
-TRAFFIC_BYTES_256 = {FWD_KERNEL: 116.6e9 + 57.1e9, ADJ_KERNEL: 129.4e9 + 49.3e9,
-                     FUSED_KERNEL: 241.4e9 + 105.1e9}
+TRAFFIC_BYTES_256 = {FWD_KERNEL: 117.9e9 + 57.2e9, ADJ_KERNEL: 129.4e9 + 49.3e9,
+                     FUSED_KERNEL: 243.9e9 + 105.0e9}
 
 
 def log(*a):

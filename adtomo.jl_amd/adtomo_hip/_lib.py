@@ -77,7 +77,7 @@ class AdtMisfit3d(ctypes.Structure):
 
 
 class AdtTune(ctypes.Structure):
-    _fields_ = [("diag_max_l", _i32), ("adj_wg", _i32), ("order", _i32), ("fuse", _i32),
+    _fields_ = [("diag_max_l", _i32), ("adj_wg", _i32), ("order", _i32), ("fuse_max_n", _i32),
                 ("fwd_ms", ctypes.c_float), ("adj_ms", ctypes.c_float)]
 
     def as_dict(self):
@@ -102,6 +102,7 @@ _SIGS = {
     "adt_eikonal3d_bwd_strided": [_p, _p, _p, _p, "R", _i64, _i64, _i64, _p, _p, _p, _p, _i32,
                                   _p],
     "adt_eikonal3d_batched": [_p, _i32, _i32, _p, _p],
+    "adt_eikonal3d_misfit_batched": [_p, _p, _i32, _p, _p, _p, _i64, _i32, _p, _p],
     "adt_selftest_arith": [_i32, _p, _p, _p, _i64, _p],
     "adt_source_u0_3d_batched": [_p, _i32, _p, _i64, _i64, _i64, "R", _p],
     "adt_misfit3d_batched": [_p, _i32, _p, _p, _p, _i64, _p],
@@ -134,6 +135,9 @@ def lib():
     L.adt_eikonal3d_bwd_plain_f64.restype = _i32
     L.adt_graph_create.argtypes = [_p, _i32, _i32, _i32, ctypes.POINTER(_p), _p]
     L.adt_graph_create.restype = _i32
+    L.adt_graph_create_misfit.argtypes = [_p, _p, _i32, _p, _p, _p, _i64, _i32, _i32,
+                                          ctypes.POINTER(_p), _p]
+    L.adt_graph_create_misfit.restype = _i32
     L.adt_graph_launch.argtypes = [_p, _p]
     L.adt_graph_launch.restype = _i32
     L.adt_graph_infos.argtypes = [_p, _p, _p]

@@ -16,7 +16,11 @@ Stations are solved ``chunk`` at a time, all phases in the same launch:
 each chunk's forward, misfit and adjoint run back to back over one set of
 slot buffers, and the per-station dL/df are summed into one accumulator per
 phase, so device memory holds ``chunk`` x phases travel-time fields (plus
-their adjoint workspace), not one per station.
+their adjoint workspace), not one per station.  When every station fits in
+one chunk, forward, misfit and adjoint of all stations are one hipGraph
+launch (``objective.InversionStepGraph``: each station's adjoint starts as
+soon as its own forward and misfit are done); with several chunks the three
+run as separate launches per chunk.
 By default ``chunk`` is every station when their slots fit in 80% of the
 free device memory, else the largest count that fits.
 Without chunking (``chunk`` = number of stations) the initial times u0_s,
@@ -102,16 +106,23 @@ class TravelTimeInversion:
         C = self.chunk
         P = len(self.phases)
         self.u0s = [torch.empty_like(self.x) for _ in range(P * C)]
-        self.grad_us = [torch.empty_like(self.x) for _ in range(P * C)]
         fs = [self.f[p] for p in self.phases for _ in range(C)]
         hs, tols = [self.h] * (P * C), [tol] * (P * C)
-        self.fwd = EikonalGraph(self.u0s, fs, hs, tols, compute_f64=compute_f64)
-        self.adj = EikonalGraph(self.u0s, fs, hs, tols, grad_us=self.grad_us, forward=False,
-                                us=self.fwd.us)
+        self.step = self.fwd = self.adj = None
         if C == ns:
             for q, p in enumerate(self.phases):
                 O.source_u0(self.stations, self.vel0, self.h, self._u0_scale(p),
                             out=self.u0s[q * C:(q + 1) * C])
+            self.step = O.InversionStepGraph(
+                self.u0s, fs, hs, tols, *self.events,
+                [self.tobs[p][s] for p in self.phases for s in range(ns)],
+                [self.weights[p][s] for p in self.phases for s in range(ns)],
+                huber_delta=self.huber, compute_f64=compute_f64)
+        else:
+            self.grad_us = [torch.empty_like(self.x) for _ in range(P * C)]
+            self.fwd = EikonalGraph(self.u0s, fs, hs, tols, compute_f64=compute_f64)
+            self.adj = EikonalGraph(self.u0s, fs, hs, tols, grad_us=self.grad_us, forward=False,
+                                    us=self.fwd.us)
         self.nevals = 0
         self.last_loss = {}
 
@@ -148,6 +159,16 @@ class TravelTimeInversion:
         phase; returns the summed misfit per phase and leaves sum_s dL/df_s
         in self.grad_f."""
         C, ns = self.chunk, len(self.stations)
+        if self.step is not None:
+            # one launch: forward, misfit and adjoint of every station (the
+            # adjoint is computed also when only the loss is asked for)
+            self.step.launch()
+            loss = {}
+            for q, p in enumerate(self.phases):
+                loss[p] = float(self.step.losses[q * C:(q + 1) * C].cpu().numpy().sum())
+                if grad:
+                    O.sum_arrays(self.step.grad_fs[q * C:(q + 1) * C], out=self.grad_f[p])
+            return loss
         loss = {p: 0.0 for p in self.phases}
         for start in range(0, ns, C):
             k = min(C, ns - start)
@@ -200,7 +221,8 @@ class TravelTimeInversion:
     def travel_times(self, phase="P"):
         """Travel times of the last chunk's forward, one (m, n, l) tensor per slot."""
         q = self.phases.index(phase)
-        return self.fwd.us[q * self.chunk:(q + 1) * self.chunk]
+        us = self.step.us if self.step is not None else self.fwd.us
+        return us[q * self.chunk:(q + 1) * self.chunk]
 
     def save_checkpoint(self, path, params):
         params = np.asarray(params)
@@ -227,5 +249,6 @@ class TravelTimeInversion:
         return res.x, history
 
     def close(self):
-        self.fwd.close()
-        self.adj.close()
+        for g in (self.step, self.fwd, self.adj):
+            if g is not None:
+                g.close()

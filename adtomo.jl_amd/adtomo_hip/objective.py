@@ -23,7 +23,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
-from .ops import _sfx, _stream, _torch
+from .ops import _checked, _event_table, _flags, _sfx, _stream, _torch
 
 
 def _ptr(t):
@@ -91,6 +91,106 @@ def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None, h
             ay.ctypes.data_as(ctypes.c_void_p), az.ctypes.data_as(ctypes.c_void_p), nrec, _stream())
     L.check(rc, "misfit")
     return {"loss": loss, "tcal": tcals, "grad_u": grads}
+
+
+def _step_items(us, tobs, weights, grad_us, losses, huber_delta):
+    items = (L.AdtMisfit3d * max(len(us), 1))()
+    esz = losses.element_size()
+    for e, u in enumerate(us):
+        m, n, l = u.shape
+        items[e] = L.AdtMisfit3d(m, n, l, u.data_ptr(), tobs[e].data_ptr(), weights[e].data_ptr(),
+                                 None, grad_us[e].data_ptr(), losses.data_ptr() + e * esz,
+                                 float(huber_delta))
+    return items
+
+
+def _coords(rx, ry, rz):
+    return tuple(np.ascontiguousarray(a, dtype=np.float64) for a in (rx, ry, rz))
+
+
+def inversion_step(u0s, fs, hs, tols, rx, ry, rz, tobs, weights, huber_delta=0.0,
+                   compute_f64=False):
+    """One inversion step in one call (adt_eikonal3d_misfit_batched_*): for
+    each solve its forward, the receiver misfit of its travel times (its
+    gradient becomes the adjoint source) and the adjoint.  Arguments as for
+    ``InversionStepGraph``.  Returns dict(u, grad_u, grad_u0, grad_f, loss,
+    info)."""
+    torch = _torch()
+    n = len(fs)
+    us = [torch.empty_like(f) for f in fs]
+    gus = [torch.empty_like(f) for f in fs]
+    gu0s = [torch.empty_like(f) for f in fs]
+    gfs = [torch.empty_like(f) for f in fs]
+    losses = torch.zeros(max(n, 1), dtype=fs[0].dtype, device=fs[0].device)
+    _check_dev(*tobs, *weights)
+    tab, dtype = _event_table(u0s, fs, hs, tols, gus, us, gu0s, gfs)
+    items = _step_items(us, tobs, weights, gus, losses, huber_delta)
+    ax, ay, az = _coords(rx, ry, rz)
+    infos = (L.AdtInfo * max(n, 1))()
+    rc = getattr(L.lib(), "adt_eikonal3d_misfit_batched" + _sfx(dtype))(
+        ctypes.byref(tab), ctypes.byref(items), n, ax.ctypes.data_as(ctypes.c_void_p),
+        ay.ctypes.data_as(ctypes.c_void_p), az.ctypes.data_as(ctypes.c_void_p), len(ax),
+        _flags(True, gus, compute_f64), ctypes.byref(infos), _stream())
+    L.check(rc, "adt_eikonal3d_misfit_batched")
+    return {"u": us, "grad_u": gus, "grad_u0": gu0s, "grad_f": gfs, "loss": losses[:n],
+            "info": [_checked(infos[e].as_dict()) for e in range(n)]}
+
+
+class InversionStepGraph:
+    """hipGraph of one inversion step over fixed buffers
+    (adt_graph_create_misfit): for each solve its forward, the receiver
+    misfit of its travel times (scripts/inversion.jl:64-105; its gradient
+    becomes the adjoint source) and the adjoint, in one launch.
+
+    ``u0s``/``fs``/``hs``/``tols`` as for ``EikonalGraph``; ``rx, ry, rz``
+    host event coordinates (1-based); ``tobs``/``weights`` per-solve
+    tensors.  After ``launch()``: ``losses`` (one per solve), ``us``,
+    ``grad_us`` (dL/du), ``grad_fs`` and ``grad_u0s``.  Every buffer is kept
+    alive by this object."""
+
+    def __init__(self, u0s, fs, hs, tols, rx, ry, rz, tobs, weights, huber_delta=0.0,
+                 compute_f64=False):
+        torch = _torch()
+        self.u0s, self.fs, self.tobs, self.weights = list(u0s), list(fs), list(tobs), list(weights)
+        self.nev = len(fs)
+        self.us = [torch.empty_like(f) for f in fs]
+        self.grad_us = [torch.empty_like(f) for f in fs]
+        self.grad_u0s = [torch.empty_like(f) for f in fs]
+        self.grad_fs = [torch.empty_like(f) for f in fs]
+        dt = fs[0].dtype
+        self.losses = torch.zeros(max(self.nev, 1), dtype=dt, device=fs[0].device)
+        _check_dev(*self.tobs, *self.weights)
+        tab, dtype = _event_table(self.u0s, self.fs, hs, tols, self.grad_us, self.us,
+                                  self.grad_u0s, self.grad_fs)
+        items = _step_items(self.us, self.tobs, self.weights, self.grad_us, self.losses,
+                            huber_delta)
+        ax, ay, az = _coords(rx, ry, rz)
+        flags = _flags(True, self.grad_us, compute_f64)
+        self._g = ctypes.c_void_p()
+        rc = L.lib().adt_graph_create_misfit(
+            ctypes.byref(tab), ctypes.byref(items), self.nev, ax.ctypes.data_as(ctypes.c_void_p),
+            ay.ctypes.data_as(ctypes.c_void_p), az.ctypes.data_as(ctypes.c_void_p), len(ax), flags,
+            1 if str(dtype).endswith("float32") else 0, ctypes.byref(self._g), _stream())
+        L.check(rc, "adt_graph_create_misfit")
+
+    def launch(self):
+        L.check(L.lib().adt_graph_launch(self._g, _stream()), "adt_graph_launch")
+
+    def infos(self):
+        arr = (L.AdtInfo * max(self.nev, 1))()
+        L.check(L.lib().adt_graph_infos(self._g, ctypes.byref(arr), _stream()), "adt_graph_infos")
+        return [_checked(arr[e].as_dict()) for e in range(self.nev)]
+
+    def close(self):
+        if self._g:
+            L.lib().adt_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def model(x, vel0, slowness=True, out=None):

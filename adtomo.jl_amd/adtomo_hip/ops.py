@@ -416,7 +416,7 @@ def get_tune(dtype="float64"):
     return t.as_dict()
 
 
-def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse=None):
+def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse_max_n=None):
     """Change the launch setting (unspecified fields keep their value)."""
     t = L.AdtTune()
     L.check(L.lib().adt_get_tune(_dtype_code(dtype), ctypes.byref(t)), "get_tune")
@@ -426,8 +426,8 @@ def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse=Non
         t.adj_wg = int(adj_wg)
     if order is not None:
         t.order = int(order)
-    if fuse is not None:
-        t.fuse = int(fuse)
+    if fuse_max_n is not None:
+        t.fuse_max_n = int(fuse_max_n)
     L.check(L.lib().adt_set_tune(_dtype_code(dtype), ctypes.byref(t)), "set_tune")
     return t.as_dict()
 
@@ -435,8 +435,8 @@ def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse=Non
 def autotune(u0s, fs, hs, tols, grad_us=None, reps=3, compute_f64=False):
     """Time the launch settings on these events and keep the fastest for
     their precision (forward kernel split, adjoint workgroup size, event
-    order, forward and adjoint fused in one kernel or not).  Returns the
-    chosen setting with its measured times."""
+    order, largest event whose forward and adjoint are fused in one kernel).
+    Returns the chosen setting with its measured times."""
     torch = _torch()
     us = [torch.empty_like(f) for f in fs]
     gu0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None

@@ -67,7 +67,9 @@ struct DevBuf {
 // [1] FP32), read at each launch / graph creation and written by
 // adt_set_tune / adt_autotune under a mutex.
 std::mutex g_tune_mu;
-adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, 1, 0.f, 0.f}, {adt::kDiagAll, 1024, 1, 1, 0.f, 0.f}};
+constexpr int32_t kFuseMaxDefault = 1 << 21;
+adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f},
+                      {adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f}};
 
 template <typename T>
 constexpr int tune_index() {
@@ -81,7 +83,7 @@ adt_tune current_tune() {
 bool tune_valid(const adt_tune& t) {
   return t.diag_max_l >= 0 && t.diag_max_l <= adt::kDiagAll &&
          (t.adj_wg == 256 || t.adj_wg == 512 || t.adj_wg == 1024) && (t.order == 0 || t.order == 1) &&
-         (t.fuse == 0 || t.fuse == 1);
+         t.fuse_max_n >= 0;
 }
 
 int check_dims3(int64_t m, int64_t n, int64_t l) {
@@ -576,12 +578,101 @@ int bwd2_strided_impl(const T* g, const T* u, const T* f, int64_t m, int64_t n, 
 // -------------------------------------------------------------- batched ---
 
 // Builds the device descriptor table and workspace for `nev` events.
+// Receiver misfit attached to a batched forward + adjoint (host view).
+struct MisfitArgs {
+  const adt_misfit3d* items;
+  const double *rx, *ry, *rz;
+  int64_t nrec;
+};
+
+// Device view of the same, for the separate-kernel path.
+struct MisfitRun {
+  const adt_misfit3d* table;
+  const double *rx, *ry, *rz;
+  int nrec;
+};
+
+// Checks one misfit item against its event and the receivers against the
+// event's box (1-based coordinates within [1, extent]).
+int check_misfit_item(const adt_misfit3d& it, int i, const double* rx, const double* ry,
+                      const double* rz, int64_t nrec) {
+  int rc = check_dims3(it.m, it.n, it.l);
+  if (rc) return rc;
+  if (!it.u || !it.loss || (nrec > 0 && (!it.tobs || !it.weight)))
+    return fail(ADT_EINVAL, "misfit: NULL buffer for solve " + std::to_string(i));
+  for (int64_t j = 0; j < nrec; ++j) {
+    const double c[3] = {rx[j], ry[j], rz[j]};
+    const double ext[3] = {(double)it.m, (double)it.n, (double)it.l};
+    for (int d = 0; d < 3; ++d)
+      if (!(c[d] >= 1.0 && c[d] <= ext[d]))
+        return fail(ADT_EINVAL, "misfit: receiver " + std::to_string(j) +
+                                    " outside the 1-based grid of solve " + std::to_string(i));
+  }
+  return ADT_OK;
+}
+
 template <typename T>
 struct BatchPlan {
   std::vector<adt::EventDesc> descs;
   size_t ws_bytes = 0;
   size_t off_desc = 0, off_info = 0;
-  int max_l = 0;  // largest k extent of the batch
+  int max_l = 0;      // largest k extent of the batch
+  int64_t max_n = 0;  // largest node count of the batch
+  // attached receiver misfit (attach_misfit): table and coordinates in the
+  // workspace
+  const MisfitArgs* mis = nullptr;
+  size_t off_mis = 0, off_coords = 0;
+
+  // Each event's receiver misfit runs between its forward and its adjoint
+  // and writes its grad_u: the item must name the event's u and grad_u.
+  int attach_misfit(const adt_event3d* ev, int nev, const MisfitArgs* m) {
+    if (!m || (nev > 0 && !m->items) || m->nrec < 0 || m->nrec > (int64_t(1) << 30) ||
+        (m->nrec > 0 && (!m->rx || !m->ry || !m->rz)))
+      return fail(ADT_EINVAL, "misfit: bad arguments");
+    for (int e = 0; e < nev; ++e) {
+      const adt_misfit3d& it = m->items[e];
+      int rc = check_misfit_item(it, e, m->rx, m->ry, m->rz, m->nrec);
+      if (rc) return rc;
+      if (it.m != ev[e].m || it.n != ev[e].n || it.l != ev[e].l || it.u != ev[e].u ||
+          it.grad_u != ev[e].grad_u)
+        return fail(ADT_EINVAL, "misfit: item " + std::to_string(e) +
+                                    " must have its event's box, u and grad_u");
+    }
+    mis = m;
+    auto take = [&](size_t b) {
+      size_t o = ws_bytes;
+      ws_bytes += ((b + 255) / 256) * 256;
+      return o;
+    };
+    off_mis = take(sizeof(adt_misfit3d) * (nev ? nev : 1));
+    off_coords = take(sizeof(double) * 3 * (m->nrec ? m->nrec : 1));
+    for (int e = 0; e < nev; ++e) {
+      // relative until bind(); descs are still in input order here
+      descs[e].mis = (const adt_misfit3d*)(off_mis + sizeof(adt_misfit3d) * e);
+      descs[e].nrec = m->nrec;
+    }
+    return ADT_OK;
+  }
+  // Device copies of the misfit table and coordinates (after bind()).
+  int upload_misfit(char* base, int nev, hipStream_t s) {
+    if (!mis) return ADT_OK;
+    ADT_HIP(hipMemcpyAsync(base + off_mis, mis->items, sizeof(adt_misfit3d) * nev,
+                           hipMemcpyHostToDevice, s));
+    double* dc = (double*)(base + off_coords);
+    const size_t nb = sizeof(double) * mis->nrec;
+    if (nb) {
+      ADT_HIP(hipMemcpyAsync(dc, mis->rx, nb, hipMemcpyHostToDevice, s));
+      ADT_HIP(hipMemcpyAsync(dc + mis->nrec, mis->ry, nb, hipMemcpyHostToDevice, s));
+      ADT_HIP(hipMemcpyAsync(dc + 2 * mis->nrec, mis->rz, nb, hipMemcpyHostToDevice, s));
+    }
+    return ADT_OK;
+  }
+  MisfitRun misfit_run(char* base) const {
+    if (!mis) return MisfitRun{nullptr, nullptr, nullptr, nullptr, 0};
+    const double* dc = (const double*)(base + off_coords);
+    return MisfitRun{(const adt_misfit3d*)(base + off_mis), dc, dc + mis->nrec,
+                     dc + 2 * mis->nrec, (int)mis->nrec};
+  }
 
   int build(const adt_event3d* ev, int nev, int flags) {
     if (nev < 0) return fail(ADT_EINVAL, "batched: nev < 0");
@@ -606,6 +697,7 @@ struct BatchPlan {
       if (N * sizeof(T) >= kMaxArrayBytes)
         return fail(ADT_EINVAL, "event " + std::to_string(e) + ": arrays of 4 GiB or more are not supported");
       if (x.l > max_l) max_l = (int)x.l;
+      if ((int64_t)N > max_n) max_n = (int64_t)N;
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
       o_lam[e] = (flags & ADT_ADJOINT) ? take(N * adt::kAdjRecBytes<T>) : 0;  // per-node records
       o_code[e] = (flags & ADT_ADJOINT) ? take(2 * N) : 0;              // 16-bit node states
@@ -656,18 +748,27 @@ struct BatchPlan {
       d.indeg = nullptr;
       d.queue = (int32_t*)(base + (size_t)d.queue);
       d.info = (adt_info*)(base + (size_t)d.info);
+      if (mis) {
+        const double* dc = (const double*)(base + off_coords);
+        d.mis = (const adt_misfit3d*)(base + (size_t)d.mis);
+        d.rx = dc;
+        d.ry = dc + mis->nrec;
+        d.rz = dc + 2 * mis->nrec;
+      }
     }
   }
 };
 
 template <typename T>
-int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, const adt_tune& tn,
-                 adt_info* infos, hipStream_t s) {
+int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, int64_t max_n,
+                 const adt_tune& tn, adt_info* infos, hipStream_t s,
+                 const MisfitRun& mr = MisfitRun{}) {
   if (nev == 0) return ADT_OK;
   adt::info_clear_kernel<<<1, 256, 0, s>>>(infos, nev);
   ADT_HIP(hipGetLastError());
   // forward and adjoint of each event in one kernel (every event diagonal)
-  if (tn.fuse && (flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && max_l <= tn.diag_max_l) {
+  if (max_n <= tn.fuse_max_n && (flags & ADT_FORWARD) && (flags & ADT_ADJOINT) &&
+      max_l <= tn.diag_max_l) {
     const bool mix = flags & ADT_F32_COMPUTE_F64;
     if constexpr (std::is_same<T, float>::value) {
       if (mix) {
@@ -692,6 +793,10 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, con
       ADT_HIP(hipGetLastError());
     }
   }
+  if (mr.table) {  // the events' receiver misfit: its gradient is the adjoint source
+    adt::misfit_kernel<T><<<nev, 256, 0, s>>>(mr.table, mr.rx, mr.ry, mr.rz, mr.nrec);
+    ADT_HIP(hipGetLastError());
+  }
   if (flags & ADT_ADJOINT) {
     adt::adj3_kahn_kernel<T><<<nev, tn.adj_wg, 0, s>>>(ddesc);
     ADT_HIP(hipGetLastError());
@@ -700,13 +805,17 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, con
 }
 
 template <typename T>
-int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, void* stream) {
+int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, void* stream,
+                 const MisfitArgs* mis = nullptr) {
   if (!(flags & (ADT_FORWARD | ADT_ADJOINT))) return fail(ADT_EINVAL, "batched: no ADT_FORWARD/ADT_ADJOINT flag");
   if (flags & ADT_HOST_POINTERS) return fail(ADT_EINVAL, "batched: device pointers only");
+  if (mis && !((flags & ADT_FORWARD) && (flags & ADT_ADJOINT)))
+    return fail(ADT_EINVAL, "misfit batched: needs ADT_FORWARD and ADT_ADJOINT");
   hipStream_t s = (hipStream_t)stream;
   BatchPlan<T> plan;
   int rc = plan.build(ev, nev, flags);
   if (rc) return rc;
+  if (mis && (rc = plan.attach_misfit(ev, nev, mis))) return rc;
   if (nev == 0) return ADT_OK;
   const adt_tune tn = current_tune<T>();
   if (tn.order) plan.order_largest_first();
@@ -716,9 +825,14 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   char* base = (char*)ws.p;
   ADT_HIP(hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
                          hipMemcpyHostToDevice, s));
+  if ((rc = plan.upload_misfit(base, nev, s))) return rc;
   if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags,
-                            plan.max_l, tn, (adt_info*)(base + plan.off_info), s)))
+                            plan.max_l, plan.max_n, tn, (adt_info*)(base + plan.off_info), s,
+                            plan.misfit_run(base))))
     return rc;
+  // the misfit table and coordinates were copied from the caller's host
+  // memory: wait before returning
+  if (mis && !infos) ADT_HIP(hipStreamSynchronize(s));
   if (infos) {
     ADT_HIP(hipMemcpyAsync(infos, base + plan.off_info, sizeof(adt_info) * nev,
                            hipMemcpyDeviceToHost, s));
@@ -901,11 +1015,11 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     }
     // time `reps` launches of one setting after one untimed launch
     auto time_it = [&](int fl, const adt_tune& t, float* ms) -> int {
-      int r = launch_batch<T>(dd, nev, fl, plan.max_l, t, (adt_info*)(base + plan.off_info), s);
+      int r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, t, (adt_info*)(base + plan.off_info), s);
       if (r) return r;
       ADT_HIP(hipEventRecord(e0, s));
       for (int k = 0; k < reps; ++k)
-        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, t, (adt_info*)(base + plan.off_info), s)))
+        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, t, (adt_info*)(base + plan.off_info), s)))
           return r;
       ADT_HIP(hipEventRecord(e1, s));
       ADT_HIP(hipEventSynchronize(e1));
@@ -916,7 +1030,7 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     adt_tune t = cur;
     t.order = order;
     float fbest = 0.f, abest = 0.f;
-    t.fuse = 0;  // the two kernels are timed apart first
+    t.fuse_max_n = 0;  // the two kernels are timed apart first
     if (flags & ADT_FORWARD) {
       fbest = 3.4e38f;
       for (int dml : dmls) {
@@ -931,7 +1045,7 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
       }
       if (rc) break;
       // leave this order's best forward in the outputs for the adjoint
-      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), plan.max_l, t,
+      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), plan.max_l, plan.max_n, t,
                                 (adt_info*)(base + plan.off_info), s)))
         break;
     }
@@ -951,15 +1065,20 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     }
     // both kernels requested and every event diagonal: the fused kernel
     // against the best pair of separate ones
+    // (the threshold moves to include this batch's largest event if fusing
+    // wins, below it if not, and stays as it was when fusion does not apply)
     float total = fbest + abest;
+    t.fuse_max_n = cur.fuse_max_n;
     if ((flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && plan.max_l <= t.diag_max_l) {
       adt_tune c = t;
-      c.fuse = 1;
+      c.fuse_max_n = (int32_t)plan.max_n;
       float ms = 0.f;
       if ((rc = time_it(flags & (ADT_FORWARD | ADT_ADJOINT | ADT_F32_COMPUTE_F64), c, &ms))) break;
       if (ms < total) {
         total = ms;
-        t.fuse = 1;
+        t.fuse_max_n = std::max(cur.fuse_max_n, (int32_t)plan.max_n);
+      } else {
+        t.fuse_max_n = std::min(cur.fuse_max_n, (int32_t)(plan.max_n - 1));
       }
     }
     if (total < win_total) {
@@ -992,11 +1111,15 @@ struct GraphBase {
 };
 
 template <typename T>
-int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out, hipStream_t s) {
+int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out, hipStream_t s,
+                      const MisfitArgs* mis = nullptr) {
   if (!(flags & (ADT_FORWARD | ADT_ADJOINT))) return fail(ADT_EINVAL, "graph: no ADT_FORWARD/ADT_ADJOINT flag");
+  if (mis && !((flags & ADT_FORWARD) && (flags & ADT_ADJOINT)))
+    return fail(ADT_EINVAL, "misfit graph: needs ADT_FORWARD and ADT_ADJOINT");
   BatchPlan<T> plan;
   int rc = plan.build(ev, nev, flags);
   if (rc) return rc;
+  if (mis && (rc = plan.attach_misfit(ev, nev, mis))) return rc;
   const adt_tune tn = current_tune<T>();
   if (tn.order) plan.order_largest_first();
   GraphBase* g = new GraphBase();
@@ -1011,6 +1134,11 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   char* base = (char*)g->ws;
   e = hipMemcpy(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
                 hipMemcpyHostToDevice);
+  if (e == hipSuccess && plan.mis) {
+    rc = plan.upload_misfit(base, nev, nullptr);
+    if (rc == ADT_OK) e = hipDeviceSynchronize();
+    else e = hipErrorUnknown;
+  }
   if (e != hipSuccess) {
     (void)hipFree(g->ws);
     delete g;
@@ -1020,8 +1148,8 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, tn,
-                         (adt_info*)(base + plan.off_info), cs);
+    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, plan.max_n, tn,
+                         (adt_info*)(base + plan.off_info), cs, plan.misfit_run(base));
     hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
     if (rc == ADT_OK && e2 != hipSuccess) e = e2;
   }
@@ -1059,7 +1187,7 @@ int adt_get_tune(int32_t dtype, adt_tune* out) {
 int adt_set_tune(int32_t dtype, const adt_tune* t) {
   if ((dtype != 0 && dtype != 1) || !t) return fail(ADT_EINVAL, "set_tune: bad arguments");
   if (!tune_valid(*t))
-    return fail(ADT_EINVAL, "set_tune: diag_max_l must be in [0, 2^30], adj_wg 256/512/1024, order 0/1, fuse 0/1");
+    return fail(ADT_EINVAL, "set_tune: diag_max_l must be in [0, 2^30], adj_wg 256/512/1024, order 0/1, fuse_max_n >= 0");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[dtype] = *t;
   return ADT_OK;
@@ -1170,6 +1298,20 @@ int adt_eikonal3d_batched_f32(const adt_event3d* events, int32_t nev, int32_t fl
                               adt_info* infos, void* stream) {
   return batched_impl<float>(events, nev, flags, infos, stream);
 }
+int adt_eikonal3d_misfit_batched_f64(const adt_event3d* events, const adt_misfit3d* items,
+                                     int32_t nev, const double* rx, const double* ry,
+                                     const double* rz, int64_t nrec, int32_t flags,
+                                     adt_info* infos, void* stream) {
+  const MisfitArgs m{items, rx, ry, rz, nrec};
+  return batched_impl<double>(events, nev, flags, infos, stream, &m);
+}
+int adt_eikonal3d_misfit_batched_f32(const adt_event3d* events, const adt_misfit3d* items,
+                                     int32_t nev, const double* rx, const double* ry,
+                                     const double* rz, int64_t nrec, int32_t flags,
+                                     adt_info* infos, void* stream) {
+  const MisfitArgs m{items, rx, ry, rz, nrec};
+  return batched_impl<float>(events, nev, flags, infos, stream, &m);
+}
 int adt_eikonal3d_fwd_plain_f64(const double* u0, const double* f, double h, int64_t m,
                                 int64_t n, int64_t l, double tol, double* u, adt_info* info,
                                 void* stream) {
@@ -1188,6 +1330,20 @@ int adt_graph_create(const adt_event3d* events, int32_t nev, int32_t flags, int3
   GraphBase* g = nullptr;
   int rc = dtype == 0 ? graph_create_impl<double>(events, nev, flags, &g, (hipStream_t)stream)
                       : graph_create_impl<float>(events, nev, flags, &g, (hipStream_t)stream);
+  if (rc) return rc;
+  *out = new adt_graph{g};
+  return ADT_OK;
+}
+int adt_graph_create_misfit(const adt_event3d* events, const adt_misfit3d* items, int32_t nev,
+                            const double* rx, const double* ry, const double* rz, int64_t nrec,
+                            int32_t flags, int32_t dtype, adt_graph** out, void* stream) {
+  if (!out) return fail(ADT_EINVAL, "graph: NULL out");
+  if (flags & ADT_HOST_POINTERS) return fail(ADT_EINVAL, "graph: device pointers only");
+  if (dtype != 0 && dtype != 1) return fail(ADT_EINVAL, "graph: dtype must be 0 (FP64) or 1 (FP32)");
+  const MisfitArgs m{items, rx, ry, rz, nrec};
+  GraphBase* g = nullptr;
+  int rc = dtype == 0 ? graph_create_impl<double>(events, nev, flags, &g, (hipStream_t)stream, &m)
+                      : graph_create_impl<float>(events, nev, flags, &g, (hipStream_t)stream, &m);
   if (rc) return rc;
   *out = new adt_graph{g};
   return ADT_OK;
@@ -1259,19 +1415,8 @@ int misfit_impl(const adt_misfit3d* items, int32_t nitem, const double* rx, cons
     return fail(ADT_EINVAL, "misfit: bad arguments");
   if (nitem == 0) return ADT_OK;
   for (int i = 0; i < nitem; ++i) {
-    const adt_misfit3d& it = items[i];
-    int rc = check_dims3(it.m, it.n, it.l);
+    int rc = check_misfit_item(items[i], i, rx, ry, rz, nrec);
     if (rc) return rc;
-    if (!it.u || !it.loss || (nrec > 0 && (!it.tobs || !it.weight)))
-      return fail(ADT_EINVAL, "misfit: NULL buffer for solve " + std::to_string(i));
-    for (int64_t j = 0; j < nrec; ++j) {
-      const double c[3] = {rx[j], ry[j], rz[j]};
-      const double ext[3] = {(double)it.m, (double)it.n, (double)it.l};
-      for (int d = 0; d < 3; ++d)
-        if (!(c[d] >= 1.0 && c[d] <= ext[d]))
-          return fail(ADT_EINVAL, "misfit: receiver " + std::to_string(j) +
-                                      " outside the 1-based grid of solve " + std::to_string(i));
-    }
   }
   hipStream_t st = (hipStream_t)stream;
   DevBuf tab, coords;

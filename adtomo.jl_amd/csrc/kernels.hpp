@@ -26,6 +26,14 @@ struct EventDesc {
   void* grad_u0;
   void* grad_f;
   adt_info* info;  // device
+  // Optional (device, NULL if absent): the receiver misfit whose gradient
+  // becomes grad_u between the forward and the adjoint (fused kernel), and
+  // the receiver coordinates it reads.
+  const adt_misfit3d* mis;
+  const double* rx;
+  const double* ry;
+  const double* rz;
+  int64_t nrec;
 };
 
 // One 2D event.

@@ -43,6 +43,7 @@
 // snapshot u_old = u.
 #include "adjoint3.hpp"
 #include "buffer_ops.hpp"
+#include "misfit3.hpp"
 #include "eikonal_device.hpp"
 #include "kernels.hpp"
 
@@ -796,8 +797,10 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
   fwd3_diag_event<T, MIX>(evs[blockIdx.x], max_iter, max_l);
 }
 
-// Forward and then adjoint of one event per workgroup, for batches whose
-// every event takes the diagonal forward: an event whose forward converges
+// Forward and then adjoint of one event per workgroup (with the event's
+// receiver misfit in between when one is attached: its gradient is then
+// the adjoint source), for batches whose every event takes the diagonal
+// forward: an event whose forward converges
 // in fewer iterations starts its adjoint while others still sweep, so the
 // memory-bound traversal overlaps the latency-bound sweeps.  The adjoint
 // runs with kDW waves, four queue entries and four prep-pass nodes per
@@ -807,10 +810,18 @@ __global__ void __launch_bounds__(kDW * 64) fwd_adj3_diag_kernel(const EventDesc
                                                                  int max_iter, int max_l) {
   const EventDesc& ev = evs[blockIdx.x];
   fwd3_diag_event<T, MIX>(ev, max_iter, max_l);
-  // the adjoint reads the travel times the waves of this workgroup stored
+  // the next phase reads the travel times the waves of this workgroup stored
   __threadfence();
   __syncthreads();
   __threadfence();
+  if (ev.mis) {
+    // receiver misfit of these travel times: its gradient is the adjoint
+    // source (same for every thread of the workgroup)
+    misfit_event<T>(*ev.mis, ev.rx, ev.ry, ev.rz, (int)ev.nrec);
+    __threadfence();
+    __syncthreads();
+    __threadfence();
+  }
   adj3_kahn_event<T, 4, 4>(ev);
 }
 

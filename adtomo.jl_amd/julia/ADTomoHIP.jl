@@ -421,7 +421,7 @@ struct AdtTune
     diag_max_l::Int32
     adj_wg::Int32
     order::Int32
-    fuse::Int32
+    fuse_max_n::Int32
     fwd_ms::Float32
     adj_ms::Float32
 end

@@ -291,7 +291,7 @@ def main():
     roofs = [roof(FWD_KERNEL, fwd_gbs, fwd_ms), roof(ADJ_KERNEL, adj_gbs, adj_ms)]
     # with the fused setting the step is one kernel doing each event's
     # forward and then its adjoint: it is the step's dominant kernel
-    fused = bool(tune["fuse"]) and tune["diag_max_l"] >= shape[2]
+    fused = N <= tune["fuse_max_n"] and tune["diag_max_l"] >= shape[2]
     if fused:
         roofs.insert(0, roof(FUSED_KERNEL, total_gbs, ms))
     dominant = roofs[0] if fused else max(roofs, key=lambda r: r["launch_ms"])
@@ -313,7 +313,7 @@ def main():
                    "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
                    "serial_depth_per_sweep": int(sum(shape) - 2),
                    "adj_sweeps_mean": float(sweeps.mean()),
-                   "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order", "fuse")}},
+                   "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n")}},
         "roofline": dominant,
         "roofline_kernels": roofs,
         "cpu_baseline": cpu,
@@ -398,7 +398,7 @@ def run_uneven(args):
           {"workload": "256 events, boxes uniform in 48x48x24..128x128x64, h=0.5 km, layered "
                        "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
            "events": nev, "nodes_total": nodes, "K_mean": float(K64.mean()),
-           "launch_setting": {k: tune64[k] for k in ("diag_max_l", "adj_wg", "order", "fuse")}},
+           "launch_setting": {k: tune64[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n")}},
           fp32=variant("f32"), fp32_storage_fp64_solve=variant("mixed"))
 
 

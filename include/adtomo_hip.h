@@ -207,13 +207,16 @@ typedef struct adt_tune {
                          64; default 2^30 = all), the others k-streaming */
   int32_t adj_wg;     /* adjoint workgroup size: 256, 512 or 1024 */
   int32_t order;      /* 0: events in the given order; 1: largest first */
-  int32_t fuse;       /* 1: a call or graph with both ADT_FORWARD and
-                         ADT_ADJOINT whose every event takes the diagonal
-                         forward runs one kernel that does each event's
-                         forward and then its adjoint (an event's adjoint
-                         starts as soon as its forward converges; adj_wg
-                         does not apply); 0: separate forward and adjoint
-                         kernels */
+  int32_t fuse_max_n; /* a call or graph with both ADT_FORWARD and
+                          ADT_ADJOINT whose every event takes the diagonal
+                          forward and has at most fuse_max_n nodes runs one
+                          kernel that does each event's forward and then its
+                          adjoint (an event's adjoint starts as soon as its
+                          forward converges; adj_wg does not apply); larger
+                          events, or 0, use separate forward and adjoint
+                          kernels.  Default 2^21: the fused adjoint runs on
+                          the forward's 8 waves, which pays off for events
+                          of about 10^6 nodes and costs on much larger ones */
   float fwd_ms;       /* adt_autotune: measured forward / adjoint time per */
   float adj_ms;       /* launch of the chosen setting (ms) */
 } adt_tune;
@@ -270,6 +273,31 @@ int adt_misfit3d_batched_f64(const adt_misfit3d *items, int32_t nitem, const dou
                              const double *ry, const double *rz, int64_t nrec, void *stream);
 int adt_misfit3d_batched_f32(const adt_misfit3d *items, int32_t nitem, const double *rx,
                              const double *ry, const double *rz, int64_t nrec, void *stream);
+
+/* ---- One inversion step per launch: for each event its forward, then the
+ * receiver misfit of its travel times (items[e], as adt_misfit3d_batched_*;
+ * items[e] must name events[e]'s box, u and grad_u), then the adjoint with
+ * that misfit gradient as the adjoint source (scripts/inversion.jl:47-105,
+ * the forward, loss and gradient of each station).  flags must hold
+ * ADT_FORWARD and ADT_ADJOINT (plus ADT_F32_COMPUTE_F64 for _f32).  With
+ * the fused launch setting and every event on the diagonal forward, one
+ * kernel runs the three phases per event; otherwise three launches run in
+ * stream order.  The receiver coordinates are host arrays; the call waits
+ * for their copy.  The reference has no single counterpart (its op, the
+ * loss and the gradient op run as separate graph nodes). */
+int adt_eikonal3d_misfit_batched_f64(const adt_event3d *events, const adt_misfit3d *items,
+                                     int32_t nev, const double *rx, const double *ry,
+                                     const double *rz, int64_t nrec, int32_t flags,
+                                     adt_info *infos, void *stream);
+int adt_eikonal3d_misfit_batched_f32(const adt_event3d *events, const adt_misfit3d *items,
+                                     int32_t nev, const double *rx, const double *ry,
+                                     const double *rz, int64_t nrec, int32_t flags,
+                                     adt_info *infos, void *stream);
+/* hipGraph of the same step (as adt_graph_create); the items and the
+ * receiver coordinates are copied into the graph's workspace at creation. */
+int adt_graph_create_misfit(const adt_event3d *events, const adt_misfit3d *items, int32_t nev,
+                            const double *rx, const double *ry, const double *rz, int64_t nrec,
+                            int32_t flags, int32_t dtype, adt_graph **out, void *stream);
 
 /* Model parametrisation (scripts/inversion.jl:42-43,61): fvar = 2 sigmoid(x)
  * - 1 + vel0, f = 1 / fvar (f may be NULL). */

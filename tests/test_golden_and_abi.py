@@ -133,18 +133,19 @@ def test_tune_get_set_validation_without_device():
     base = A.get_tune("float64")
     try:
         assert base["diag_max_l"] == 1 << 30 and base["adj_wg"] in (256, 512, 1024)
-        assert base["fuse"] == 1
-        t = A.set_tune("float64", diag_max_l=0, adj_wg=256, order=0, fuse=0)
-        assert A.get_tune("float64")["adj_wg"] == 256 and t["diag_max_l"] == 0 and t["fuse"] == 0
+        assert base["fuse_max_n"] == 1 << 21
+        t = A.set_tune("float64", diag_max_l=0, adj_wg=256, order=0, fuse_max_n=0)
+        assert A.get_tune("float64")["adj_wg"] == 256 and t["diag_max_l"] == 0
+        assert t["fuse_max_n"] == 0
         assert A.get_tune("float32")["adj_wg"] == base["adj_wg"]  # per precision
-        assert A.get_tune("float32")["fuse"] == base["fuse"]
+        assert A.get_tune("float32")["fuse_max_n"] == base["fuse_max_n"]
         for bad in ({"diag_max_l": (1 << 30) + 1}, {"diag_max_l": -1}, {"adj_wg": 300},
-                    {"order": 2}, {"fuse": 2}, {"fuse": -1}):
+                    {"order": 2}, {"fuse_max_n": -1}):
             with pytest.raises(A.AdtError):
                 A.set_tune("float64", **bad)
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"], fuse=base["fuse"])
+                   order=base["order"], fuse_max_n=base["fuse_max_n"])
 
 
 def _c_struct_fields(name):
@@ -216,3 +217,19 @@ def test_objective_and_tuning_argument_errors_without_device():
     assert lib.adt_autotune(ev, 1, L.ADT_FORWARD, 2, 1, ctypes.byref(best), None) == -1
     g = ctypes.c_void_p()
     assert lib.adt_graph_create(ev, 1, 0, 0, ctypes.byref(g), None) == -1
+    # the one-launch inversion step: both flags, and items that name their
+    # event's box, u and grad_u
+    evs = (L.AdtEvent3d * 1)(L.AdtEvent3d(4, 4, 4, 1.0, 1e-3, p, p, p, p, None, p))
+    good = L.AdtMisfit3d(4, 4, 4, p, p, p, None, p, p, 0.0)
+    other_u = L.AdtMisfit3d(4, 4, 4, p + 8, p, p, None, p, p, 0.0)
+    xyz = np.array([2.0, 2.0, 2.0]).ctypes.data
+    both = L.ADT_FORWARD | L.ADT_ADJOINT
+    assert lib.adt_eikonal3d_misfit_batched_f64(evs, (L.AdtMisfit3d * 1)(good), 1, xyz, xyz, xyz,
+                                                1, L.ADT_FORWARD, None, None) == -1
+    assert lib.adt_eikonal3d_misfit_batched_f64(evs, (L.AdtMisfit3d * 1)(other_u), 1, xyz, xyz,
+                                                xyz, 1, both, None, None) == -1
+    assert b"u and grad_u" in lib.adt_last_error()
+    far = np.array([9.0]).ctypes.data
+    assert lib.adt_graph_create_misfit(evs, (L.AdtMisfit3d * 1)(good), 1, xyz, xyz, far, 1, both,
+                                       0, ctypes.byref(g), None) == -1
+    assert b"outside" in lib.adt_last_error()

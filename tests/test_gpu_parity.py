@@ -545,12 +545,12 @@ def _mixed_batch(seed=300):
 
 
 @pytest.mark.parametrize("setting", [
-    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1, "fuse": 1},
-    {"diag_max_l": 1 << 30, "adj_wg": 512, "order": 0, "fuse": 0},
-    {"diag_max_l": 64, "adj_wg": 1024, "order": 1, "fuse": 1},
-    {"diag_max_l": 0, "adj_wg": 256, "order": 0, "fuse": 1},
-    {"diag_max_l": 32, "adj_wg": 512, "order": 1, "fuse": 0},
-    {"diag_max_l": 7, "adj_wg": 256, "order": 1, "fuse": 1},
+    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30},
+    {"diag_max_l": 1 << 30, "adj_wg": 512, "order": 0, "fuse_max_n": 0},
+    {"diag_max_l": 64, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30},
+    {"diag_max_l": 0, "adj_wg": 256, "order": 0, "fuse_max_n": 1 << 30},
+    {"diag_max_l": 32, "adj_wg": 512, "order": 1, "fuse_max_n": 0},
+    {"diag_max_l": 7, "adj_wg": 256, "order": 1, "fuse_max_n": 1 << 30},
 ])
 def test_every_tune_setting_matches_oracle(setting):
     import adtomo_hip as A
@@ -576,7 +576,7 @@ def test_every_tune_setting_matches_oracle(setting):
         G.close()
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"], fuse=base["fuse"])
+                   order=base["order"], fuse_max_n=base["fuse_max_n"])
 
 
 @pytest.mark.parametrize("dtype,compute_f64", [("float64", False), ("float32", False),
@@ -594,7 +594,7 @@ def test_fused_forward_adjoint_matches_separate_kernels(dtype, compute_f64):
     out = {}
     try:
         for fuse in (1, 0):
-            A.set_tune(dtype, diag_max_l=1 << 30, fuse=fuse)
+            A.set_tune(dtype, diag_max_l=1 << 30, fuse_max_n=(1 << 30) if fuse else 0)
             G = A.EikonalGraph(*args, grad_us=[dev(a, npd) for a in gs], compute_f64=compute_f64)
             G.launch()
             G.launch()
@@ -603,7 +603,7 @@ def test_fused_forward_adjoint_matches_separate_kernels(dtype, compute_f64):
                          [host(g) for g in G.grad_u0s], [i["iterations"] for i in G.infos()])
             G.close()
     finally:
-        A.set_tune(dtype, diag_max_l=base["diag_max_l"], fuse=base["fuse"])
+        A.set_tune(dtype, diag_max_l=base["diag_max_l"], fuse_max_n=base["fuse_max_n"])
     for k in range(3):
         for a, b in zip(out[1][k], out[0][k]):
             assert bits_equal(a, b), (dtype, compute_f64, k)
@@ -620,16 +620,16 @@ def test_autotune_stores_a_valid_setting():
         print("autotune:", best, flush=True)
         assert best["diag_max_l"] in (0, 64, 1 << 30) and best["adj_wg"] in (256, 512, 1024)
         assert best["order"] in (0, 1) and best["fwd_ms"] > 0 and best["adj_ms"] > 0
-        assert best["fuse"] in (0, 1)
+        assert best["fuse_max_n"] >= 0
         cur = A.get_tune("float64")
-        assert all(cur[k] == best[k] for k in ("diag_max_l", "adj_wg", "order", "fuse"))
+        assert all(cur[k] == best[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n"))
         res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
                                   [1e-3] * len(fs))
         for e in range(len(fs)):
             assert bits_equal(host(res["u"][e]), oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-3)[0])
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"], fuse=base["fuse"])
+                   order=base["order"], fuse_max_n=base["fuse_max_n"])
 
 
 # ------------------------------------------- FP32 storage, FP64 solves ---

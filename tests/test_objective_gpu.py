@@ -331,3 +331,59 @@ def test_joint_inversion_objective_matches_oracle(chunk):
             assert abs(g[-1] - Rg[-1]) <= 1e-9 * abs(Rg[-1])
     finally:
         inv.close()
+
+
+# --------------------------------------- inversion step in one launch ---
+
+@pytest.mark.parametrize("fuse,dtype", [(1, "float64"), (0, "float64"), (1, "float32")])
+def test_inversion_step_matches_separate_launches(fuse, dtype):
+    # forward, misfit and adjoint of each station in one launch (graph and
+    # single call; fused kernel or three kernels) against the forward graph,
+    # the misfit call and the adjoint graph on the same inputs: travel times
+    # and losses bit for bit, dL/du and dL/df to rounding (dL/du is summed
+    # with atomics); float32 runs FP32 storage with FP64 solves
+    import adtomo_hip as A
+    from adtomo_hip import objective as O
+    torch = _torch()
+    vel0, h, stations, ev, tobs, w = _inversion_case(seed=50, ns=4, ne=30)
+    n = len(stations)
+    npd = np.float64 if dtype == "float64" else np.float32
+    mix = dtype == "float32"
+    tol_g = 1e-12 if dtype == "float64" else 1e-5
+    base = A.get_tune(dtype)
+    graphs = []
+    try:
+        A.set_tune(dtype, fuse_max_n=(1 << 30) if fuse else 0)
+        f = dev(1.0 / vel0, npd)
+        u0s = O.source_u0(stations, dev(vel0, npd), h)
+        tb = [dev(tobs[s], npd) for s in range(n)]
+        wb = [dev(w[s], npd) for s in range(n)]
+        args = (u0s, [f] * n, [h] * n, [1e-3] * n)
+        step = O.InversionStepGraph(*args, *ev, tb, wb, huber_delta=0.05, compute_f64=mix)
+        graphs.append(step)
+        step.launch()
+        step.launch()
+        one = O.inversion_step(*args, *ev, tb, wb, huber_delta=0.05, compute_f64=mix)
+        fwd = A.EikonalGraph(*args, compute_f64=mix)
+        graphs.append(fwd)
+        fwd.launch()
+        gus = [torch.empty_like(f) for _ in range(n)]
+        mis = O.misfit(fwd.us, *ev, tb, wb, tcal=False, grad_out=gus, huber_delta=0.05)
+        adj = A.EikonalGraph(*args, grad_us=gus, forward=False, us=fwd.us)
+        graphs.append(adj)
+        adj.launch()
+        torch.cuda.synchronize()
+        for res_u, res_gu, res_gf, res_loss in (
+                (step.us, step.grad_us, step.grad_fs, step.losses[:n]),
+                (one["u"], one["grad_u"], one["grad_f"], one["loss"])):
+            for e in range(n):
+                assert np.array_equal(host(res_u[e]), host(fwd.us[e])), e
+                assert rel(host(res_gu[e]), host(gus[e])) <= tol_g, e
+                assert rel(host(res_gf[e]), host(adj.grad_fs[e])) <= 1e3 * tol_g, e
+            assert np.array_equal(host(res_loss), host(mis["loss"]))
+        assert all(i["converged"] and i["adj_converged"] for i in step.infos())
+        assert all(i["converged"] and i["adj_converged"] for i in one["info"])
+    finally:
+        A.set_tune(dtype, fuse_max_n=base["fuse_max_n"])
+        for g in graphs:
+            g.close()

@@ -22,7 +22,7 @@ using Libdl
 
 export eikonal, eikonal3d, eikonal3d_batched!, EikonalGraph, launch!, close!, source_u0!,
        misfit!, model!, model_grad!, slowness!, slowness_grad!, smooth_l1!, sum_arrays!,
-       AdtTune, get_tune, set_tune!, autotune!
+       AdtTune, get_tune, set_tune!, autotune!, eikonal3d_misfit_batched!
 
 const libadt = normpath(joinpath(@__DIR__, "..", "libadtomo_hip.so"))
 const ADT_HOST_POINTERS = Int32(0x1)
@@ -317,6 +317,38 @@ function misfit!(loss, us, rx, ry, rz, tobs, weights; grad_us=nothing, tcals=not
                length(rx), stream)
     check(rc, "misfit")
     return loss
+end
+
+"""
+    eikonal3d_misfit_batched!(us, loss, u0s, fs, hs, tols, grad_us, rx, ry, rz, tobs, weights;
+                              grad_u0s, grad_fs, huber_delta=0.0, compute_f64=false,
+                              stream=C_NULL)
+
+One inversion step in one launch (scripts/inversion.jl:47-105): for each
+event its forward, the receiver misfit of its travel times (`loss[e]`, its
+gradient written to `grad_us[e]` as the adjoint source) and the adjoint into
+`grad_fs` / `grad_u0s`.  Device arrays as for `eikonal3d_batched!`; `rx, ry,
+rz` host event coordinates (1-based); returns the per-event `AdtInfo`s.
+"""
+function eikonal3d_misfit_batched!(us, loss, u0s, fs, hs, tols, grad_us, rx, ry, rz, tobs,
+                                   weights; grad_u0s=nothing, grad_fs, huber_delta=0.0,
+                                   compute_f64::Bool=false, stream::Ptr{Cvoid}=C_NULL)
+    tab = event_table(u0s, fs, us, hs, tols, grad_us, grad_u0s, grad_fs)
+    T = eltype(fs[1])
+    lp = devptr(loss)
+    items = [AdtMisfit3d(griddims(us[e])..., devptr(us[e]), devptr(tobs[e]), devptr(weights[e]),
+                         C_NULL, devptr(grad_us[e]), lp + (e - 1) * sizeof(T),
+                         Float64(huber_delta)) for e in eachindex(us)]
+    flags = ADT_FORWARD | ADT_ADJOINT | (compute_f64 ? ADT_F32_COMPUTE_F64 : Int32(0))
+    infos = [AdtInfo() for _ in tab]
+    fn = Symbol("adt_eikonal3d_misfit_batched", sfx(fs[1]))
+    rc = ccall(Libdl.dlsym(Libdl.dlopen(libadt), fn), Cint,
+               (Ptr{AdtEvent3d}, Ptr{AdtMisfit3d}, Int32, Ptr{Float64}, Ptr{Float64},
+                Ptr{Float64}, Int64, Int32, Ptr{AdtInfo}, Ptr{Cvoid}),
+               tab, items, length(tab), Vector{Float64}(rx), Vector{Float64}(ry),
+               Vector{Float64}(rz), length(rx), flags, infos, stream)
+    check(rc, "eikonal3d_misfit_batched")
+    return infos
 end
 
 # ------------------------------------- model, regulariser, sum, tuning ---

@@ -260,6 +260,26 @@ def test_inversion_objective_matches_oracle():
         inv.close()
 
 
+@pytest.mark.parametrize("chunk", [None, 2])
+def test_inversion_objective_fp32_storage_fp64_solves(chunk):
+    # the precision of large grids: FP32 arrays with every forward local
+    # solve in FP64 (one launch per evaluation, or chunks of 2 stations),
+    # against the FP64 oracle composite; the bound is binary32 storage
+    # (about 6e-8 relative per value) carried through the sums
+    from adtomo_hip.inversion import TravelTimeInversion
+    vel0, h, stations, ev, tobs, w = _inversion_case(seed=60)
+    inv = TravelTimeInversion(vel0, h, stations, ev, tobs, w, smooth=(3, 3), lam=0.05,
+                              dtype="float32", compute_f64=True, chunk=chunk)
+    try:
+        x = np.random.default_rng(61).normal(scale=0.2, size=vel0.shape)
+        L, g = inv.objective(x.reshape(-1))
+        RL, Rg = _oracle_objective(x, vel0, h, stations, ev, tobs, w, 3, 3, 0.05)
+        assert abs(L - RL) <= 1e-5 * abs(RL), (L, RL)
+        assert rel(g, Rg.reshape(-1)) <= 1e-3, rel(g, Rg.reshape(-1))
+    finally:
+        inv.close()
+
+
 def test_inversion_run_decreases_loss_and_checkpoints(tmp_path):
     from adtomo_hip.inversion import TravelTimeInversion
     vel0, h, stations, ev, tobs, w = _inversion_case(seed=22)

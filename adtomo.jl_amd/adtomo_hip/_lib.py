@@ -78,7 +78,7 @@ class AdtMisfit3d(ctypes.Structure):
 
 class AdtTune(ctypes.Structure):
     _fields_ = [("diag_max_l", _i32), ("adj_wg", _i32), ("order", _i32), ("fuse_max_n", _i32),
-                ("fwd_ms", ctypes.c_float), ("adj_ms", ctypes.c_float)]
+                ("fwd_ms", ctypes.c_float), ("adj_ms", ctypes.c_float), ("adj_algo", _i32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -416,8 +416,10 @@ def get_tune(dtype="float64"):
     return t.as_dict()
 
 
-def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse_max_n=None):
-    """Change the launch setting (unspecified fields keep their value)."""
+def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse_max_n=None,
+             adj_algo=None):
+    """Change the launch setting (unspecified fields keep their value).
+    ``adj_algo``: 0 = adjoint over 8x8x8 tiles (default), 1 = node by node."""
     t = L.AdtTune()
     L.check(L.lib().adt_get_tune(_dtype_code(dtype), ctypes.byref(t)), "get_tune")
     if diag_max_l is not None:
@@ -428,6 +430,8 @@ def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse_max
         t.order = int(order)
     if fuse_max_n is not None:
         t.fuse_max_n = int(fuse_max_n)
+    if adj_algo is not None:
+        t.adj_algo = int(adj_algo)
     L.check(L.lib().adt_set_tune(_dtype_code(dtype), ctypes.byref(t)), "set_tune")
     return t.as_dict()
 

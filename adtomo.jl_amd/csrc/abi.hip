@@ -68,8 +68,8 @@ struct DevBuf {
 // adt_set_tune / adt_autotune under a mutex.
 std::mutex g_tune_mu;
 constexpr int32_t kFuseMaxDefault = 1 << 21;
-adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f},
-                      {adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f}};
+adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f, 0},
+                      {adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f, 0}};
 
 template <typename T>
 constexpr int tune_index() {
@@ -83,7 +83,56 @@ adt_tune current_tune() {
 bool tune_valid(const adt_tune& t) {
   return t.diag_max_l >= 0 && t.diag_max_l <= adt::kDiagAll &&
          (t.adj_wg == 256 || t.adj_wg == 512 || t.adj_wg == 1024) && (t.order == 0 || t.order == 1) &&
-         t.fuse_max_n >= 0;
+         t.fuse_max_n >= 0 && (t.adj_algo == 0 || t.adj_algo == 1);
+}
+
+// ------------------------------------------------------------- adjoint ---
+// Two adjoint traversals (adt_tune.adj_algo): over 8x8x8 tiles (0; events
+// of at most kTileLargeMax tiles) or node by node over the event (1).
+
+int64_t tiles_of(int64_t m, int64_t n, int64_t l) {
+  const int64_t e = adt::kTileEdge;
+  return ((m + e - 1) / e) * ((n + e - 1) / e) * ((l + e - 1) / e);
+}
+
+// The traversal an event set takes: the requested one, or the node-level
+// one when some event has more tiles than the tile kernels hold.
+int adj_algo_for(int requested, int64_t max_tiles) {
+  return (requested == 0 && max_tiles <= adt::kTileLargeMax) ? 0 : 1;
+}
+
+// Adjoint workspace of one event (bytes): the EventDesc lam, code and queue
+// buffers of the traversal.
+template <typename T>
+void adj_ws_bytes(int algo, int64_t N, int64_t tiles, size_t* lam, size_t* code, size_t* queue) {
+  if (algo == 0) {
+    *lam = sizeof(T) * size_t(N);  // multipliers
+    *code = size_t(N);             // upwind codes
+    *queue = 4 * size_t(tiles);    // external-edge count per tile
+  } else {
+    *lam = adt::kAdjRecBytes<T> * size_t(N);  // per-node records
+    *code = 2 * size_t(N);                    // 16-bit node states
+    *queue = 4 * size_t(N);                   // topological queue
+  }
+}
+
+template <typename T>
+void launch_adj(int algo, int64_t max_tiles, int nev, const adt::EventDesc* d, int adj_wg,
+                hipStream_t s) {
+  if (algo != 0) {
+    adt::adj3_kahn_kernel<T><<<nev, adj_wg, 0, s>>>(d);
+    return;
+  }
+  if (max_tiles <= adt::kTileSmallMax) {
+    adt::adj3_tile_kernel<T, adt::kTileWavesSmall, adt::kTileSmallMax>
+        <<<nev, adt::kTileWavesSmall * 64, 0, s>>>(d);
+  } else if (std::is_same<T, double>::value) {
+    adt::adj3_tile_kernel<T, adt::kTileWavesLarge64, adt::kTileLargeMax>
+        <<<nev, adt::kTileWavesLarge64 * 64, 0, s>>>(d);
+  } else {
+    adt::adj3_tile_kernel<T, adt::kTileWavesLarge32, adt::kTileLargeMax>
+        <<<nev, adt::kTileWavesLarge32 * 64, 0, s>>>(d);
+  }
 }
 
 int check_dims3(int64_t m, int64_t n, int64_t l) {
@@ -234,9 +283,14 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
       (rc = st.out(grad_u0, bytes, hp, &dgu0, s)) || (rc = st.out(grad_f, bytes, hp, &dgf, s)))
     return rc;
   DevBuf ws;
-  const size_t a = ((adt::kAdjRecBytes<T> * size_t(N) + 255) / 256) * 256;  // per-node records
-  const size_t c = ((size_t(N) * 2 + 255) / 256) * 256;  // 16-bit node states
-  const size_t q = ((size_t(N) * 4 + 255) / 256) * 256;
+  const int64_t tiles = tiles_of(m, n, l);
+  const adt_tune tn = current_tune<T>();
+  const int algo = adj_algo_for(tn.adj_algo, tiles);
+  size_t a, c, q;
+  adj_ws_bytes<T>(algo, N, tiles, &a, &c, &q);
+  a = ((a + 255) / 256) * 256;
+  c = ((c + 255) / 256) * 256;
+  q = ((q + 255) / 256) * 256;
   if ((rc = ws.alloc(a + c + q + 256 + sizeof(adt_info), s))) return rc;
   char* base = (char*)ws.p;
   adt::EventDesc d{};
@@ -258,7 +312,7 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   d.info = (adt_info*)(base + od + 256);
   ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
   adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
-  adt::adj3_kahn_kernel<T><<<1, current_tune<T>().adj_wg, 0, s>>>((const adt::EventDesc*)(base + od));
+  launch_adj<T>(algo, tiles, 1, (const adt::EventDesc*)(base + od), tn.adj_wg, s);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -618,6 +672,8 @@ struct BatchPlan {
   size_t off_desc = 0, off_info = 0;
   int max_l = 0;      // largest k extent of the batch
   int64_t max_n = 0;  // largest node count of the batch
+  int64_t max_tiles = 0;  // largest 8x8x8 tile count of the batch
+  int adj_algo = 0;       // adjoint traversal the workspace is laid out for
   // attached receiver misfit (attach_misfit): table and coordinates in the
   // workspace
   const MisfitArgs* mis = nullptr;
@@ -637,6 +693,16 @@ struct BatchPlan {
           it.grad_u != ev[e].grad_u)
         return fail(ADT_EINVAL, "misfit: item " + std::to_string(e) +
                                     " must have its event's box, u and grad_u");
+    }
+    // each event's misfit zeroes and accumulates its grad_u and reads its u
+    // while other workgroups run: two events must not share either buffer
+    for (int which = 0; which < 2; ++which) {
+      std::vector<const void*> ptrs(nev);
+      for (int e = 0; e < nev; ++e) ptrs[e] = which ? (const void*)ev[e].grad_u : ev[e].u;
+      std::sort(ptrs.begin(), ptrs.end());
+      if (std::adjacent_find(ptrs.begin(), ptrs.end()) != ptrs.end())
+        return fail(ADT_EINVAL, std::string("misfit: two events share one ") +
+                                    (which ? "grad_u" : "u") + " buffer");
     }
     mis = m;
     auto take = [&](size_t b) {
@@ -674,9 +740,14 @@ struct BatchPlan {
                      dc + 2 * mis->nrec, (int)mis->nrec};
   }
 
-  int build(const adt_event3d* ev, int nev, int flags) {
+  int build(const adt_event3d* ev, int nev, int flags, int adj_algo_req) {
     if (nev < 0) return fail(ADT_EINVAL, "batched: nev < 0");
     if (nev > 0 && !ev) return fail(ADT_EINVAL, "batched: NULL events");
+    for (int e = 0; e < nev; ++e) {
+      if (check_dims3(ev[e].m, ev[e].n, ev[e].l)) continue;  // reported below
+      max_tiles = std::max(max_tiles, tiles_of(ev[e].m, ev[e].n, ev[e].l));
+    }
+    adj_algo = adj_algo_for(adj_algo_req, max_tiles);
     descs.resize(nev);
     size_t off = 0;
     auto take = [&](size_t b) {
@@ -699,10 +770,12 @@ struct BatchPlan {
       if (x.l > max_l) max_l = (int)x.l;
       if ((int64_t)N > max_n) max_n = (int64_t)N;
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
-      o_lam[e] = (flags & ADT_ADJOINT) ? take(N * adt::kAdjRecBytes<T>) : 0;  // per-node records
-      o_code[e] = (flags & ADT_ADJOINT) ? take(2 * N) : 0;              // 16-bit node states
+      size_t wl, wc, wq;
+      adj_ws_bytes<T>(adj_algo, (int64_t)N, tiles_of(x.m, x.n, x.l), &wl, &wc, &wq);
+      o_lam[e] = (flags & ADT_ADJOINT) ? take(wl) : 0;
+      o_code[e] = (flags & ADT_ADJOINT) ? take(wc) : 0;
       o_indeg[e] = 0;
-      o_queue[e] = (flags & ADT_ADJOINT) ? take(4 * N) : 0;
+      o_queue[e] = (flags & ADT_ADJOINT) ? take(wq) : 0;
     }
     off_desc = take(sizeof(adt::EventDesc) * (nev ? nev : 1));
     off_info = take(sizeof(adt_info) * (nev ? nev : 1));
@@ -761,13 +834,13 @@ struct BatchPlan {
 
 template <typename T>
 int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, int64_t max_n,
-                 const adt_tune& tn, adt_info* infos, hipStream_t s,
+                 int adj_algo, int64_t max_tiles, const adt_tune& tn, adt_info* infos, hipStream_t s,
                  const MisfitRun& mr = MisfitRun{}) {
   if (nev == 0) return ADT_OK;
   adt::info_clear_kernel<<<1, 256, 0, s>>>(infos, nev);
   ADT_HIP(hipGetLastError());
   // forward and adjoint of each event in one kernel (every event diagonal)
-  if (max_n <= tn.fuse_max_n && (flags & ADT_FORWARD) && (flags & ADT_ADJOINT) &&
+  if (adj_algo == 1 && max_n <= tn.fuse_max_n && (flags & ADT_FORWARD) && (flags & ADT_ADJOINT) &&
       max_l <= tn.diag_max_l) {
     const bool mix = flags & ADT_F32_COMPUTE_F64;
     if constexpr (std::is_same<T, float>::value) {
@@ -798,7 +871,7 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, int
     ADT_HIP(hipGetLastError());
   }
   if (flags & ADT_ADJOINT) {
-    adt::adj3_kahn_kernel<T><<<nev, tn.adj_wg, 0, s>>>(ddesc);
+    launch_adj<T>(adj_algo, max_tiles, nev, ddesc, tn.adj_wg, s);
     ADT_HIP(hipGetLastError());
   }
   return ADT_OK;
@@ -812,12 +885,12 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   if (mis && !((flags & ADT_FORWARD) && (flags & ADT_ADJOINT)))
     return fail(ADT_EINVAL, "misfit batched: needs ADT_FORWARD and ADT_ADJOINT");
   hipStream_t s = (hipStream_t)stream;
+  const adt_tune tn = current_tune<T>();
   BatchPlan<T> plan;
-  int rc = plan.build(ev, nev, flags);
+  int rc = plan.build(ev, nev, flags, tn.adj_algo);
   if (rc) return rc;
   if (mis && (rc = plan.attach_misfit(ev, nev, mis))) return rc;
   if (nev == 0) return ADT_OK;
-  const adt_tune tn = current_tune<T>();
   if (tn.order) plan.order_largest_first();
   DevBuf ws;
   if ((rc = ws.alloc(plan.ws_bytes, s))) return rc;
@@ -827,7 +900,7 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
                          hipMemcpyHostToDevice, s));
   if ((rc = plan.upload_misfit(base, nev, s))) return rc;
   if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags,
-                            plan.max_l, plan.max_n, tn, (adt_info*)(base + plan.off_info), s,
+                            plan.max_l, plan.max_n, plan.adj_algo, plan.max_tiles, tn, (adt_info*)(base + plan.off_info), s,
                             plan.misfit_run(base))))
     return rc;
   // the misfit table and coordinates were copied from the caller's host
@@ -1000,7 +1073,7 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
   float win_total = 3.4e38f;
   for (int order = 0; order < 2 && rc == ADT_OK; ++order) {
     BatchPlan<T> plan;
-    if ((rc = plan.build(ev, nev, flags))) break;
+    if ((rc = plan.build(ev, nev, flags, cur.adj_algo))) break;
     if (order) plan.order_largest_first();
     DevBuf ws;
     if ((rc = ws.alloc(plan.ws_bytes, s))) break;
@@ -1015,11 +1088,11 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     }
     // time `reps` launches of one setting after one untimed launch
     auto time_it = [&](int fl, const adt_tune& t, float* ms) -> int {
-      int r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, t, (adt_info*)(base + plan.off_info), s);
+      int r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, plan.adj_algo, plan.max_tiles, t, (adt_info*)(base + plan.off_info), s);
       if (r) return r;
       ADT_HIP(hipEventRecord(e0, s));
       for (int k = 0; k < reps; ++k)
-        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, t, (adt_info*)(base + plan.off_info), s)))
+        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, plan.adj_algo, plan.max_tiles, t, (adt_info*)(base + plan.off_info), s)))
           return r;
       ADT_HIP(hipEventRecord(e1, s));
       ADT_HIP(hipEventSynchronize(e1));
@@ -1045,13 +1118,15 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
       }
       if (rc) break;
       // leave this order's best forward in the outputs for the adjoint
-      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), plan.max_l, plan.max_n, t,
+      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), plan.max_l, plan.max_n,
+                                plan.adj_algo, plan.max_tiles, t,
                                 (adt_info*)(base + plan.off_info), s)))
         break;
     }
     if (flags & ADT_ADJOINT) {
       abest = 3.4e38f;
       for (int wg : wgs) {
+        if (plan.adj_algo == 0 && wg != wgs[0]) break;  // the tile traversal has one shape
         adt_tune c = t;
         c.adj_wg = wg;
         float ms = 0.f;
@@ -1069,7 +1144,8 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     // wins, below it if not, and stays as it was when fusion does not apply)
     float total = fbest + abest;
     t.fuse_max_n = cur.fuse_max_n;
-    if ((flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && plan.max_l <= t.diag_max_l) {
+    if ((flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && plan.max_l <= t.diag_max_l &&
+        plan.adj_algo == 1) {
       adt_tune c = t;
       c.fuse_max_n = (int32_t)plan.max_n;
       float ms = 0.f;
@@ -1116,11 +1192,11 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   if (!(flags & (ADT_FORWARD | ADT_ADJOINT))) return fail(ADT_EINVAL, "graph: no ADT_FORWARD/ADT_ADJOINT flag");
   if (mis && !((flags & ADT_FORWARD) && (flags & ADT_ADJOINT)))
     return fail(ADT_EINVAL, "misfit graph: needs ADT_FORWARD and ADT_ADJOINT");
+  const adt_tune tn = current_tune<T>();
   BatchPlan<T> plan;
-  int rc = plan.build(ev, nev, flags);
+  int rc = plan.build(ev, nev, flags, tn.adj_algo);
   if (rc) return rc;
   if (mis && (rc = plan.attach_misfit(ev, nev, mis))) return rc;
-  const adt_tune tn = current_tune<T>();
   if (tn.order) plan.order_largest_first();
   GraphBase* g = new GraphBase();
   g->nev = nev;
@@ -1148,7 +1224,8 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, plan.max_n, tn,
+    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, plan.max_n,
+                         plan.adj_algo, plan.max_tiles, tn,
                          (adt_info*)(base + plan.off_info), cs, plan.misfit_run(base));
     hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
     if (rc == ADT_OK && e2 != hipSuccess) e = e2;

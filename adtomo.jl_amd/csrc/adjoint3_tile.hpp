@@ -1,0 +1,684 @@
+// 3D adjoint of one event as a tile-ordered topological solve (device code
+// of adj3_tile_kernel).
+//
+// The system is the one of Eikonal3D.cpp:96-198: the reference assembles the
+// upwind Jacobian and solves A^T lambda = g with a sparse LU.  Every edge of
+// the upwind graph (node c "feeds" node r when r is c's argmin neighbour
+// along one of c's active dimensions) goes from a larger to a smaller travel
+// time, so A is triangular in decreasing u and each node resolves once all
+// its feeders have:
+//     lambda_r = (g_r + sum_c 2 (u_c - u_r) lambda_c) / D_r,
+//     D_r = sum over r's active dimensions (x, y, z order) of 2 (u_r - u_up),
+// the feeders pulled in the fixed order (i-1, i+1, j-1, j+1, k-1, k+1).
+// The result does not depend on the order in which nodes are resolved.
+//
+// Instead of one global traversal (levels of scattered single-node
+// transactions), the event is cut into 8x8x8 tiles.  A wave visits one tile
+// at a time: it loads the tile with a one-node halo into its own LDS boxes
+// (coalesced 64-byte rows), resolves every node of the tile that can be
+// resolved given the multipliers already final outside the tile (a
+// topological traversal in LDS), and writes the multipliers and dL/df of the
+// nodes it resolved.  Tiles are scheduled by a per-tile count of the edges
+// that enter them from unresolved nodes of other tiles: a tile whose count
+// reaches zero resolves completely in one visit.  Tiles whose external
+// feeders depend on each other in a cycle (the node graph is acyclic, the
+// tile graph need not be) are marked dirty whenever one of their external
+// feeders resolves; when no tile is ready and no wave is visiting, the dirty
+// tiles are visited as they are, which always makes progress (the
+// unresolved node of largest u has all its feeders final).
+//
+// One workgroup per event: the scheduling state (bitmaps, counters) lives
+// in LDS and global memory is shared only among the waves of one compute
+// unit, so workgroup-scope release/acquire order the multiplier stores of a
+// visit before the loads of the visits they enable.
+#pragma once
+
+#include "adjoint3.hpp"
+#include "buffer_ops.hpp"
+#include "eikonal_device.hpp"
+#include "kernels.hpp"
+
+namespace adt {
+namespace tile {
+
+constexpr int kT = 8;                  // tile edge (nodes)
+constexpr int kTN = kT * kT * kT;      // nodes per tile
+constexpr int kB = kT + 2;             // box edge with a one-node halo
+constexpr int kBox = kB * kB * kB;     // box entries
+constexpr int kBI = kB * kB, kBJ = kB;  // box strides (k stride 1)
+constexpr uint8_t kCZ = 0x40;          // code flag: Z (lambda = 0, no upwind)
+constexpr uint8_t kCA = 0x80;          // code flag (LDS only): outside the grid
+// Per-node record of a visit (16 bits): bits 0-3 the number of feeders not
+// yet final, 4-9 the feeder mask (neighbour order i-1, i+1, j-1, j+1, k-1,
+// k+1), 10-12 which upwind neighbours (x, y, z) are present and not Z (they
+// are released when the node resolves), 14 not part of the visit (Z,
+// absent, or final before it), 15 resolved in this visit.
+constexpr uint32_t kPCount = 0xFu;
+constexpr int kPFeedShift = 4, kPRelShift = 10;
+constexpr uint32_t kPNA = 1u << 14, kPDone = 1u << 15;
+
+// Multiplier of a node that is not final yet: a signalling-NaN pattern that
+// arithmetic never produces (computed NaNs are quiet).
+template <typename T> struct Sent;
+template <> struct Sent<double> {
+  using U = unsigned long long;
+  static constexpr U bits = 0x7FF4DEADBEEF0001ull;
+  static __device__ __forceinline__ double value() { return __longlong_as_double((long long)bits); }
+  static __device__ __forceinline__ bool is(double x) { return (U)__double_as_longlong(x) == bits; }
+};
+template <> struct Sent<float> {
+  using U = unsigned int;
+  static constexpr U bits = 0x7FA0BEEFu;
+  static __device__ __forceinline__ float value() { return __uint_as_float(bits); }
+  static __device__ __forceinline__ bool is(float x) { return __float_as_uint(x) == bits; }
+};
+
+// LDS of one wave: the tile and its halo (u, multipliers, codes), the
+// pending-feeder count of each tile node and the traversal queue.
+template <typename T>
+struct WaveBuf {
+  T u[kBox];
+  T lam[kBox];
+  uint8_t code[kBox];
+  uint16_t pend[kTN];
+  uint16_t queue[kTN];
+};
+
+// Scheduling state of the workgroup (LDS): five bitmaps over the tiles and
+// a few counters.
+enum { kReady = 0, kDirty = 1, kBusy = 2, kDone = 3, kVisited = 4, kMaps = 5 };
+struct Sched {
+  int remaining;   // non-Z nodes not final
+  int active;      // waves holding a tile
+  int lock;        // deadlock-round mover
+  int fault;       // a bounded wait gave up
+  int visits;      // tile visits (diagnostics)
+  int rounds;      // dirty rounds (diagnostics)
+  int cursor;      // bitmap word where the next pop starts
+  int pad;
+};
+
+// Bounded idle polling of a wave: past this many polls without any node
+// resolving, the kernel reports a fault instead of waiting on.
+constexpr int kIdleCap = 1 << 22;
+constexpr int kVisitCapPerTile = 64;
+
+// Per-event constants.
+template <typename T>
+struct Ev {
+  int m, n, l;
+  int nti, ntj, ntk, ntiles, nwords;
+  int64_t sI;  // n * l
+  T h;
+  gptr<const T> u, u0, g, f;
+  gptr<T> gu0, gf, lam;
+  gptr<uint8_t> code;
+  gptr<int> ext;
+};
+
+// Node of slot s (0..13) of a lane in the tile box: slots 0-7 are the tile
+// slabs ii = s, 8/9 the i faces (ii = -1 / 8), 10/11 the j faces, 12/13 the
+// k faces.  Returns the box index; (gi, gj, gk) the grid coordinates.
+__device__ __forceinline__ int slot_node(int s, int lane, int i0, int j0, int k0, int& gi,
+                                         int& gj, int& gk) {
+  const int a = lane >> 3, b = lane & 7;
+  int ii, jj, kk;
+  if (s < 8) {
+    ii = s; jj = a; kk = b;
+  } else if (s < 10) {
+    ii = s == 8 ? -1 : kT; jj = a; kk = b;
+  } else if (s < 12) {
+    ii = a; jj = s == 10 ? -1 : kT; kk = b;
+  } else {
+    ii = a; jj = b; kk = s == 12 ? -1 : kT;
+  }
+  gi = i0 + ii;
+  gj = j0 + jj;
+  gk = k0 + kk;
+  return (ii + 1) * kBI + (jj + 1) * kBJ + (kk + 1);
+}
+
+__device__ __forceinline__ int box_of_id(int id) {
+  return ((id >> 6) + 1) * kBI + (((id >> 3) & 7) + 1) * kBJ + (id & 7) + 1;
+}
+
+// Neighbour q (i-1, i+1, j-1, j+1, k-1, k+1) of a box index.
+__device__ __forceinline__ int box_nb(int b, int q) {
+  const int st = (q >> 1) == 0 ? kBI : ((q >> 1) == 1 ? kBJ : 1);
+  return (q & 1) ? b + st : b - st;
+}
+
+// Does the node with code cn, lying in direction q of node r, feed r?  It
+// must be present, not Z, and its argmin side along q's axis must point back
+// to r (a neighbour at -1 has its upwind at +1, and the reverse).
+__device__ __forceinline__ bool feeds(uint8_t cn, int q) {
+  const int d = q >> 1;
+  return !(cn & (kCZ | kCA)) && (((cn >> (2 * d)) & 3u) == ((q & 1) ? 1u : 2u));
+}
+
+// Wave-level ordering of LDS accesses by different lanes (the wave's LDS
+// instructions execute in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Subtracts `by` (0 or 1) from the pending count of record `id` through a
+// 32-bit atomic on the word holding it; returns the record's previous value.
+// Counts never go below zero, so no borrow crosses into the other half of
+// the word.  Lanes with nothing to release subtract 0, so every lane issues
+// its atomics unconditionally and the wave waits once.
+__device__ __forceinline__ unsigned pend_dec(uint16_t* pend, int id, unsigned by) {
+  unsigned* w = (unsigned*)(pend + (id & ~1));
+  const int sh = (id & 1) * 16;
+  return (atomicSub(w, by << sh) >> sh) & 0xffffu;
+}
+
+// ---------------------------------------------------------------- phases ---
+
+// Phase 0 (whole workgroup, grid-stride over the event's nodes): upwind
+// codes (the reference's Z rule, Eikonal3D.cpp:105-171), grad_u0 = g where
+// u == u0 (:105-110), multipliers set to 0 on Z and to the "not final"
+// pattern elsewhere, and dL/df = -0 * (-2 f h h) on Z (the value the
+// reference's final loop gives those nodes, :194-196).  Returns this
+// thread's count of non-Z nodes.
+template <typename T, int kPrep>
+__device__ int phase_codes(const Ev<T>& E) {
+  const int m = E.m, n = E.n, l = E.l;
+  const int64_t nl = E.sI, N = (int64_t)m * nl;
+  const int ex[3] = {m, n, l};
+  const int64_t st[3] = {nl, l, 1};
+  const int64_t stride = kPrep * (int64_t)blockDim.x;
+  int nonz = 0;
+  Walk wk[kPrep];
+#pragma unroll
+  for (int k = 0; k < kPrep; ++k) wk[k] = Walk(threadIdx.x + k * blockDim.x, stride, m, n, l);
+  for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
+    const PrepGroup<kPrep> G = prep_group<kPrep>(p0, N, wk);
+    T uc[kPrep], u0c[kPrep], gc[kPrep], nb[kPrep][6];
+#pragma unroll
+    for (int k = 0; k < kPrep; ++k) {
+      const int64_t q = G.q[k];
+      uc[k] = E.u[q];
+      u0c[k] = E.u0[q];
+      gc[k] = E.g[q];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {  // clamped into the grid; edges use one side
+        nb[k][2 * d] = E.u[G.co[k][d] > 0 ? q - st[d] : q + st[d]];
+        nb[k][2 * d + 1] = E.u[G.co[k][d] < ex[d] - 1 ? q + st[d] : q - st[d]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPrep; ++k) {
+      const uint32_t code = code3<T>(uc[k], u0c[k], nb[k], G.co[k], ex);
+      if (G.valid[k]) {
+        const int64_t q = G.q[k];
+        const bool z = code & kZ;
+        E.code[q] = (uint8_t)code;
+        if (E.gu0) E.gu0[q] = (uc[k] == u0c[k]) ? gc[k] : T(0);
+        E.lam[q] = z ? T(0) : Sent<T>::value();
+        if (z) {
+          const T rhs = T(-2.0) * E.f[q] * E.h * E.h;
+          E.gf[q] = -T(0) * rhs;
+        }
+        nonz += z ? 0 : 1;
+      }
+    }
+  }
+  return nonz;
+}
+
+// Phase 1 (waves over tiles, kPT tiles per iteration with their loads in
+// flight together): the count of edges that enter each tile from non-Z
+// nodes of other tiles (its external feeders, all unresolved at the start),
+// and the initial bitmaps: done for tiles without non-Z nodes, dirty for the
+// others, ready for those without external feeders.  Lane (a, b) checks, on
+// each face, the halo node of its face position against the tile node next
+// to it, and counts the non-Z nodes of its column (a, b) of the tile.
+template <typename T>
+__device__ __forceinline__ uint8_t code_at(const Ev<T>& E, int i, int j, int k) {
+  const bool in = i >= 0 && i < E.m && j >= 0 && j < E.n && k >= 0 && k < E.l;
+  return in ? E.code[(int64_t)i * E.sI + (int64_t)j * E.l + k] : kCA;
+}
+
+template <typename T>
+__device__ void phase_counts(const Ev<T>& E, uint32_t* maps, int nw, int w) {
+  constexpr int kPT = 4;
+  const int lane = lane_id();
+  const int a = lane >> 3, b = lane & 7;
+  for (int t0 = w * kPT; t0 < E.ntiles; t0 += nw * kPT) {
+    uint8_t col[kPT][8], hal[kPT][6], inn[kPT][4];
+#pragma unroll
+    for (int x = 0; x < kPT; ++x) {
+      const int t = t0 + x < E.ntiles ? t0 + x : t0;
+      const int i0 = t / (E.ntj * E.ntk) * kT, j0 = (t / E.ntk) % E.ntj * kT, k0 = t % E.ntk * kT;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) col[x][s] = code_at(E, i0 + s, j0 + a, k0 + b);  // (s, a, b)
+      hal[x][0] = code_at(E, i0 - 1, j0 + a, k0 + b);   // i faces, position (a, b)
+      hal[x][1] = code_at(E, i0 + kT, j0 + a, k0 + b);
+      hal[x][2] = code_at(E, i0 + a, j0 - 1, k0 + b);   // j faces, position (a, b)
+      hal[x][3] = code_at(E, i0 + a, j0 + kT, k0 + b);
+      hal[x][4] = code_at(E, i0 + a, j0 + b, k0 - 1);   // k faces, position (a, b)
+      hal[x][5] = code_at(E, i0 + a, j0 + b, k0 + kT);
+      inn[x][0] = code_at(E, i0 + a, j0, k0 + b);       // tile nodes next to the j, k faces
+      inn[x][1] = code_at(E, i0 + a, j0 + kT - 1, k0 + b);
+      inn[x][2] = code_at(E, i0 + a, j0 + b, k0);
+      inn[x][3] = code_at(E, i0 + a, j0 + b, k0 + kT - 1);
+    }
+#pragma unroll
+    for (int x = 0; x < kPT; ++x) {
+      const int t = t0 + x;
+      int nz = 0, ext = 0;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) nz += (col[x][s] & (kCZ | kCA)) ? 0 : 1;
+      const uint8_t tn[6] = {col[x][0], col[x][7], inn[x][0], inn[x][1], inn[x][2], inn[x][3]};
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        ext += (!(tn[q] & (kCZ | kCA)) && feeds(hal[x][q], q)) ? 1 : 0;
+      nz = wave_sum(nz);
+      ext = wave_sum(ext);
+      if (lane == 0 && t < E.ntiles) {
+        E.ext[t] = ext;
+        const uint32_t bit = 1u << (t & 31);
+        uint32_t* wd = maps + (t >> 5);
+        if (nz == 0) {
+          atomicOr(wd + kDone * E.nwords, bit);
+        } else {
+          atomicOr(wd + kDirty * E.nwords, bit);
+          if (ext == 0) atomicOr(wd + kReady * E.nwords, bit);
+        }
+      }
+    }
+  }
+}
+
+// One visit of tile t by one wave.  Resolves every node of the tile whose
+// feeders are final (inside the tile: transitively in this visit), stores
+// their multipliers and dL/df, and returns the number resolved; `left` = the
+// tile's nodes still unresolved; face[q] = edges from the resolved nodes to
+// non-Z nodes of the neighbouring tile in direction q.
+template <typename T>
+__device__ int visit(const Ev<T>& E, WaveBuf<T>& B, int t, bool revisit, int& left,
+                     int (&face)[6]) {
+  const int lane = lane_id();
+  const int ti = t / (E.ntj * E.ntk), tj = (t / E.ntk) % E.ntj, tk = t % E.ntk;
+  const int i0 = ti * kT, j0 = tj * kT, k0 = tk * kT;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) face[q] = 0;
+  // --- codes and multipliers of the box (halo always, tile on revisits)
+  int bx[14];
+  int64_t gx[14];
+  bool in[14];
+  {
+    uint8_t c[14];
+    T lm[14];
+#pragma unroll
+    for (int s = 0; s < 14; ++s) {
+      int gi, gj, gk;
+      bx[s] = slot_node(s, lane, i0, j0, k0, gi, gj, gk);
+      in[s] = gi >= 0 && gi < E.m && gj >= 0 && gj < E.n && gk >= 0 && gk < E.l;
+      gx[s] = in[s] ? (int64_t)gi * E.sI + (int64_t)gj * E.l + gk : 0;
+      c[s] = in[s] ? E.code[gx[s]] : kCA;
+      lm[s] = (in[s] && (s >= 8 || revisit)) ? E.lam[gx[s]] : Sent<T>::value();
+    }
+#pragma unroll
+    for (int s = 0; s < 14; ++s) {
+      B.code[bx[s]] = c[s];
+      B.lam[bx[s]] = lm[s];
+    }
+  }
+  wave_sync();
+  // --- records of the tile's nodes (lane owns slab nodes s*64+lane)
+  uint32_t todo = 0;  // bit s: node (s, lane) unresolved and not Z
+  int nready = 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int bi = bx[s];
+    const uint8_t c = B.code[bi];
+    uint8_t cn[6];
+    T ln[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      cn[q] = B.code[box_nb(bi, q)];
+      ln[q] = B.lam[box_nb(bi, q)];
+    }
+    uint32_t rec = kPNA;
+    if (!(c & (kCZ | kCA)) && Sent<T>::is(B.lam[bi])) {
+      uint32_t cnt = 0, fm = 0, rel = 0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const bool fd = feeds(cn[q], q);
+        fm |= fd ? (1u << q) : 0u;
+        cnt += (fd && Sent<T>::is(ln[q])) ? 1u : 0u;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const uint32_t sd = (c >> (2 * d)) & 3u;
+        const uint8_t cx = cn[2 * d + (sd == 2 ? 1 : 0)];
+        rel |= (sd && !(cx & (kCZ | kCA))) ? (1u << d) : 0u;
+      }
+      rec = cnt | (fm << kPFeedShift) | (rel << kPRelShift);
+      todo |= 1u << s;
+      nready += cnt == 0 ? 1 : 0;
+    }
+    B.pend[s * 64 + lane] = (uint16_t)rec;
+  }
+  left = wave_sum(__builtin_popcount(todo));
+  if (wave_sum(nready) == 0) return 0;  // nothing can resolve yet
+  // --- travel times of the box, adjoint sources and slowness of the tile
+  T fr[8];
+  {
+    T uv[14], gv[8];
+    const rsrc_t ru = make_rsrc((const void*)E.u, (uint64_t)E.m * E.sI * sizeof(T));
+#pragma unroll
+    for (int s = 0; s < 14; ++s)
+      uv[s] = bload(ru, in[s] ? (uint32_t)(gx[s] * (int64_t)sizeof(T)) : kOOB, T(0));
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const bool need = (todo >> s) & 1u;
+      gv[s] = need ? E.g[gx[s]] : T(0);
+      fr[s] = need ? E.f[gx[s]] : T(0);
+    }
+#pragma unroll
+    for (int s = 0; s < 14; ++s) B.u[bx[s]] = uv[s];
+    // lambda of an unresolved tile node starts as its source g
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      if ((todo >> s) & 1u) B.lam[bx[s]] = gv[s];
+  }
+  wave_sync();
+  // --- initial queue: unresolved nodes with no pending feeder
+  int qtail = 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const bool r = ((todo >> s) & 1u) && (B.pend[s * 64 + lane] & kPCount) == 0;
+    const uint64_t bal = __ballot(r);
+    const int pos = qtail + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+    if (r && pos < kTN) B.queue[pos] = (uint16_t)(s * 64 + lane);
+    qtail += __builtin_popcountll(bal);
+  }
+  wave_sync();
+  // --- traversal: each round resolves up to 64 queued nodes, all of whose
+  // feeders are final, then releases their upwind neighbours.  A round reads
+  // a node's record, its own and its six neighbours' values in one batch.
+  int qhead = 0;
+  int fcnt[6] = {0, 0, 0, 0, 0, 0};
+  while (qhead < qtail) {
+    const int qend = qhead + 64 < qtail ? qhead + 64 : qtail;  // this round's entries
+    const int qi = qhead + lane;
+    const bool act = qi < qend;
+    const int id = act ? (int)B.queue[qi] : 0;
+    const int br = box_of_id(id);
+    const uint32_t rec = B.pend[id];
+    const uint8_t cr = B.code[br];
+    const T ur = B.u[br];
+    const T g0 = B.lam[br];
+    T un[6], ln[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      un[q] = B.u[box_nb(br, q)];
+      ln[q] = B.lam[box_nb(br, q)];
+    }
+    T num = g0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const T c = num + T(2.0) * (un[q] - ur) * ln[q];
+      num = ((rec >> (kPFeedShift + q)) & 1u) ? c : num;
+    }
+    T D = T(0);
+    bool have = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t sd = (cr >> (2 * d)) & 3u;
+      const T w = T(2.0) * (ur - (sd == 2 ? un[2 * d + 1] : un[2 * d]));
+      D = sd ? (have ? D + w : w) : D;
+      have = have || sd;
+    }
+    const T lr = num / D;
+    wave_sync();  // every lane has read its feeders before any writes
+    if (act) {
+      B.lam[br] = lr;
+      B.pend[id] = (uint16_t)(rec | kPDone);
+    }
+    wave_sync();
+    // release the upwind neighbours (pend decrement; halo ones are counted
+    // for the neighbouring tile's external count)
+    bool push[3];
+    int pid[3];
+    unsigned old[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t sd = (cr >> (2 * d)) & 3u;
+      const int co = d == 0 ? (id >> 6) : (d == 1 ? ((id >> 3) & 7) : (id & 7));
+      const int cn = sd == 1 ? co - 1 : co + 1;
+      const bool rel = act && ((rec >> (kPRelShift + d)) & 1u);
+      const bool inside = cn >= 0 && cn < kT;
+      const bool dec = rel && inside;
+      pid[d] = dec ? id + (cn - co) * (d == 0 ? 64 : (d == 1 ? 8 : 1)) : id;
+      old[d] = pend_dec(B.pend, pid[d], dec ? 1u : 0u);
+      push[d] = dec && (old[d] & kPCount) == 1u;
+      fcnt[2 * d + (sd == 2 ? 1 : 0)] += (rel && !inside) ? 1 : 0;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const uint64_t bal = __ballot(push[d]);
+      const int pos = qtail + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+      if (push[d] && pos < kTN) B.queue[pos] = (uint16_t)pid[d];
+      qtail += __builtin_popcountll(bal);
+    }
+    qhead = qend;
+    qtail = qtail < kTN ? qtail : kTN;  // each node enters once: never clamps
+    wave_sync();
+  }
+  // --- results of the nodes resolved in this visit (owners store coalesced)
+  int nres = 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    if (((todo >> s) & 1u) && (B.pend[s * 64 + lane] & kPDone)) {
+      const T lr = B.lam[bx[s]];
+      E.lam[gx[s]] = (lr == lr) ? lr : T(__builtin_nan(""));
+      const T rhs = T(-2.0) * fr[s] * E.h * E.h;  // Eikonal3D.cpp:113-116, 194-196
+      E.gf[gx[s]] = -lr * rhs;
+      ++nres;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) face[q] = wave_sum(fcnt[q]);
+  nres = wave_sum(nres);
+  left -= nres;
+  return nres;
+}
+
+// Tile of neighbouring direction q of tile t, or -1 outside the grid.
+template <typename T>
+__device__ __forceinline__ int tile_nb(const Ev<T>& E, int t, int q) {
+  int c[3] = {t / (E.ntj * E.ntk), (t / E.ntk) % E.ntj, t % E.ntk};
+  const int ext[3] = {E.nti, E.ntj, E.ntk};
+  const int d = q >> 1;
+  c[d] += (q & 1) ? 1 : -1;
+  if (c[d] < 0 || c[d] >= ext[d]) return -1;
+  return (c[0] * E.ntj + c[1]) * E.ntk + c[2];
+}
+
+// Takes a ready tile that no wave is visiting: clears its ready and dirty
+// bits, marks it busy and counts the wave as active.  Returns -1 if none was
+// found (wave-uniform).
+__device__ __forceinline__ int pop_tile(uint32_t* maps, int nwords, Sched& S) {
+  const int lane = lane_id();
+  const int start = S.cursor;
+  for (int base = 0; base < nwords; base += 64) {
+    int wi = start + base + lane;
+    wi = wi >= nwords ? wi - nwords : wi;
+    wi = wi >= nwords ? wi - nwords : wi;
+    uint32_t cand = 0;
+    if (base + lane < nwords)
+      cand = maps[kReady * nwords + wi] & ~maps[kBusy * nwords + wi] & ~maps[kDone * nwords + wi];
+    const uint64_t bal = __ballot(cand != 0);
+    if (!bal) continue;
+    const int src = __builtin_ctzll(bal);
+    int t = -1;
+    if (lane == src) {
+      const uint32_t bit = cand & (0u - cand);
+      const uint32_t old = atomicAnd(&maps[kReady * nwords + wi], ~bit);
+      if (old & bit) {
+        const uint32_t ob = atomicOr(&maps[kBusy * nwords + wi], bit);
+        if (ob & bit) {
+          atomicOr(&maps[kReady * nwords + wi], bit);  // being visited: leave it ready
+        } else {
+          atomicAdd(&S.active, 1);
+          atomicAnd(&maps[kDirty * nwords + wi], ~bit);
+          t = wi * 32 + __builtin_ctz(bit);
+          S.cursor = wi;
+        }
+      }
+    }
+    return __shfl(t, src);
+  }
+  return -1;
+}
+
+}  // namespace tile
+
+// The whole adjoint of one event by one workgroup of NW waves (at most
+// MAXT tiles).  Returns nothing; results and diagnostics go to the event.
+template <typename T, int NW, int MAXT>
+__device__ void adj3_tile_event(const EventDesc& ev) {
+  using namespace tile;
+  constexpr int MW = MAXT / 32;
+  __shared__ WaveBuf<T> wb[NW];
+  __shared__ uint32_t maps[kMaps * MW];
+  __shared__ Sched S;
+  Ev<T> E;
+  E.m = ev.m;
+  E.n = ev.n;
+  E.l = ev.l;
+  E.nti = (E.m + kT - 1) / kT;
+  E.ntj = (E.n + kT - 1) / kT;
+  E.ntk = (E.l + kT - 1) / kT;
+  E.ntiles = E.nti * E.ntj * E.ntk;
+  E.nwords = (E.ntiles + 31) / 32;
+  E.sI = (int64_t)E.n * E.l;
+  E.h = (T)ev.h;
+  E.u = (gptr<const T>)ev.u;
+  E.u0 = (gptr<const T>)ev.u0;
+  E.g = (gptr<const T>)ev.grad_u;
+  E.f = (gptr<const T>)ev.f;
+  E.gu0 = (gptr<T>)ev.grad_u0;
+  E.gf = (gptr<T>)ev.grad_f;
+  E.lam = (gptr<T>)ev.lam;
+  E.code = (gptr<uint8_t>)ev.code;
+  E.ext = (gptr<int>)ev.queue;
+  const int w = threadIdx.x >> 6;
+  const int lane = lane_id();
+  if (E.ntiles > MAXT) {  // the host routes such events elsewhere
+    if (threadIdx.x == 0) ev.info->fault = 1;
+    return;
+  }
+  for (int i = threadIdx.x; i < kMaps * E.nwords; i += blockDim.x) maps[i] = 0;
+  if (threadIdx.x == 0) {
+    S.remaining = 0;
+    S.active = 0;
+    S.lock = 0;
+    S.fault = 0;
+    S.visits = 0;
+    S.rounds = 0;
+    S.cursor = 0;
+  }
+  __syncthreads();
+  const int nz = phase_codes<T, 4>(E);
+  atomicAdd(&S.remaining, nz);
+  __syncthreads();
+  phase_counts<T>(E, maps, NW, w);
+  __syncthreads();
+  // --- tile visits until every non-Z node is final
+  int idle = 0;
+  int last_rem = -1;
+  for (;;) {
+    const int rem = __hip_atomic_load(&S.remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (rem == 0 || __hip_atomic_load(&S.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+      break;
+    // a visit either resolves nodes or clears the bit that scheduled it, so
+    // the visits are bounded; this cap only turns a logic error into a fault
+    if (__hip_atomic_load(&S.visits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >
+        kVisitCapPerTile * E.ntiles + 1024) {
+      if (lane == 0) atomicExch(&S.fault, 1);
+      break;
+    }
+    const int t = pop_tile(maps, E.nwords, S);
+    if (t >= 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const uint32_t bit = 1u << (t & 31);
+      const bool revisit = maps[kVisited * E.nwords + (t >> 5)] & bit;
+      int left = 0, face[6];
+      const int nres = visit<T>(E, wb[w], t, revisit, left, face);
+      // the visit's stores before the bits and counts that publish them
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane < 6 && face[lane] > 0) {
+        const int t2 = tile_nb(E, t, lane);
+        if (t2 >= 0) {
+          const int old = __hip_atomic_fetch_sub(&E.ext[t2], face[lane], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          const uint32_t b2 = 1u << (t2 & 31);
+          atomicOr(&maps[kDirty * E.nwords + (t2 >> 5)], b2);
+          if (old == face[lane]) atomicOr(&maps[kReady * E.nwords + (t2 >> 5)], b2);
+        }
+      }
+      if (lane == 0) {
+        atomicOr(&maps[kVisited * E.nwords + (t >> 5)], bit);
+        if (left == 0) atomicOr(&maps[kDone * E.nwords + (t >> 5)], bit);
+        atomicAnd(&maps[kBusy * E.nwords + (t >> 5)], ~bit);
+        atomicAdd(&S.visits, 1);
+        if (nres) atomicSub(&S.remaining, nres);
+        atomicSub(&S.active, 1);
+      }
+      wave_sync();
+      idle = 0;
+      continue;
+    }
+    // nothing ready: when no wave holds a tile, every dirty tile is visited
+    // as it is (some node is always resolvable; see the file comment)
+    if (__hip_atomic_load(&S.active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
+      int got = 0;
+      if (lane == 0) got = atomicCAS(&S.lock, 0, 1) == 0;
+      got = __shfl(got, 0);
+      if (got) {
+        if (__hip_atomic_load(&S.active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
+          for (int i = lane; i < E.nwords; i += 64) {
+            const uint32_t d = maps[kDirty * E.nwords + i] & ~maps[kDone * E.nwords + i];
+            if (d) atomicOr(&maps[kReady * E.nwords + i], d);
+          }
+          if (lane == 0) atomicAdd(&S.rounds, 1);
+        }
+        wave_sync();
+        if (lane == 0) atomicExch(&S.lock, 0);
+      }
+    }
+    if (rem != last_rem) {
+      last_rem = rem;
+      idle = 0;
+    }
+    if (++idle > kIdleCap) {
+      if (lane == 0) atomicExch(&S.fault, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ev.info->adj_sweeps = S.visits;
+    ev.info->adj_converged = (S.remaining == 0 && !S.fault) ? 1 : 0;
+    if (S.fault) ev.info->fault = 1;
+  }
+}
+
+}  // namespace adt

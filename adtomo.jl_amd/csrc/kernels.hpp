@@ -124,6 +124,15 @@ template <typename T>
 __global__ void fwd2_event_kernel(const Event2Desc* evs, int max_iter);
 template <typename T>
 __global__ void adj3_kahn_kernel(const EventDesc* evs);
+// Tile-ordered 3D adjoint: NW waves per event, at most MAXT tiles of 8^3
+// nodes.  Workspace per event: lam = multipliers [N] (T), code = upwind codes
+// [N] (bytes), queue = per-tile external-edge counts [tiles] (int32).
+template <typename T, int NW, int MAXT>
+__global__ void adj3_tile_kernel(const EventDesc* evs);
+constexpr int kTileEdge = 8;
+constexpr int kTileSmallMax = 8192;   // tiles of the 8-wave FP64 variant
+constexpr int kTileLargeMax = 65536;  // tiles of the large-event variants
+constexpr int kTileWavesSmall = 8, kTileWavesLarge64 = 6, kTileWavesLarge32 = 8;
 template <typename T>
 __global__ void adj2_kahn_kernel(const Event2Desc* evs);
 template <typename T>

@@ -6,6 +6,7 @@
 // graph is acyclic, so Kahn's algorithm solves the system exactly once per
 // node, pulling each node's feeders in a fixed order.
 #include "adjoint3.hpp"
+#include "adjoint3_tile.hpp"
 #include "adjoint_device.hpp"
 #include "kernels.hpp"
 
@@ -26,6 +27,11 @@ __device__ __forceinline__ unsigned dec_byte(uint8_t* base, int64_t idx) {
 template <typename T>
 __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __restrict__ evs) {
   adj3_kahn_event<T, 2, 2>(evs[blockIdx.x]);
+}
+
+template <typename T, int NW, int MAXT>
+__global__ void __launch_bounds__(NW * 64) adj3_tile_kernel(const EventDesc* __restrict__ evs) {
+  adj3_tile_event<T, NW, MAXT>(evs[blockIdx.x]);
 }
 
 // ----------------------------------------------------------------- 2D ---
@@ -142,6 +148,10 @@ __global__ void __launch_bounds__(1024) adj2_kahn_kernel(const Event2Desc* __res
 
 template __global__ void adj3_kahn_kernel<double>(const EventDesc*);
 template __global__ void adj3_kahn_kernel<float>(const EventDesc*);
+template __global__ void adj3_tile_kernel<double, kTileWavesSmall, kTileSmallMax>(const EventDesc*);
+template __global__ void adj3_tile_kernel<double, kTileWavesLarge64, kTileLargeMax>(const EventDesc*);
+template __global__ void adj3_tile_kernel<float, kTileWavesSmall, kTileSmallMax>(const EventDesc*);
+template __global__ void adj3_tile_kernel<float, kTileWavesLarge32, kTileLargeMax>(const EventDesc*);
 template __global__ void adj2_kahn_kernel<double>(const Event2Desc*);
 template __global__ void adj2_kahn_kernel<float>(const Event2Desc*);
 

@@ -456,6 +456,7 @@ struct AdtTune
     fuse_max_n::Int32
     fwd_ms::Float32
     adj_ms::Float32
+    adj_algo::Int32
 end
 
 dtype_code(::Type{Float32}) = Int32(1)
@@ -467,7 +468,7 @@ dtype_code(::Type{Float64}) = Int32(0)
 Launch setting of the 3D kernels for element type `T`.
 """
 function get_tune(::Type{T}=Float64) where {T}
-    t = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0))
+    t = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0, 0))
     check(ccall((:adt_get_tune, libadt), Cint, (Int32, Ref{AdtTune}), dtype_code(T), t),
           "get_tune")
     return t[]
@@ -497,7 +498,7 @@ function autotune!(us, u0s, fs, hs, tols; grad_us=nothing, grad_u0s=nothing, gra
     tab = event_table(u0s, fs, us, hs, tols, grad_us, grad_u0s, grad_fs)
     flags = ADT_FORWARD | (grad_us === nothing ? Int32(0) : ADT_ADJOINT) |
             (compute_f64 ? ADT_F32_COMPUTE_F64 : Int32(0))
-    best = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0))
+    best = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0, 0))
     rc = ccall((:adt_autotune, libadt), Cint,
                (Ptr{AdtEvent3d}, Int32, Int32, Int32, Int32, Ref{AdtTune}, Ptr{Cvoid}),
                tab, length(tab), flags, dtype_code(eltype(fs[1])), Int32(reps), best, stream)

@@ -45,6 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # kernel of known size: profiles/pmc_traffic_fwd3_diag_adj3_kahn_256x128x128x64.md
 FWD_KERNEL = "fwd3_diag_kernel<double, false>"
 ADJ_KERNEL = "adj3_kahn_kernel<double>"
+ADJ_TILE_KERNEL = "adj3_tile_kernel<double, 8, 8192>"
 FUSED_KERNEL = "fwd_adj3_diag_kernel<double, false>"
 TRAFFIC_BYTES_256 = {FWD_KERNEL: 117.9e9 + 57.2e9, ADJ_KERNEL: 129.4e9 + 49.3e9,
                      FUSED_KERNEL: 243.9e9 + 105.0e9}
@@ -288,10 +289,12 @@ def main():
                 "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "launch_ms": round(kms, 3),
                 "traffic": TRAFFIC_BYTES_256.get(kernel) if nev == 256 else None}
 
-    roofs = [roof(FWD_KERNEL, fwd_gbs, fwd_ms), roof(ADJ_KERNEL, adj_gbs, adj_ms)]
-    # with the fused setting the step is one kernel doing each event's
-    # forward and then its adjoint: it is the step's dominant kernel
-    fused = N <= tune["fuse_max_n"] and tune["diag_max_l"] >= shape[2]
+    adj_kernel = ADJ_TILE_KERNEL if tune.get("adj_algo", 1) == 0 else ADJ_KERNEL
+    roofs = [roof(FWD_KERNEL, fwd_gbs, fwd_ms), roof(adj_kernel, adj_gbs, adj_ms)]
+    # with the fused setting (node-level adjoint only) the step is one kernel
+    # doing each event's forward and then its adjoint: the dominant kernel
+    fused = (tune.get("adj_algo", 1) == 1 and N <= tune["fuse_max_n"]
+             and tune["diag_max_l"] >= shape[2])
     if fused:
         roofs.insert(0, roof(FUSED_KERNEL, total_gbs, ms))
     dominant = roofs[0] if fused else max(roofs, key=lambda r: r["launch_ms"])
@@ -313,7 +316,8 @@ def main():
                    "events": nev, "grid": list(shape), "K_mean": float(Ks.mean()),
                    "serial_depth_per_sweep": int(sum(shape) - 2),
                    "adj_sweeps_mean": float(sweeps.mean()),
-                   "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n")}},
+                   "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n",
+                                                           "adj_algo")}},
         "roofline": dominant,
         "roofline_kernels": roofs,
         "cpu_baseline": cpu,
@@ -454,26 +458,20 @@ def run_joint(args):
         L0, g0 = inv.objective(p0)
     ms = _time(lambda: inv.objective(p0), args.steps)
     log(f"joint: {ms:.1f} ms per gradient evaluation (200 solves fwd+adj + objective), loss {L0}")
-    from scipy.optimize import minimize
-    hist = []
-
-    def fun(x):
-        L, g = inv.objective(x)
-        hist.append(L)
-        return L, g
     t = time.perf_counter()
-    res = minimize(fun, p0, jac=True, method="L-BFGS-B", options={"maxiter": 1})
+    x1, hist = inv.run(iterations=1)
     t = time.perf_counter() - t
-    log(f"one L-BFGS step: {len(hist)} evaluations in {t:.2f}s, loss {hist[0]:.4g} -> {res.fun:.4g}, "
-        f"vp/vs {p0[-1]:.3f} -> {res.x[-1]:.4f}")
+    res = inv.result
+    log(f"one L-BFGS step: {res.f_calls} values, {res.g_calls} gradients in {t:.2f}s, loss "
+        f"{hist[0]:.4g} -> {hist[-1]:.4g}, vp/vs {p0[-1]:.3f} -> {x1[-1]:.4f}")
     inv.close()
     _line("joint P/S gradient evaluations: solves/s (fwd+adjoint) at 200x200x80",
           200 / ms * 1e3, "solves/s", ms, args, "f64",
           {"workload": "joint Vp/Vs gradient, 200x200x80, h=0.5 km, 100 stations x {P,S}, "
                        "500 events, Huber delta 0.1 s, noise N(0, 0.05 s)",
            "solves": 200, "events": 500, "chunk": chunk},
-          lbfgs_step={"evaluations": len(hist), "seconds": t, "loss_before": hist[0],
-                      "loss_after": float(res.fun), "vpvs_after": float(res.x[-1])})
+          lbfgs_step={"values": res.f_calls, "gradients": res.g_calls, "seconds": t,
+                      "loss_before": hist[0], "loss_after": hist[-1], "vpvs_after": float(x1[-1])})
 
 
 def run_large(args):

@@ -219,6 +219,11 @@ typedef struct adt_tune {
                           of about 10^6 nodes and costs on much larger ones */
   float fwd_ms;       /* adt_autotune: measured forward / adjoint time per */
   float adj_ms;       /* launch of the chosen setting (ms) */
+  int32_t adj_algo;   /* adjoint traversal: 0 = over 8x8x8 tiles, each
+                         resolved in LDS by one wave (default; events of at
+                         most 65536 tiles, larger ones take 1), 1 = node by
+                         node over the whole event (the only one that fuses
+                         with the forward, see fuse_max_n) */
 } adt_tune;
 int adt_get_tune(int32_t dtype, adt_tune *out);
 int adt_set_tune(int32_t dtype, const adt_tune *t);

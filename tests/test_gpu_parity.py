@@ -545,12 +545,15 @@ def _mixed_batch(seed=300):
 
 
 @pytest.mark.parametrize("setting", [
-    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30},
-    {"diag_max_l": 1 << 30, "adj_wg": 512, "order": 0, "fuse_max_n": 0},
-    {"diag_max_l": 64, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30},
-    {"diag_max_l": 0, "adj_wg": 256, "order": 0, "fuse_max_n": 1 << 30},
-    {"diag_max_l": 32, "adj_wg": 512, "order": 1, "fuse_max_n": 0},
-    {"diag_max_l": 7, "adj_wg": 256, "order": 1, "fuse_max_n": 1 << 30},
+    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30, "adj_algo": 1},
+    {"diag_max_l": 1 << 30, "adj_wg": 512, "order": 0, "fuse_max_n": 0, "adj_algo": 1},
+    {"diag_max_l": 64, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30, "adj_algo": 1},
+    {"diag_max_l": 0, "adj_wg": 256, "order": 0, "fuse_max_n": 1 << 30, "adj_algo": 1},
+    {"diag_max_l": 32, "adj_wg": 512, "order": 1, "fuse_max_n": 0, "adj_algo": 1},
+    {"diag_max_l": 7, "adj_wg": 256, "order": 1, "fuse_max_n": 1 << 30, "adj_algo": 1},
+    {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30, "adj_algo": 0},
+    {"diag_max_l": 0, "adj_wg": 256, "order": 0, "fuse_max_n": 0, "adj_algo": 0},
+    {"diag_max_l": 32, "adj_wg": 512, "order": 1, "fuse_max_n": 0, "adj_algo": 0},
 ])
 def test_every_tune_setting_matches_oracle(setting):
     import adtomo_hip as A
@@ -576,7 +579,7 @@ def test_every_tune_setting_matches_oracle(setting):
         G.close()
     finally:
         A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"], fuse_max_n=base["fuse_max_n"])
+                   order=base["order"], fuse_max_n=base["fuse_max_n"], adj_algo=base["adj_algo"])
 
 
 @pytest.mark.parametrize("dtype,compute_f64", [("float64", False), ("float32", False),
@@ -594,7 +597,8 @@ def test_fused_forward_adjoint_matches_separate_kernels(dtype, compute_f64):
     out = {}
     try:
         for fuse in (1, 0):
-            A.set_tune(dtype, diag_max_l=1 << 30, fuse_max_n=(1 << 30) if fuse else 0)
+            # the fused kernel runs the node-level traversal
+            A.set_tune(dtype, diag_max_l=1 << 30, fuse_max_n=(1 << 30) if fuse else 0, adj_algo=1)
             G = A.EikonalGraph(*args, grad_us=[dev(a, npd) for a in gs], compute_f64=compute_f64)
             G.launch()
             G.launch()
@@ -603,11 +607,77 @@ def test_fused_forward_adjoint_matches_separate_kernels(dtype, compute_f64):
                          [host(g) for g in G.grad_u0s], [i["iterations"] for i in G.infos()])
             G.close()
     finally:
-        A.set_tune(dtype, diag_max_l=base["diag_max_l"], fuse_max_n=base["fuse_max_n"])
+        A.set_tune(dtype, diag_max_l=base["diag_max_l"], fuse_max_n=base["fuse_max_n"],
+                   adj_algo=base["adj_algo"])
     for k in range(3):
         for a, b in zip(out[1][k], out[0][k]):
             assert bits_equal(a, b), (dtype, compute_f64, k)
     assert out[1][3] == out[0][3]
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_tile_adjoint_matches_node_adjoint_bitwise(dtype):
+    # both traversals resolve every node with the same expression in the same
+    # operand order, so they agree bit for bit (uneven boxes, partial tiles)
+    import adtomo_hip as A
+    base = A.get_tune(dtype)
+    npd = np.float64 if dtype == "float64" else np.float32
+    fs, u0s, gs = _mixed_batch(700)
+    u = [dev(oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-4, dtype=npd)[0], npd) for e in range(len(fs))]
+    out = {}
+    try:
+        for algo in (0, 1):
+            A.set_tune(dtype, adj_algo=algo)
+            r = A.eikonal3d_batched([dev(a, npd) for a in u0s], [dev(a, npd) for a in fs],
+                                    [0.5] * len(fs), [1e-4] * len(fs),
+                                    grad_us=[dev(a, npd) for a in gs], us=u, forward=False)
+            assert all(i["adj_converged"] for i in r["info"]), (dtype, algo)
+            out[algo] = ([host(x) for x in r["grad_f"]], [host(x) for x in r["grad_u0"]])
+    finally:
+        A.set_tune(dtype, adj_algo=base["adj_algo"])
+    for k in range(2):
+        for e, (a, b) in enumerate(zip(out[0][k], out[1][k])):
+            assert bits_equal(a, b), (dtype, k, e)
+
+
+def test_tile_adjoint_random_u0_large_z_set():
+    # random u0 makes many nodes keep u == u0 (the Z set, lambda = 0) and cuts
+    # the upwind graph into many pieces; against the oracle's exact solve
+    import adtomo_hip as A
+    shp = (21, 34, 19)
+    r = synth.rng(811)
+    f = r.uniform(0.5, 2.0, size=shp)
+    u0 = r.uniform(0.0, 6.0, size=shp)
+    g = r.normal(size=shp)
+    u, _, _ = oracle.eikonal3d(u0, f, 0.5, 1e-6)
+    gu0_ref, gf_ref, _ = oracle.eikonal3d_grad(g, u, u0, f, 0.5)
+    nz = int(np.sum(u == u0))
+    print(f"Z nodes {nz} of {u.size}", flush=True)
+    gu0, gf = A.eikonal3d_grad(dev(g), dev(u), dev(u0), dev(f), 0.5, *shp)
+    info = A.ops.last_info
+    assert info["adj_converged"] == 1 and info["fault"] == 0
+    assert rel(host(gf), gf_ref) <= GRAD_RTOL
+    assert bits_equal(host(gu0), gu0_ref)
+
+
+def test_tile_adjoint_large_tile_count_variant():
+    # more than 8192 tiles of 8^3 (here 34 x 32 x 8 = 8704): the variant with
+    # fewer waves and larger scheduling bitmaps
+    import adtomo_hip as A
+    shp = (272, 256, 64)
+    h, tol = 0.5, 1e-3
+    f = 1.0 / synth.layered_grf_model(shp, h, seed=21)
+    u0 = synth.point_source_u0(shp, (200, 40, 30))
+    g = synth.sparse_adjoint_source(shp, 80, seed=22)
+    print(f"event {shp}: oracle", flush=True)
+    u, it, _ = oracle.eikonal3d(u0, f, h, tol)
+    _, gf_ref, _ = oracle.eikonal3d_grad(g, u, u0, f, h)
+    print(f"event {shp}: device, K={it}", flush=True)
+    _, gf = A.eikonal3d_grad(dev(g), dev(u), dev(u0), dev(f), h, *shp)
+    info = A.ops.last_info
+    print(f"tile visits {info['adj_sweeps']}", flush=True)
+    assert info["adj_converged"] == 1 and info["fault"] == 0
+    assert rel(host(gf), gf_ref) <= GRAD_RTOL
 
 
 def test_autotune_stores_a_valid_setting():

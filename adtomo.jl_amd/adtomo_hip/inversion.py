@@ -27,22 +27,43 @@ Without chunking (``chunk`` = number of stations) the initial times u0_s,
 which depend on the fixed starting model only (inversion.jl:48-60), are
 built once; with chunking they are rebuilt on the device per chunk.
 
-The optimiser is SciPy's L-BFGS-B on the flattened parameters (the
-reference uses Optim.jl's LBFGS with a backtracking line search, which is
-not available here); objective and gradient are the reference's.
-Checkpoints: every ``steps`` gradient evaluations the parameters are saved
+The optimiser is the reference's: L-BFGS (history 10) with a static unit
+initial step and order-3 backtracking (``adtomo_hip.optim``, restating
+Optim.LBFGS(alphaguess=InitialStatic(), linesearch=BackTracking()) of
+src/mpi_optimize.jl:35-39), driven with the value alone at line-search trial
+points and value plus gradient at accepted points, as Optim drives the
+separate f and g! of src/mpi_optimize.jl:10-33.
+Checkpoints: every ``steps`` gradient calls the point of that call is saved
 as ``iter_<k>.npz`` (keys ``x`` and, for the joint inversion, ``vpvs``)
-under ``checkpoint_dir`` (the reference writes ``iter_<k>.h5``,
-src/mpi_optimize.jl:24-28).
+under ``checkpoint_dir`` (the reference writes ``iter_<k>.h5`` from its g!,
+src/mpi_optimize.jl:22-28; h5py is not installed).
 """
 from __future__ import annotations
 
 import os
+import warnings
 
 import numpy as np
 
+from . import _lib as L
 from . import objective as O
 from .ops import EikonalGraph, _torch
+
+
+def _check_solves(infos, forward=True, adjoint=True):
+    """Checks the per-solve diagnostics of a launch before its loss or
+    gradient is used: a kernel fault (``infos`` raises) or an adjoint that
+    did not resolve every node invalidates the results and raises; a forward
+    that reached the reference's 20-iteration cap without meeting the tolerance
+    is what the reference op also returns silently (Eikonal3D.cpp:74-88), so
+    it only warns."""
+    if forward and any(not i["converged"] for i in infos):
+        bad = sum(1 for i in infos if not i["converged"])
+        warnings.warn(f"{bad} of {len(infos)} forward solves reached the iteration cap "
+                      "without meeting the tolerance", RuntimeWarning, stacklevel=3)
+    if adjoint and any(not i["adj_converged"] for i in infos):
+        bad = [e for e, i in enumerate(infos) if not i["adj_converged"]]
+        raise L.AdtError(f"adjoint did not resolve every node for solves {bad[:8]}")
 
 
 class TravelTimeInversion:
@@ -129,9 +150,10 @@ class TravelTimeInversion:
     @staticmethod
     def slot_bytes(N, elem):
         """Device bytes of one solve slot: u0, u, dL/du, dL/du0 and dL/df
-        (N elements each) plus the batched workspace (forward snapshot,
-        adjoint records of 4 elements, 16-bit node states, int32 queue)."""
-        return N * (5 * elem + elem + 4 * elem + 2 + 4)
+        (N elements each) plus the batched workspace: the forward snapshot
+        and the tile adjoint's multipliers (N elements each), its upwind
+        codes (N bytes) and per-tile counts (N/512 int32)."""
+        return N * (7 * elem + 1) + 4 * (N // 512 + 64)
 
     def fitting_chunk(self, ns, nphases, N, elem, budget=0.8):
         """All stations if their slots fit in ``budget`` of the free device
@@ -163,6 +185,7 @@ class TravelTimeInversion:
             # one launch: forward, misfit and adjoint of every station (the
             # adjoint is computed also when only the loss is asked for)
             self.step.launch()
+            _check_solves(self.step.infos())
             loss = {}
             for q, p in enumerate(self.phases):
                 loss[p] = float(self.step.losses[q * C:(q + 1) * C].cpu().numpy().sum())
@@ -177,6 +200,7 @@ class TravelTimeInversion:
                     O.source_u0(self.stations[start:start + k], self.vel0, self.h,
                                 self._u0_scale(p), out=self.u0s[q * C:q * C + k])
             self.fwd.launch()
+            _check_solves(self.fwd.infos()[:len(self.u0s)], adjoint=False)
             for q, p in enumerate(self.phases):
                 sl = slice(q * C, q * C + k)
                 mis = O.misfit(self.fwd.us[sl], *self.events, self.tobs[p][start:start + k],
@@ -187,6 +211,7 @@ class TravelTimeInversion:
             if not grad:
                 continue
             self.adj.launch()
+            _check_solves(self.adj.infos(), forward=False)
             for q, p in enumerate(self.phases):
                 parts = self.adj.grad_fs[q * C:q * C + k]
                 O.sum_arrays(parts if start == 0 else [self.grad_f[p]] + parts, out=self.grad_f[p])
@@ -229,24 +254,30 @@ class TravelTimeInversion:
         extra = {"vpvs": params[-1]} if self.joint else {}
         np.savez(path, x=params[:self.N].reshape(self.shape), **extra)
 
-    def run(self, iterations=10, steps=None, checkpoint_dir=None, x0=None):
-        """L-BFGS over the parameters; returns (params, loss history)."""
-        from scipy.optimize import minimize
-        history = []
+    def run(self, iterations=10, steps=None, checkpoint_dir=None, x0=None, callback=None):
+        """L-BFGS over the parameters as src/mpi_optimize.jl:35-48 runs it
+        (``adtomo_hip.optim.lbfgs``: the value alone at line-search trial
+        points, value and gradient at accepted points); every ``steps``
+        gradient calls the point of the call is checkpointed before its
+        gradient is computed (src/mpi_optimize.jl:22-28).  Returns (params,
+        loss after each iteration); the full optimiser result is kept in
+        ``self.result``."""
+        from .optim import lbfgs
         calls = {"grad": 0}
 
-        def fun(xf):
-            L, g = self.objective(xf)
-            history.append(L)
+        def f(xf):
+            return self.objective(xf, grad=False)
+
+        def fg(xf):
             calls["grad"] += 1
             if checkpoint_dir and steps and calls["grad"] % steps == 0:
                 os.makedirs(checkpoint_dir, exist_ok=True)
                 self.save_checkpoint(os.path.join(checkpoint_dir, f"iter_{calls['grad']}.npz"), xf)
-            return L, g
+            return self.objective(xf)
 
         x0 = self.initial_params() if x0 is None else np.asarray(x0, dtype=np.float64).reshape(-1)
-        res = minimize(fun, x0, jac=True, method="L-BFGS-B", options={"maxiter": iterations})
-        return res.x, history
+        self.result = lbfgs(f, fg, x0, iterations=iterations, callback=callback)
+        return self.result.x, self.result.history
 
     def close(self):
         for g in (self.step, self.fwd, self.adj):

@@ -65,8 +65,9 @@ def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None, h
     dict(loss=tensor[nstation], tcal=[...], grad_u=[...])."""
     torch = _torch()
     ns = len(us)
-    nrec = len(rx)
     dt = us[0].dtype
+    ax, ay, az = _check_picks(dt, ns, rx, ry, rz, tobs, weights)
+    nrec = len(ax)
     _check_dev(*us, *tobs, *weights)
     loss = torch.zeros(ns, dtype=dt, device=us[0].device)
     tcals = [torch.empty(nrec, dtype=dt, device=us[0].device) for _ in range(ns)] if tcal else None
@@ -83,9 +84,6 @@ def misfit(us, rx, ry, rz, tobs, weights, grad=True, tcal=True, grad_out=None, h
                                tcals[e].data_ptr() if tcal else None,
                                grads[e].data_ptr() if grad else None, loss.data_ptr() + e * esz,
                                float(huber_delta))
-    ax = np.ascontiguousarray(rx, dtype=np.float64)
-    ay = np.ascontiguousarray(ry, dtype=np.float64)
-    az = np.ascontiguousarray(rz, dtype=np.float64)
     fn = getattr(L.lib(), "adt_misfit3d_batched" + _sfx(dt))
     rc = fn(ctypes.byref(tab), ns, ax.ctypes.data_as(ctypes.c_void_p),
             ay.ctypes.data_as(ctypes.c_void_p), az.ctypes.data_as(ctypes.c_void_p), nrec, _stream())
@@ -105,7 +103,32 @@ def _step_items(us, tobs, weights, grad_us, losses, huber_delta):
 
 
 def _coords(rx, ry, rz):
-    return tuple(np.ascontiguousarray(a, dtype=np.float64) for a in (rx, ry, rz))
+    """Host coordinate arrays of the receivers (equal lengths)."""
+    ax, ay, az = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rx, ry, rz))
+    if not (len(ax) == len(ay) == len(az)):
+        raise ValueError(f"rx, ry and rz must have equal lengths (got {len(ax)}, {len(ay)}, "
+                         f"{len(az)})")
+    return ax, ay, az
+
+
+def _check_picks(dtype, nsolve, rx, ry, rz, tobs, weights):
+    """Validates the receivers and picks of ``nsolve`` solves before any
+    device work (the library cannot check device array lengths): equal
+    coordinate lengths, one tobs and one weights tensor per solve, each of
+    the travel times' dtype with one value per receiver.  Returns the host
+    coordinate arrays."""
+    ax, ay, az = _coords(rx, ry, rz)
+    nrec = len(ax)
+    if len(tobs) != nsolve or len(weights) != nsolve:
+        raise ValueError(f"expected one tobs and one weights tensor per solve ({nsolve}), got "
+                         f"{len(tobs)} and {len(weights)}")
+    for name, ts in (("tobs", tobs), ("weights", weights)):
+        for e, t in enumerate(ts):
+            if t.dtype != dtype:
+                raise ValueError(f"{name}[{e}] has dtype {t.dtype}, the travel times {dtype}")
+            if t.numel() != nrec:
+                raise ValueError(f"{name}[{e}] has {t.numel()} values for {nrec} receivers")
+    return ax, ay, az
 
 
 def inversion_step(u0s, fs, hs, tols, rx, ry, rz, tobs, weights, huber_delta=0.0,
@@ -117,6 +140,7 @@ def inversion_step(u0s, fs, hs, tols, rx, ry, rz, tobs, weights, huber_delta=0.0
     info)."""
     torch = _torch()
     n = len(fs)
+    ax, ay, az = _check_picks(fs[0].dtype, n, rx, ry, rz, tobs, weights)
     us = [torch.empty_like(f) for f in fs]
     gus = [torch.empty_like(f) for f in fs]
     gu0s = [torch.empty_like(f) for f in fs]
@@ -125,7 +149,6 @@ def inversion_step(u0s, fs, hs, tols, rx, ry, rz, tobs, weights, huber_delta=0.0
     _check_dev(*tobs, *weights)
     tab, dtype = _event_table(u0s, fs, hs, tols, gus, us, gu0s, gfs)
     items = _step_items(us, tobs, weights, gus, losses, huber_delta)
-    ax, ay, az = _coords(rx, ry, rz)
     infos = (L.AdtInfo * max(n, 1))()
     rc = getattr(L.lib(), "adt_eikonal3d_misfit_batched" + _sfx(dtype))(
         ctypes.byref(tab), ctypes.byref(items), n, ax.ctypes.data_as(ctypes.c_void_p),
@@ -153,6 +176,7 @@ class InversionStepGraph:
         torch = _torch()
         self.u0s, self.fs, self.tobs, self.weights = list(u0s), list(fs), list(tobs), list(weights)
         self.nev = len(fs)
+        ax, ay, az = _check_picks(fs[0].dtype, self.nev, rx, ry, rz, self.tobs, self.weights)
         self.us = [torch.empty_like(f) for f in fs]
         self.grad_us = [torch.empty_like(f) for f in fs]
         self.grad_u0s = [torch.empty_like(f) for f in fs]
@@ -164,7 +188,6 @@ class InversionStepGraph:
                                   self.grad_u0s, self.grad_fs)
         items = _step_items(self.us, self.tobs, self.weights, self.grad_us, self.losses,
                             huber_delta)
-        ax, ay, az = _coords(rx, ry, rz)
         flags = _flags(True, self.grad_us, compute_f64)
         self._g = ctypes.c_void_p()
         rc = L.lib().adt_graph_create_misfit(

@@ -233,3 +233,44 @@ def test_objective_and_tuning_argument_errors_without_device():
     assert lib.adt_graph_create_misfit(evs, (L.AdtMisfit3d * 1)(good), 1, xyz, xyz, far, 1, both,
                                        0, ctypes.byref(g), None) == -1
     assert b"outside" in lib.adt_last_error()
+    # two events of one inversion step sharing a grad_u (or a u) buffer
+    p2 = p + 8 * 32
+    ev2 = (L.AdtEvent3d * 2)(L.AdtEvent3d(4, 4, 4, 1.0, 1e-3, p, p, p, p, None, p),
+                             L.AdtEvent3d(4, 4, 4, 1.0, 1e-3, p, p, p, p2, None, p))
+    it2 = (L.AdtMisfit3d * 2)(L.AdtMisfit3d(4, 4, 4, p, p, p, None, p, p, 0.0),
+                              L.AdtMisfit3d(4, 4, 4, p2, p, p, None, p, p, 0.0))
+    assert lib.adt_eikonal3d_misfit_batched_f64(ev2, it2, 2, xyz, xyz, xyz, 1, both, None,
+                                                None) == -1
+    assert b"share one grad_u" in lib.adt_last_error()
+    ev3 = (L.AdtEvent3d * 2)(L.AdtEvent3d(4, 4, 4, 1.0, 1e-3, p, p, p, p, None, p),
+                             L.AdtEvent3d(4, 4, 4, 1.0, 1e-3, p, p, p2, p, None, p))
+    it3 = (L.AdtMisfit3d * 2)(L.AdtMisfit3d(4, 4, 4, p, p, p, None, p, p, 0.0),
+                              L.AdtMisfit3d(4, 4, 4, p, p, p, None, p2, p, 0.0))
+    assert lib.adt_graph_create_misfit(ev3, it3, 2, xyz, xyz, xyz, 1, both, 0, ctypes.byref(g),
+                                       None) == -1
+    assert b"share one u" in lib.adt_last_error()
+
+
+def test_picks_and_coordinates_rejected_before_device_work():
+    # the device cannot check the lengths of the pick arrays: the host
+    # validates dtype, length and coordinate lengths first (CPU tensors here)
+    import torch
+    from adtomo_hip import objective as O
+    u = [torch.zeros((4, 4, 4), dtype=torch.float64) for _ in range(2)]
+    rx = ry = rz = [2.0, 3.0, 1.5]
+    t64 = [torch.zeros(3, dtype=torch.float64) for _ in range(2)]
+    with pytest.raises(ValueError, match="equal lengths"):
+        O.misfit(u, rx, ry, [2.0, 3.0], t64, t64)
+    with pytest.raises(ValueError, match="dtype"):
+        O.misfit(u, rx, ry, rz, [t.float() for t in t64], t64)
+    with pytest.raises(ValueError, match="values for 3 receivers"):
+        O.misfit(u, rx, ry, rz, t64, [torch.zeros(2, dtype=torch.float64)] * 2)
+    with pytest.raises(ValueError, match="one tobs and one weights"):
+        O.misfit(u, rx, ry, rz, t64[:1], t64)
+    for fn in (O.inversion_step, O.InversionStepGraph):
+        with pytest.raises(ValueError, match="equal lengths"):
+            fn(u, u, [1.0] * 2, [1e-3] * 2, rx, [2.0], rz, t64, t64)
+        with pytest.raises(ValueError, match="values for 3 receivers"):
+            fn(u, u, [1.0] * 2, [1e-3] * 2, rx, ry, rz, t64, [torch.zeros(4, dtype=torch.float64)] * 2)
+        with pytest.raises(ValueError, match="dtype"):
+            fn(u, u, [1.0] * 2, [1e-3] * 2, rx, ry, rz, [t.float() for t in t64], t64)

@@ -129,6 +129,9 @@ def lib():
             fn = getattr(L, base + sfx)
             fn.argtypes = [real if a == "R" else a for a in args]
             fn.restype = _i32
+    L.adt_eikonal3d_fwd_noalloc_f64.argtypes = [_p, _p, _d, _i64, _i64, _i64, _d, _i32, _p, _p,
+                                                _p]
+    L.adt_eikonal3d_fwd_noalloc_f64.restype = _i32
     L.adt_eikonal3d_fwd_plain_f64.argtypes = [_p, _p, _d, _i64, _i64, _i64, _d, _p, _p, _p]
     L.adt_eikonal3d_fwd_plain_f64.restype = _i32
     L.adt_eikonal3d_bwd_plain_f64.argtypes = [_p, _p, _p, _p, _d, _i64, _i64, _i64, _p, _p, _p, _p]

@@ -263,6 +263,87 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   return ADT_OK;
 }
 
+// ------------------------------------------- 3D forward, no allocation ---
+//
+// adt_eikonal3d_fwd_noalloc_f64: the forward of adt_eikonal3d_fwd_f64 with its
+// workspace (iteration snapshot, event descriptor, diagnostics) in a fixed
+// device array instead of a stream-ordered allocation, for callers that
+// cannot allocate.  Every call on a device shares that device's array, so
+// calls that overlap on one device race.
+
+constexpr size_t kNoAllocSnapBytes = size_t(16) << 20;  // snapshot: 2^21 FP64 nodes
+constexpr size_t kNoAllocDescOff = kNoAllocSnapBytes;
+constexpr size_t kNoAllocInfoOff = kNoAllocDescOff + 256;
+constexpr size_t kNoAllocBytes = kNoAllocInfoOff + ((sizeof(adt_info) + 255) / 256) * 256;
+static_assert(sizeof(adt::EventDesc) <= 256, "descriptor slot");
+
+// Held by each device the library runs on until the process ends.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device with that device current (the array's name is a host-side stand-in).
+int scratch_ptr(void** out) {
+  int dev = -1;
+  ADT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ADT_ENODEV, "noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ADT_EHIP, std::string("noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ADT_OK;
+}
+
+int fwd3_noalloc_impl(const double* u0, const double* f, double h, int64_t m, int64_t n, int64_t l,
+                      double tol, int verbose, double* u, adt_info* info, void* stream) {
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (!u0 || !f || !u) return fail(ADT_EINVAL, "eikonal3d noalloc: NULL buffer");
+  const int64_t N = m * n * l;
+  if ((size_t)N * sizeof(double) > kNoAllocSnapBytes)
+    return fail(ADT_EINVAL, "eikonal3d noalloc: " + std::to_string(N) + " nodes exceed the " +
+                                std::to_string(kNoAllocSnapBytes / sizeof(double)) +
+                                "-node limit of the fixed workspace");
+  void* base = nullptr;
+  if ((rc = scratch_ptr(&base))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  char* b = (char*)base;
+  adt::EventDesc d{};
+  d.m = (int)m;
+  d.n = (int)n;
+  d.l = (int)l;
+  d.h = h;
+  d.tol = tol;
+  d.u0 = u0;
+  d.f = f;
+  d.u = u;
+  d.uold = b;  // written before it is read (u0 is the first snapshot)
+  d.info = (adt_info*)(b + kNoAllocInfoOff);
+  ADT_HIP(hipMemcpyAsync(b + kNoAllocDescOff, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
+  const adt_tune tn = current_tune<double>();
+  if (l <= tn.diag_max_l)
+    launch_fwd_diag<double>(false, 1, (const adt::EventDesc*)(b + kNoAllocDescOff), tn.diag_max_l, s);
+  else
+    launch_fwd_stream<double>(false, 1, (const adt::EventDesc*)(b + kNoAllocDescOff), tn.diag_max_l, s);
+  ADT_HIP(hipGetLastError());
+  adt_info hinf{};
+  ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
+  ADT_HIP(hipStreamSynchronize(s));
+  if (verbose) print_verbose(hinf);
+  if (info) *info = hinf;
+  if (hinf.fault) return fail(ADT_EHIP, "eikonal3d: wave synchronisation timed out in the kernel");
+  return ADT_OK;
+}
+
 // ------------------------------------------------------------ 3D adjoint ---
 
 template <typename T>
@@ -1303,6 +1384,11 @@ int adt_eikonal3d_fwd_f32(const float* u0, const float* f, float h, int64_t m, i
                           int64_t l, float tol, int32_t verbose, float* u, adt_info* info,
                           int32_t flags, void* stream) {
   return fwd3_impl<float>(u0, f, h, m, n, l, tol, verbose, u, info, flags, stream);
+}
+int adt_eikonal3d_fwd_noalloc_f64(const double* u0, const double* f, double h, int64_t m,
+                                  int64_t n, int64_t l, double tol, int32_t verbose, double* u,
+                                  adt_info* info, void* stream) {
+  return fwd3_noalloc_impl(u0, f, h, m, n, l, tol, verbose, u, info, stream);
 }
 int adt_eikonal3d_bwd_f64(const double* grad_u, const double* u, const double* u0,
                           const double* f, double h, int64_t m, int64_t n, int64_t l,

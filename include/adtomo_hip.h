@@ -114,6 +114,18 @@ int adt_eikonal3d_fwd_f32(const float *u0, const float *f, float h, int64_t m, i
                           int64_t l, float tol, int32_t verbose, float *u, adt_info *info,
                           int32_t flags, void *stream);
 
+/* ---- 3D forward without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as adt_eikonal3d_fwd_f64 (bit for bit), device pointers only; the
+ * iteration snapshot, descriptor and diagnostics live in a fixed device
+ * array of 16 MiB + 4 KiB that each device holds until the process ends, so
+ * events are limited to 2^21 nodes (ADT_EINVAL beyond) and calls that
+ * overlap on one device race (serialise them, e.g. on one stream). */
+int adt_eikonal3d_fwd_noalloc_f64(const double *u0, const double *f, double h, int64_t m,
+                                  int64_t n, int64_t l, double tol, int32_t verbose, double *u,
+                                  adt_info *info, void *stream);
+
 /* ---- 3D: op EikonalThreeDGrad (EikonalThreeD.cpp:47-65, Compute :154-250).
  * Produces grad_u0 and grad_f (the only meaningful outputs of the reference). */
 int adt_eikonal3d_bwd_f64(const double *grad_u, const double *u, const double *u0,

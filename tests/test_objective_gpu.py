@@ -373,7 +373,9 @@ def test_inversion_step_matches_separate_launches(fuse, dtype):
     base = A.get_tune(dtype)
     graphs = []
     try:
-        A.set_tune(dtype, fuse_max_n=(1 << 30) if fuse else 0)
+        # the fused kernel (forward, misfit, adjoint in one workgroup) runs
+        # with the node-level adjoint
+        A.set_tune(dtype, fuse_max_n=(1 << 30) if fuse else 0, adj_algo=1 if fuse else 0)
         f = dev(1.0 / vel0, npd)
         u0s = O.source_u0(stations, dev(vel0, npd), h)
         tb = [dev(tobs[s], npd) for s in range(n)]
@@ -404,6 +406,6 @@ def test_inversion_step_matches_separate_launches(fuse, dtype):
         assert all(i["converged"] and i["adj_converged"] for i in step.infos())
         assert all(i["converged"] and i["adj_converged"] for i in one["info"])
     finally:
-        A.set_tune(dtype, fuse_max_n=base["fuse_max_n"])
+        A.set_tune(dtype, fuse_max_n=base["fuse_max_n"], adj_algo=base["adj_algo"])
         for g in graphs:
             g.close()

@@ -85,8 +85,6 @@ __global__ void model_grad_kernel(const T* x, const T* fvar, const T* grad_f, co
 template <typename T>
 __global__ void fwd3_plane_kernel(T* u, const T* f, T h, int m, int n, int l, int dir, int s,
                                   int a_lo, int a_cnt);
-template <typename T>
-__global__ void fwd3_event_kernel(const EventDesc* evs, int max_iter);
 // MIX (T = float only): FP32 storage with every local solve in FP64,
 // rounded once to FP32.
 template <typename T, bool MIX>

@@ -617,10 +617,16 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       const lptr<T> b_up = pn + p + 1, b_dn = px + p + 1;
       const lptr<T> a_plus = S.flipA ? a_up : a_dn, a_minus = S.flipA ? a_dn : a_up;
       const lptr<T> b_plus = S.flipB ? b_up : b_dn, b_minus = S.flipB ? b_dn : b_up;
-      const lptr<T> x_plus = g.a_is_i ? a_plus : b_plus, x_minus = g.a_is_i ? a_minus : b_minus;
-      const lptr<T> y_plus = g.a_is_i ? b_plus : a_plus, y_minus = g.a_is_i ? b_minus : a_minus;
+      const lptr<T> x_plus0 = g.a_is_i ? a_plus : b_plus, x_minus0 = g.a_is_i ? a_minus : b_minus;
+      const lptr<T> y_plus0 = g.a_is_i ? b_plus : a_plus, y_minus0 = g.a_is_i ? b_minus : a_minus;
       const bool xf = g.a_is_i ? af : bf, xl = g.a_is_i ? al : bl;
       const bool yf = g.a_is_i ? bf : af, yl = g.a_is_i ? bl : al;
+      // One-sided neighbours at the grid edges (Eikonal3D.cpp:47-52): both
+      // operands of the minimum read the one existing neighbour, and
+      // smin(v, v) is v bit for bit (NaN included), so the per-node edge
+      // selects become one address choice per step.
+      const lptr<T> x_plus = xl ? x_minus0 : x_plus0, x_minus = xf ? x_plus0 : x_minus0;
+      const lptr<T> y_plus = yl ? y_minus0 : y_plus0, y_minus = yf ? y_plus0 : y_minus0;
 #pragma unroll
       for (int e = 0; e < kLW; ++e) {
         if (e == 0) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
@@ -642,8 +648,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
           if (cv.valid) {
             const int cg = S.cs + c;  // canonical c in the whole event
             const int rc = S.flipC ? S.Lfull - 1 - cg : cg;
-            const T ux = xf ? xp : (xl ? xm : smin(xp, xm));
-            const T uy = yf ? yp : (yl ? ym : smin(yp, ym));
+            const T ux = smin(xp, xm);
+            const T uy = smin(yp, ym);
             const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == S.Lfull - 1);
             T cand;
             if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once

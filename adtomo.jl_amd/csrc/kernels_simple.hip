@@ -81,85 +81,6 @@ __global__ void fwd3_plane_kernel(T* __restrict__ u, const T* __restrict__ f, T 
   fwd3_node(u, f, h, m, n, l, i, j, k);
 }
 
-// One 8-sweep iteration of one event inside one workgroup.
-template <typename T>
-__device__ void fwd3_iteration_wg(T* __restrict__ u, const T* __restrict__ f, T h, int m, int n,
-                                  int l) {
-  const int S = m + n + l - 2;
-  for (int sw = 0; sw < 8; ++sw) {
-    const int dir = kDir3[sw];
-    for (int s = 0; s < S; ++s) {
-      int a_lo, a_hi;
-      plane_a_range(s, m, n, l, &a_lo, &a_hi);
-      const int cnt = (a_hi - a_lo + 1) * n;
-      for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-        int a = a_lo + t / n, b = t % n, c = s - a - b;
-        if (c < 0 || c >= l) continue;
-        int i = (dir & 1) ? m - 1 - a : a;
-        int j = (dir & 2) ? n - 1 - b : b;
-        int k = (dir & 4) ? l - 1 - c : c;
-        fwd3_node(u, f, h, m, n, l, i, j, k);
-      }
-      __syncthreads();
-    }
-  }
-}
-
-template <typename T>
-__device__ T wg_max_key(uint64_t key, uint64_t* red) {
-  for (int off = 32; off > 0; off >>= 1) {
-    uint64_t o = __shfl_xor(key, off);
-    key = o > key ? o : key;
-  }
-  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[w] = key;
-  __syncthreads();
-  uint64_t r = 0;
-  for (int q = 0; q < nw; ++q) r = red[q] > r ? red[q] : r;
-  return key_to_real(r, T(0));
-}
-
-// Persistent forward: blockIdx.x = event.  Implements Eikonal3D.cpp:71-88.
-template <typename T>
-__global__ void __launch_bounds__(1024) fwd3_event_kernel(const EventDesc* __restrict__ evs,
-                                                          int max_iter) {
-  __shared__ uint64_t red[16];
-  const EventDesc& ev = evs[blockIdx.x];
-  const int m = ev.m, n = ev.n, l = ev.l;
-  const int64_t N = (int64_t)m * n * l;
-  T* u = (T*)ev.u;
-  T* uold = (T*)ev.uold;
-  const T* f = (const T*)ev.f;
-  const T* u0 = (const T*)ev.u0;
-  const T h = (T)ev.h, tol = (T)ev.tol;
-  for (int64_t p = threadIdx.x; p < N; p += blockDim.x) u[p] = u0[p];
-  __syncthreads();
-  int it = 0;
-  T err = T(0);
-  for (it = 0; it < max_iter; ++it) {
-    for (int64_t p = threadIdx.x; p < N; p += blockDim.x) uold[p] = u[p];
-    __syncthreads();
-    fwd3_iteration_wg(u, f, h, m, n, l);
-    uint64_t key = 0;
-    for (int64_t p = threadIdx.x; p < N; p += blockDim.x) {
-      uint64_t q = maxkey(Real<T>::fabs_(u[p] - uold[p]));
-      key = q > key ? q : key;
-    }
-    err = wg_max_key<T>(key, red);
-    if (threadIdx.x == 0 && it < kMaxLoggedIters) ev.info->err_log[it] = (double)err;
-    if (err < tol) {
-      ++it;
-      break;
-    }
-  }
-  if (threadIdx.x == 0) {
-    ev.info->iterations = it > max_iter ? max_iter : it;
-    ev.info->converged = (err < tol) ? 1 : 0;
-    ev.info->err = (double)err;
-  }
-}
-
 // Element-wise helpers for the per-launch path.
 template <typename T>
 __global__ void copy_kernel(T* __restrict__ dst, const T* __restrict__ src, int64_t N) {
@@ -397,7 +318,7 @@ __global__ void arith_kernel(int op, const T* __restrict__ a, const T* __restric
     T r;
     switch (op) {
       case 0: r = Real<T>::sqrt_(x); break;
-      case 1: r = x / T(3.0); break;
+      case 1: r = div3(x); break;  // the division the local solve uses
       case 2: r = x / T(2.0); break;
       case 3: r = x / b[p]; break;
       case 4: r = solution3d(a[3 * p], a[3 * p + 1], a[3 * p + 2], b[2 * p], b[2 * p + 1]); break;
@@ -412,7 +333,6 @@ __global__ void arith_kernel(int op, const T* __restrict__ a, const T* __restric
 #define ADT_INSTANTIATE(T)                                                                      \
   template __global__ void fwd3_plane_kernel<T>(T*, const T*, T, int, int, int, int, int, int, \
                                                 int);                                           \
-  template __global__ void fwd3_event_kernel<T>(const EventDesc*, int);                         \
   template __global__ void copy_kernel<T>(T*, const T*, int64_t);                               \
   template __global__ void maxdiff_kernel<T>(const T*, const T*, int64_t, unsigned long long*); \
   template __global__ void adj3_prep_kernel<T>(const T*, const T*, const T*, int, int, int,     \

@@ -68,7 +68,9 @@ def test_device_arith_is_ieee():
     edges = np.concatenate([edges, np.nextafter(edges, np.inf), np.nextafter(edges, -np.inf)])
     bits = r.integers(0, 2 ** 63, 400000, dtype=np.int64).view(np.float64)
     x = np.concatenate([x, edges, -edges, bits, -bits])
-    for op, ref in ((0, np.sqrt), (1, lambda v: v / 3.0), (2, lambda v: v / 2.0)):
+    # the division by 3 of the local solve is IEEE's except that -0 gives +0
+    div3 = lambda v: np.where(v == 0, np.zeros_like(v), v / 3.0)  # noqa: E731
+    for op, ref in ((0, np.sqrt), (1, div3), (2, lambda v: v / 2.0)):
         with np.errstate(invalid="ignore"):
             want = ref(x)
         got = host(A.selftest_arith(op, dev(x)))
@@ -82,7 +84,12 @@ def test_device_arith_is_ieee():
                               r.integers(0, 2 ** 31, 400000, dtype=np.int32).view(np.float32)])
         x32 = np.concatenate([x32, -x32])
         assert bits_equal(host(A.selftest_arith(0, dev(x32))), np.sqrt(x32))
-        assert bits_equal(host(A.selftest_arith(1, dev(x32))), x32 / np.float32(3))
+        assert bits_equal(host(A.selftest_arith(1, dev(x32))),
+                          np.where(x32 == 0, np.float32(0), x32 / np.float32(3)))
+    # every binary32 subnormal and the lowest binades, both signs
+    sub = np.arange(1, 1 << 25, dtype=np.uint32).view(np.float32)
+    sub = np.concatenate([sub, -sub])
+    assert bits_equal(host(A.selftest_arith(1, dev(sub))), sub / np.float32(3))
 
 
 def test_selftest_copy():
@@ -106,6 +113,18 @@ def test_device_local_solvers_match_numpy():
     got2 = host(A.selftest_arith(5, dev(a[:, :2].reshape(-1)), dev(fh.reshape(-1))))
     want2 = hp.solution2d(a[:, 0], a[:, 1], fh[:, 0], fh[:, 1])
     assert bits_equal(got2, want2)
+    # every combination of special neighbour values (signed zeros,
+    # subnormals, values whose squares underflow or overflow, infinities,
+    # NaN) with special f and h: the 3D candidate's divisions by 3 must not
+    # change the result
+    sv = np.array([0.0, -0.0, 5e-324, 1e-310, 1e-160, 3e-155, 1.0, 7.0, -1.0, 1e154, 1e200,
+                   1.7976931348623157e308, np.inf, -np.inf, np.nan])
+    fv = np.array([0.0, 5e-324, 1e-160, 0.5, 1.0, 1e155, 1e200, np.inf])
+    g = np.stack(np.meshgrid(sv, sv, sv, fv, fv, indexing="ij"), -1).reshape(-1, 5)
+    got3 = host(A.selftest_arith(4, dev(g[:, :3].reshape(-1)), dev(g[:, 3:].reshape(-1))))
+    with np.errstate(all="ignore"):
+        want3 = hp.solution3d(g[:, 0], g[:, 1], g[:, 2], g[:, 3], g[:, 4])
+    assert bits_equal(got3, want3)
 
 
 # ------------------------------------------------------------ golden ---

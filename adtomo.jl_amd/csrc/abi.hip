@@ -4,6 +4,7 @@
 // tables, launches, diagnostics.  All arithmetic runs in the HIP kernels.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -11,6 +12,7 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -723,8 +725,106 @@ struct MisfitArgs {
 // Device view of the same, for the separate-kernel path.
 struct MisfitRun {
   const adt_misfit3d* table;
+  const adt::MisfitNodes* nodes;
   const double *rx, *ry, *rz;
   int nrec;
+};
+
+// The receivers of a misfit grouped by interpolation node, for each box
+// among its items (host side of the deterministic misfit gradient, see
+// adt::MisfitNodes), and each item's seed scratch.  build() reserves
+// workspace through take(bytes) -> offset; upload() copies the tables
+// (the host vectors must outlive the copies: callers synchronise).
+template <typename T>
+struct RecvNodes {
+  struct Shape {
+    int64_t n = 0, l = 0;
+    std::vector<int64_t> node, off, ent;
+    size_t o_node = 0, o_off = 0, o_ent = 0;
+  };
+  std::vector<Shape> shapes;
+  std::vector<int> shape_of;             // per item
+  std::vector<adt::MisfitNodes> descs;  // per item (upload)
+  size_t o_tab = 0, o_gj = 0, gj_bytes = 0;
+
+  // Corner nodes exactly as the device's trilinear_setup takes them: floor
+  // and ceil of each 1-based coordinate, one corner where they coincide.
+  template <class Take>
+  void build(const adt_misfit3d* items, int nitem, const double* rx, const double* ry,
+             const double* rz, int64_t nrec, Take take) {
+    shape_of.assign(nitem, 0);
+    for (int e = 0; e < nitem; ++e) {
+      size_t k = 0;
+      while (k < shapes.size() && !(shapes[k].n == items[e].n && shapes[k].l == items[e].l)) ++k;
+      if (k == shapes.size()) {
+        shapes.emplace_back();
+        shapes.back().n = items[e].n;
+        shapes.back().l = items[e].l;
+      }
+      shape_of[e] = (int)k;
+    }
+    for (auto& sh : shapes) {
+      std::vector<std::pair<int64_t, int64_t>> pr;  // (node, entry), receivers in order
+      pr.reserve((size_t)nrec * 8);
+      for (int64_t j = 0; j < nrec; ++j) {
+        const int xs[2] = {(int)std::floor(rx[j]), (int)std::ceil(rx[j])};
+        const int ys[2] = {(int)std::floor(ry[j]), (int)std::ceil(ry[j])};
+        const int zs[2] = {(int)std::floor(rz[j]), (int)std::ceil(rz[j])};
+        const int na = xs[0] != xs[1] ? 2 : 1, nb = ys[0] != ys[1] ? 2 : 1,
+                  nc = zs[0] != zs[1] ? 2 : 1;
+        for (int a = 0; a < na; ++a)
+          for (int b = 0; b < nb; ++b)
+            for (int c = 0; c < nc; ++c)
+              pr.emplace_back((int64_t)(xs[a] - 1) * sh.n * sh.l + (int64_t)(ys[b] - 1) * sh.l +
+                                  (zs[c] - 1),
+                              j * 8 + (a << 2) + (b << 1) + c);
+      }
+      std::stable_sort(pr.begin(), pr.end(),
+                       [](const std::pair<int64_t, int64_t>& x, const std::pair<int64_t, int64_t>& y) {
+                         return x.first < y.first;
+                       });
+      sh.off.push_back(0);
+      for (size_t i = 0; i < pr.size(); ++i) {
+        if (i == 0 || pr[i].first != pr[i - 1].first) {
+          if (i) sh.off.push_back((int64_t)i);
+          sh.node.push_back(pr[i].first);
+        }
+        sh.ent.push_back(pr[i].second);
+      }
+      sh.off.push_back((int64_t)pr.size());
+      if (sh.node.empty()) sh.off.assign(1, 0);
+      sh.o_node = take(sizeof(int64_t) * (sh.node.size() + 1));
+      sh.o_off = take(sizeof(int64_t) * sh.off.size());
+      sh.o_ent = take(sizeof(int64_t) * (sh.ent.size() + 1));
+    }
+    o_tab = take(sizeof(adt::MisfitNodes) * (nitem ? nitem : 1));
+    gj_bytes = ((sizeof(T) * (nrec ? nrec : 1) + 255) / 256) * 256;
+    o_gj = take(gj_bytes * (nitem ? nitem : 1));
+  }
+  int upload(char* base, int nitem, hipStream_t s) {
+    for (const auto& sh : shapes) {
+      if (!sh.node.empty()) {
+        ADT_HIP(hipMemcpyAsync(base + sh.o_node, sh.node.data(), sizeof(int64_t) * sh.node.size(),
+                               hipMemcpyHostToDevice, s));
+        ADT_HIP(hipMemcpyAsync(base + sh.o_ent, sh.ent.data(), sizeof(int64_t) * sh.ent.size(),
+                               hipMemcpyHostToDevice, s));
+      }
+      ADT_HIP(hipMemcpyAsync(base + sh.o_off, sh.off.data(), sizeof(int64_t) * sh.off.size(),
+                             hipMemcpyHostToDevice, s));
+    }
+    descs.resize(nitem);
+    for (int e = 0; e < nitem; ++e) {
+      const Shape& sh = shapes[shape_of[e]];
+      descs[e] = adt::MisfitNodes{(const int64_t*)(base + sh.o_node), (const int64_t*)(base + sh.o_off),
+                                  (const int64_t*)(base + sh.o_ent), (int64_t)sh.node.size(),
+                                  base + o_gj + gj_bytes * e};
+    }
+    if (nitem)
+      ADT_HIP(hipMemcpyAsync(base + o_tab, descs.data(), sizeof(adt::MisfitNodes) * nitem,
+                             hipMemcpyHostToDevice, s));
+    return ADT_OK;
+  }
+  const adt::MisfitNodes* table(char* base) const { return (const adt::MisfitNodes*)(base + o_tab); }
 };
 
 // Checks one misfit item against its event and the receivers against the
@@ -759,6 +859,7 @@ struct BatchPlan {
   // workspace
   const MisfitArgs* mis = nullptr;
   size_t off_mis = 0, off_coords = 0;
+  RecvNodes<T> rnodes;
 
   // Each event's receiver misfit runs between its forward and its adjoint
   // and writes its grad_u: the item must name the event's u and grad_u.
@@ -793,9 +894,11 @@ struct BatchPlan {
     };
     off_mis = take(sizeof(adt_misfit3d) * (nev ? nev : 1));
     off_coords = take(sizeof(double) * 3 * (m->nrec ? m->nrec : 1));
+    rnodes.build(m->items, nev, m->rx, m->ry, m->rz, m->nrec, take);
     for (int e = 0; e < nev; ++e) {
       // relative until bind(); descs are still in input order here
       descs[e].mis = (const adt_misfit3d*)(off_mis + sizeof(adt_misfit3d) * e);
+      descs[e].mnodes = (const adt::MisfitNodes*)(rnodes.o_tab + sizeof(adt::MisfitNodes) * e);
       descs[e].nrec = m->nrec;
     }
     return ADT_OK;
@@ -812,13 +915,13 @@ struct BatchPlan {
       ADT_HIP(hipMemcpyAsync(dc + mis->nrec, mis->ry, nb, hipMemcpyHostToDevice, s));
       ADT_HIP(hipMemcpyAsync(dc + 2 * mis->nrec, mis->rz, nb, hipMemcpyHostToDevice, s));
     }
-    return ADT_OK;
+    return rnodes.upload(base, nev, s);
   }
   MisfitRun misfit_run(char* base) const {
-    if (!mis) return MisfitRun{nullptr, nullptr, nullptr, nullptr, 0};
+    if (!mis) return MisfitRun{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
     const double* dc = (const double*)(base + off_coords);
-    return MisfitRun{(const adt_misfit3d*)(base + off_mis), dc, dc + mis->nrec,
-                     dc + 2 * mis->nrec, (int)mis->nrec};
+    return MisfitRun{(const adt_misfit3d*)(base + off_mis), rnodes.table(base), dc,
+                     dc + mis->nrec, dc + 2 * mis->nrec, (int)mis->nrec};
   }
 
   int build(const adt_event3d* ev, int nev, int flags, int adj_algo_req) {
@@ -905,6 +1008,7 @@ struct BatchPlan {
       if (mis) {
         const double* dc = (const double*)(base + off_coords);
         d.mis = (const adt_misfit3d*)(base + (size_t)d.mis);
+        d.mnodes = (const adt::MisfitNodes*)(base + (size_t)d.mnodes);
         d.rx = dc;
         d.ry = dc + mis->nrec;
         d.rz = dc + 2 * mis->nrec;
@@ -948,7 +1052,7 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, int
     }
   }
   if (mr.table) {  // the events' receiver misfit: its gradient is the adjoint source
-    adt::misfit_kernel<T><<<nev, 256, 0, s>>>(mr.table, mr.rx, mr.ry, mr.rz, mr.nrec);
+    adt::misfit_kernel<T><<<nev, 256, 0, s>>>(mr.table, mr.nodes, mr.rx, mr.ry, mr.rz, mr.nrec);
     ADT_HIP(hipGetLastError());
   }
   if (flags & ADT_ADJOINT) {
@@ -1582,19 +1686,30 @@ int misfit_impl(const adt_misfit3d* items, int32_t nitem, const double* rx, cons
     if (rc) return rc;
   }
   hipStream_t st = (hipStream_t)stream;
-  DevBuf tab, coords;
+  size_t used = 0;
+  auto take = [&](size_t b) {
+    size_t o = used;
+    used += ((b + 255) / 256) * 256;
+    return o;
+  };
+  const size_t o_tab = take(sizeof(adt_misfit3d) * nitem);
+  const size_t o_coords = take(sizeof(double) * 3 * (nrec ? nrec : 1));
+  RecvNodes<T> rn;
+  rn.build(items, nitem, rx, ry, rz, nrec, take);
+  DevBuf ws;
   int rc;
-  if ((rc = tab.alloc(sizeof(adt_misfit3d) * nitem, st))) return rc;
-  if ((rc = coords.alloc(sizeof(double) * 3 * (nrec ? nrec : 1), st))) return rc;
-  double* dc = (double*)coords.p;
-  ADT_HIP(hipMemcpyAsync(tab.p, items, sizeof(adt_misfit3d) * nitem, hipMemcpyHostToDevice, st));
+  if ((rc = ws.alloc(used, st))) return rc;
+  char* base = (char*)ws.p;
+  double* dc = (double*)(base + o_coords);
+  ADT_HIP(hipMemcpyAsync(base + o_tab, items, sizeof(adt_misfit3d) * nitem, hipMemcpyHostToDevice, st));
   if (nrec) {
     ADT_HIP(hipMemcpyAsync(dc, rx, sizeof(double) * nrec, hipMemcpyHostToDevice, st));
     ADT_HIP(hipMemcpyAsync(dc + nrec, ry, sizeof(double) * nrec, hipMemcpyHostToDevice, st));
     ADT_HIP(hipMemcpyAsync(dc + 2 * nrec, rz, sizeof(double) * nrec, hipMemcpyHostToDevice, st));
   }
-  adt::misfit_kernel<T><<<nitem, 256, 0, st>>>((const adt_misfit3d*)tab.p, dc, dc + nrec,
-                                                dc + 2 * nrec, (int)nrec);
+  if ((rc = rn.upload(base, nitem, st))) return rc;
+  adt::misfit_kernel<T><<<nitem, 256, 0, st>>>((const adt_misfit3d*)(base + o_tab), rn.table(base),
+                                                dc, dc + nrec, dc + 2 * nrec, (int)nrec);
   ADT_HIP(hipGetLastError());
   // the staged coordinates are host copies: wait before they go out of scope
   ADT_HIP(hipStreamSynchronize(st));

@@ -10,6 +10,8 @@ namespace adt {
 
 constexpr int kMaxLoggedIters = ADT_MAX_LOGGED_ITERS;
 
+struct MisfitNodes;
+
 // One 3D event as seen by the persistent kernels (device resident).
 struct EventDesc {
   int m, n, l, pad;
@@ -30,6 +32,7 @@ struct EventDesc {
   // becomes grad_u between the forward and the adjoint (fused kernel), and
   // the receiver coordinates it reads.
   const adt_misfit3d* mis;
+  const MisfitNodes* mnodes;
   const double* rx;
   const double* ry;
   const double* rz;
@@ -56,11 +59,24 @@ struct Event2Desc {
 using SourceDesc = adt_source3d;
 using MisfitDesc = adt_misfit3d;
 
+// A solve's receivers grouped by interpolation node (built on the host for
+// each box of a receiver set): node[u] (flat index, increasing, u < nuniq)
+// takes the entries ent[off[u] .. off[u+1]), each (receiver j) * 8 +
+// corner (bit 2: the x1 / x2 side, bit 1: y, bit 0: z), in increasing j.
+// gj: the solve's scratch of nrec adjoint seeds d loss / d t_j.
+struct MisfitNodes {
+  const int64_t* node;
+  const int64_t* off;
+  const int64_t* ent;
+  int64_t nuniq;
+  void* gj;
+};
+
 template <typename T>
 __global__ void source_u0_kernel(const SourceDesc* src, const T* vel0, int m, int n, int l, T h);
 template <typename T>
-__global__ void misfit_kernel(const MisfitDesc* d, const double* rx, const double* ry,
-                              const double* rz, int nrec);
+__global__ void misfit_kernel(const MisfitDesc* d, const MisfitNodes* nodes, const double* rx,
+                              const double* ry, const double* rz, int nrec);
 template <typename T>
 __global__ void model_kernel(const T* x, const T* vel0, int64_t N, T* fvar, T* f);
 template <typename T>

@@ -823,7 +823,7 @@ __global__ void __launch_bounds__(kDW * 64) fwd_adj3_diag_kernel(const EventDesc
   if (ev.mis) {
     // receiver misfit of these travel times: its gradient is the adjoint
     // source (same for every thread of the workgroup)
-    misfit_event<T>(*ev.mis, ev.rx, ev.ry, ev.rz, (int)ev.nrec);
+    misfit_event<T>(*ev.mis, *ev.mnodes, ev.rx, ev.ry, ev.rz, (int)ev.nrec);
     __threadfence();
     __syncthreads();
     __threadfence();

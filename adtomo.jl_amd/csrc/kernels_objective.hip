@@ -46,10 +46,11 @@ __global__ void __launch_bounds__(256) source_u0_kernel(const SourceDesc* __rest
 // One workgroup of 256 threads per solve (station).
 template <typename T>
 __global__ void __launch_bounds__(256) misfit_kernel(const MisfitDesc* __restrict__ d,
+                                                     const MisfitNodes* __restrict__ nodes,
                                                      const double* __restrict__ rx,
                                                      const double* __restrict__ ry,
                                                      const double* __restrict__ rz, int nrec) {
-  misfit_event<T>(d[blockIdx.x], rx, ry, rz, nrec);
+  misfit_event<T>(d[blockIdx.x], nodes[blockIdx.x], rx, ry, rz, nrec);
 }
 
 // ------------------------------------------------ model parametrisation ---
@@ -207,7 +208,7 @@ __global__ void sum_partials_kernel(const T* __restrict__ partial, int npart, T*
 
 #define ADT_OBJ_INSTANTIATE(T)                                                                    \
   template __global__ void source_u0_kernel<T>(const SourceDesc*, const T*, int, int, int, T);    \
-  template __global__ void misfit_kernel<T>(const MisfitDesc*, const double*, const double*,      \
+  template __global__ void misfit_kernel<T>(const MisfitDesc*, const MisfitNodes*, const double*, const double*,      \
                                             const double*, int);                                  \
   template __global__ void model_kernel<T>(const T*, const T*, int64_t, T*, T*);                 \
   template __global__ void sum_arrays_kernel<T>(const T* const*, int, int64_t, T*);              \

@@ -53,20 +53,16 @@ __device__ __forceinline__ T trilinear_eval(const Trilinear<T>& t, const T* __re
   return t.iz ? t.wz[0] * txy[0] + t.wz[1] * txy[1] : txy[0];
 }
 
+// d t / d u at corner (x side a, y side b, z side c) times the seed g, as
+// the chain rule of trilinear_eval gives it: the weights applied in reverse
+// order of the interpolation (z, y, x); a dimension that is not
+// interpolated has the single corner 0 and no weight.
 template <typename T>
-__device__ __forceinline__ void trilinear_adjoint(const Trilinear<T>& t, T g, T* grad_u) {
-  T gz[2] = {t.iz ? g * t.wz[0] : g, t.iz ? g * t.wz[1] : T(0)};
-  for (int c = 0; c < (t.iz ? 2 : 1); ++c) {
-    const T gy[2] = {t.iy ? gz[c] * t.wy[0] : gz[c], t.iy ? gz[c] * t.wy[1] : T(0)};
-    for (int b = 0; b < (t.iy ? 2 : 1); ++b) {
-      if (t.ix) {
-        atomicAdd(&grad_u[t.q[0][b][c]], gy[b] * t.wx[0]);
-        atomicAdd(&grad_u[t.q[1][b][c]], gy[b] * t.wx[1]);
-      } else {
-        atomicAdd(&grad_u[t.q[0][b][c]], gy[b]);
-      }
-    }
-  }
+__device__ __forceinline__ T trilinear_adjoint_at(const Trilinear<T>& t, T g, int a, int b,
+                                                  int c) {
+  const T gz = t.iz ? g * t.wz[c] : g;
+  const T gy = t.iy ? gz * t.wy[b] : gz;
+  return t.ix ? gy * t.wx[a] : gy;
 }
 
 // Misfit of one solve (station): loss = sum over receivers with tobs != -1
@@ -76,11 +72,20 @@ __device__ __forceinline__ void trilinear_adjoint(const Trilinear<T>& t, T g, T*
 // order and then over threads in a fixed tree order; grad_u = d loss / d u,
 // zero except at the interpolation nodes.
 //
+// grad_u is deterministic: each receiver's seed g_j = -w_j rho'(r_j) goes
+// to nodes.gj, and then one thread per interpolation node sums the node's
+// terms in increasing receiver order (nodes lists them so), starting from
+// +0, exactly as a sequential loop over the receivers adding into a zeroed
+// grad_u does.  A receiver without a pick has seed +0; its terms are +0
+// (the weights are >= 0) and leave the sum unchanged (a sum that starts at
+// +0 is never -0).
+//
 // Run by one workgroup of at least 256 threads: every thread zeroes grad_u,
 // the first 256 take the receivers (receiver j on thread j mod 256), so the
 // loss is summed in the same order whatever the workgroup size.
 template <typename T>
-__device__ __forceinline__ void misfit_event(const MisfitDesc& s, const double* __restrict__ rx,
+__device__ __forceinline__ void misfit_event(const MisfitDesc& s, const MisfitNodes& nodes,
+                                             const double* __restrict__ rx,
                                              const double* __restrict__ ry,
                                              const double* __restrict__ rz, int nrec) {
   constexpr int kRecThreads = 256;
@@ -90,6 +95,7 @@ __device__ __forceinline__ void misfit_event(const MisfitDesc& s, const double* 
   const T* __restrict__ w = (const T*)s.weight;
   T* grad_u = (T*)s.grad_u;
   T* tcal = (T*)s.tcal;
+  T* gj = (T*)nodes.gj;
   const T delta = (T)s.huber_delta;
   const int64_t N = (int64_t)s.m * s.n * s.l;
   if (grad_u)
@@ -102,6 +108,7 @@ __device__ __forceinline__ void misfit_event(const MisfitDesc& s, const double* 
     const T t = trilinear_eval(tr, u);
     if (tcal) tcal[j] = t;
     const T ob = tobs[j];
+    T seed = T(0);
     if (ob != T(-1)) {
       const T r = ob - t;
       T rho, drho;  // rho(r) and d rho / d r
@@ -114,11 +121,25 @@ __device__ __forceinline__ void misfit_event(const MisfitDesc& s, const double* 
         drho = T(2.0) * r;
       }
       acc = acc + w[j] * rho;
-      if (grad_u) trilinear_adjoint(tr, -(w[j] * drho), grad_u);
+      seed = -(w[j] * drho);
     }
+    if (grad_u) gj[j] = seed;
   }
   if (threadIdx.x < kRecThreads) part[threadIdx.x] = acc;
-  __syncthreads();
+  __syncthreads();  // seeds (and the zeroed grad_u) before the node sums
+  if (grad_u) {
+    for (int64_t q = threadIdx.x; q < nodes.nuniq; q += blockDim.x) {
+      T sum = T(0);
+      for (int64_t e = nodes.off[q]; e < nodes.off[q + 1]; ++e) {
+        const int64_t en = nodes.ent[e];
+        const int64_t j = en >> 3;
+        const Trilinear<T> tr = trilinear_setup<T>(rx[j], ry[j], rz[j], s.n, s.l);
+        sum = sum + trilinear_adjoint_at(tr, gj[j], (int)(en >> 2) & 1, (int)(en >> 1) & 1,
+                                         (int)en & 1);
+      }
+      grad_u[nodes.node[q]] = sum;
+    }
+  }
   for (int off = kRecThreads / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + off];
     __syncthreads();

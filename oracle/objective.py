@@ -91,6 +91,20 @@ def interpolation_weights(shape, jx, jy, jz):
     return out
 
 
+def interpolation_adjoint(g, jx, jy, jz):
+    """g * d t / d u of ``interpolate`` as {(i, j, k) 0-based: value}, in the
+    order the chain rule of the interpolation takes it (backpropagation of
+    scripts/inversion.jl:67-92): the z weight first, then y, then x,
+    ((g * wz) * wy) * wx; a dimension that is not interpolated contributes
+    the exact factor 1."""
+    (x1, x2), (y1, y2), (z1, z2) = _corners(jx, jy, jz)
+    wx = [(x1, 1.0)] if x1 == x2 else [(x1, x2 - jx), (x2, jx - x1)]
+    wy = [(y1, 1.0)] if y1 == y2 else [(y1, y2 - jy), (y2, jy - y1)]
+    wz = [(z1, 1.0)] if z1 == z2 else [(z1, z2 - jz), (z2, jz - z1)]
+    return {(i - 1, j - 1, k - 1): ((g * c) * b) * a
+            for k, c in wz for j, b in wy for i, a in wx}
+
+
 def huber(r, delta):
     """(rho, d rho / d r): r^2 inside [-delta, delta], 2 delta |r| - delta^2
     outside; delta <= 0 gives the reference's r^2."""
@@ -112,8 +126,9 @@ def misfit(u, rx, ry, rz, tobs, w, delta=0.0):
         rho, drho = huber(r, delta)
         loss += w[j] * rho
         g = -(w[j] * drho)
-        for key, wt in interpolation_weights(u.shape, rx[j], ry[j], rz[j]).items():
-            grad[key] += g * wt
+        # receivers in order, each node's terms added into the zeroed grad
+        for key, val in interpolation_adjoint(g, rx[j], ry[j], rz[j]).items():
+            grad[key] += val
     return tcal, loss, grad
 
 

@@ -1,10 +1,10 @@
 """GPU parity of the device objective (adtomo_hip.objective) and of one
 inversion step (adtomo_hip.inversion) against the CPU oracles.
 
-Bar: initial travel times and interpolated travel times bit-exact in FP64
-(same operation order, no contraction); the misfit loss within 1e-13 and
-dL/du within 1e-12 relative (the device adds the per-event contributions
-with atomics, so their order differs); model / smoothing / chain rule
+Bar: initial travel times, interpolated travel times and dL/du bit-exact
+in FP64 (same operation order, no contraction; each node's terms are summed
+in receiver order on both sides); the misfit loss within 1e-13 (the device
+sums it over threads in a tree); model / smoothing / chain rule
 within 1e-13 relative (exp and the box sums differ in the last bits);
 the whole objective gradient within 1e-9 relative (the Eikonal adjoint's
 own bar is 1e-10).
@@ -95,7 +95,7 @@ def test_misfit_parity(shape, ns, ne):
         tcal, L, g = OO.misfit(us[s], rx, ry, rz, tobs[s], w[s])
         np.testing.assert_array_equal(host(out["tcal"][s]), tcal)
         assert abs(loss[s] - L) <= 1e-13 * abs(L)
-        assert rel(host(out["grad_u"][s]), g) <= 1e-12
+        np.testing.assert_array_equal(host(out["grad_u"][s]), g)
 
 
 def test_misfit_f32():
@@ -190,7 +190,10 @@ def test_misfit_huber_parity(dtype, tol):
     for s in range(3):
         _, L, g = OO.misfit(us[s], rx, ry, rz, tobs[s], w[s], delta=0.1)
         assert abs(host(out["loss"])[s] - L) <= tol * abs(L)
-        assert rel(host(out["grad_u"][s]), g) <= tol
+        if dtype == np.float64:
+            np.testing.assert_array_equal(host(out["grad_u"][s]), g)
+        else:
+            assert rel(host(out["grad_u"][s]), g) <= tol
 
 
 def test_slowness_and_grad_parity():
@@ -359,9 +362,9 @@ def test_joint_inversion_objective_matches_oracle(chunk):
 def test_inversion_step_matches_separate_launches(fuse, dtype):
     # forward, misfit and adjoint of each station in one launch (graph and
     # single call; fused kernel or three kernels) against the forward graph,
-    # the misfit call and the adjoint graph on the same inputs: travel times
-    # and losses bit for bit, dL/du and dL/df to rounding (dL/du is summed
-    # with atomics); float32 runs FP32 storage with FP64 solves
+    # the misfit call and the adjoint graph on the same inputs: travel times,
+    # losses, dL/du and dL/df bit for bit, and the same again on a second
+    # launch of the step graph; float32 runs FP32 storage with FP64 solves
     import adtomo_hip as A
     from adtomo_hip import objective as O
     torch = _torch()
@@ -369,7 +372,6 @@ def test_inversion_step_matches_separate_launches(fuse, dtype):
     n = len(stations)
     npd = np.float64 if dtype == "float64" else np.float32
     mix = dtype == "float32"
-    tol_g = 1e-12 if dtype == "float64" else 1e-5
     base = A.get_tune(dtype)
     graphs = []
     try:
@@ -384,7 +386,12 @@ def test_inversion_step_matches_separate_launches(fuse, dtype):
         step = O.InversionStepGraph(*args, *ev, tb, wb, huber_delta=0.05, compute_f64=mix)
         graphs.append(step)
         step.launch()
+        torch.cuda.synchronize()
+        first = [host(x).copy() for x in step.grad_us + step.grad_fs]
         step.launch()
+        torch.cuda.synchronize()
+        for a, b in zip(first, [host(x) for x in step.grad_us + step.grad_fs]):
+            assert np.array_equal(a, b)
         one = O.inversion_step(*args, *ev, tb, wb, huber_delta=0.05, compute_f64=mix)
         fwd = A.EikonalGraph(*args, compute_f64=mix)
         graphs.append(fwd)
@@ -400,8 +407,8 @@ def test_inversion_step_matches_separate_launches(fuse, dtype):
                 (one["u"], one["grad_u"], one["grad_f"], one["loss"])):
             for e in range(n):
                 assert np.array_equal(host(res_u[e]), host(fwd.us[e])), e
-                assert rel(host(res_gu[e]), host(gus[e])) <= tol_g, e
-                assert rel(host(res_gf[e]), host(adj.grad_fs[e])) <= 1e3 * tol_g, e
+                assert np.array_equal(host(res_gu[e]), host(gus[e])), e
+                assert np.array_equal(host(res_gf[e]), host(adj.grad_fs[e])), e
             assert np.array_equal(host(res_loss), host(mis["loss"]))
         assert all(i["converged"] and i["adj_converged"] for i in step.infos())
         assert all(i["converged"] and i["adj_converged"] for i in one["info"])

@@ -95,6 +95,12 @@ def test_interpolate_node_values_and_weights():
         assert all(v >= 0 for v in w.values())
         lin = sum(wt * u[key] for key, wt in w.items())
         assert O.interpolate(u, x, y, z) == pytest.approx(lin, rel=1e-13, abs=1e-13)
+        # the chain-rule form carries the same weights, scaled by the seed
+        g = r.normal()
+        adj = O.interpolation_adjoint(g, x, y, z)
+        assert adj.keys() == w.keys()
+        for key, wt in w.items():
+            assert adj[key] == pytest.approx(g * wt, rel=1e-15, abs=1e-300)
 
 
 # ------------------------------------------------------------------ misfit

@@ -12,11 +12,13 @@ from . import objective
 from .ops import (EikonalGraph, autotune, eikonal, eikonal3d, eikonal3d_batched, eikonal3d_grad,
                   eikonal3d_grad_plain, eikonal3d_grad_strided, eikonal3d_plain,
                   eikonal3d_strided, eikonal_grad, eikonal_grad_strided, eikonal_strided,
-                  get_tune, last_info, selftest_arith, set_tune)
+                  clear_tune_classes, get_tune, last_info, load_tune, save_tune, selftest_arith,
+                  set_tune, set_tune_class, tune_classes)
 from ._lib import LIB_PATH, AdtError, lib
 
 __all__ = ["eikonal", "eikonal_grad", "eikonal3d", "eikonal3d_grad", "eikonal3d_batched",
            "EikonalGraph", "eikonal3d_plain", "eikonal3d_grad_plain", "selftest_arith",
            "last_info", "lib", "LIB_PATH", "AdtError", "objective", "autotune", "get_tune",
-           "set_tune", "eikonal_strided", "eikonal_grad_strided", "eikonal3d_strided",
+           "set_tune", "set_tune_class", "tune_classes", "clear_tune_classes", "save_tune",
+           "load_tune", "eikonal_strided", "eikonal_grad_strided", "eikonal3d_strided",
            "eikonal3d_grad_strided"]

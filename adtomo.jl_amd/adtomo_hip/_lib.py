@@ -78,10 +78,20 @@ class AdtMisfit3d(ctypes.Structure):
 
 class AdtTune(ctypes.Structure):
     _fields_ = [("diag_max_l", _i32), ("adj_wg", _i32), ("order", _i32), ("fuse_max_n", _i32),
-                ("fwd_ms", ctypes.c_float), ("adj_ms", ctypes.c_float), ("adj_algo", _i32)]
+                ("fwd_ms", ctypes.c_float), ("adj_ms", ctypes.c_float), ("adj_algo", _i32),
+                ("fwd_segs", _i32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AdtTuneClass(ctypes.Structure):
+    _fields_ = [("l_blocks", _i32), ("bands", _i32), ("log2n", _i32), ("tune", AdtTune)]
+
+    def as_dict(self):
+        d = {"l_blocks": self.l_blocks, "bands": self.bands, "log2n": self.log2n}
+        d.update(self.tune.as_dict())
+        return d
 
 
 class AdtError(RuntimeError):
@@ -153,6 +163,18 @@ def lib():
     L.adt_set_tune.restype = _i32
     L.adt_autotune.argtypes = [_p, _i32, _i32, _i32, _i32, _p, _p]
     L.adt_autotune.restype = _i32
+    L.adt_tune_class_set.argtypes = [_i32, _i64, _i64, _i64, _p]
+    L.adt_tune_class_set.restype = _i32
+    L.adt_tune_class_count.argtypes = [_i32]
+    L.adt_tune_class_count.restype = _i32
+    L.adt_tune_class_get.argtypes = [_i32, _i32, _p]
+    L.adt_tune_class_get.restype = _i32
+    L.adt_tune_class_clear.argtypes = [_i32]
+    L.adt_tune_class_clear.restype = _i32
+    L.adt_tune_save.argtypes = [ctypes.c_char_p]
+    L.adt_tune_save.restype = _i32
+    L.adt_tune_load.argtypes = [ctypes.c_char_p]
+    L.adt_tune_load.restype = _i32
     L.adt_version.restype = ctypes.c_char_p
     L.adt_last_error.restype = ctypes.c_char_p
     _lib = L

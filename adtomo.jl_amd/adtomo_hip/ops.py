@@ -17,6 +17,7 @@ There is no CPU fallback: every call runs the HIP kernels.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -416,31 +417,90 @@ def get_tune(dtype="float64"):
     return t.as_dict()
 
 
-def set_tune(dtype="float64", diag_max_l=None, adj_wg=None, order=None, fuse_max_n=None,
-             adj_algo=None):
-    """Change the launch setting (unspecified fields keep their value).
-    ``adj_algo``: 0 = adjoint over 8x8x8 tiles (default), 1 = node by node."""
+_TUNE_FIELDS = ("diag_max_l", "adj_wg", "order", "fuse_max_n", "adj_algo", "fwd_segs")
+
+
+def _apply(t, fields):
+    for k, v in fields.items():
+        if k not in _TUNE_FIELDS:
+            raise TypeError(f"unknown launch setting field {k!r}")
+        if v is not None:
+            setattr(t, k, int(v))
+
+
+def set_tune(dtype="float64", **fields):
+    """Change the global launch setting (unspecified fields keep their
+    value): ``diag_max_l``, ``adj_wg``, ``order``, ``fuse_max_n``,
+    ``adj_algo`` (0 = adjoint over 8x8x8 tiles, the default; 1 = node by
+    node) and ``fwd_segs`` (c segments of the diagonal forward, 1..8; 0 =
+    8)."""
     t = L.AdtTune()
     L.check(L.lib().adt_get_tune(_dtype_code(dtype), ctypes.byref(t)), "get_tune")
-    if diag_max_l is not None:
-        t.diag_max_l = int(diag_max_l)
-    if adj_wg is not None:
-        t.adj_wg = int(adj_wg)
-    if order is not None:
-        t.order = int(order)
-    if fuse_max_n is not None:
-        t.fuse_max_n = int(fuse_max_n)
-    if adj_algo is not None:
-        t.adj_algo = int(adj_algo)
+    _apply(t, fields)
     L.check(L.lib().adt_set_tune(_dtype_code(dtype), ctypes.byref(t)), "set_tune")
     return t.as_dict()
 
 
-def autotune(u0s, fs, hs, tols, grad_us=None, reps=3, compute_f64=False):
+def set_tune_class(shape, dtype="float64", clear=False, **fields):
+    """Give the box-shape class of ``shape`` = (m, n, l) its own per-event
+    fields (``diag_max_l``, ``fwd_segs``, ``adj_algo``; the others stay
+    global), starting from the global setting; ``clear=True`` removes the
+    class's setting."""
+    m, n, l = (int(x) for x in shape)
+    if clear:
+        L.check(L.lib().adt_tune_class_set(_dtype_code(dtype), m, n, l, None), "tune_class_set")
+        return None
+    t = L.AdtTune()
+    L.check(L.lib().adt_get_tune(_dtype_code(dtype), ctypes.byref(t)), "get_tune")
+    _apply(t, fields)
+    L.check(L.lib().adt_tune_class_set(_dtype_code(dtype), m, n, l, ctypes.byref(t)),
+            "tune_class_set")
+    return t.as_dict()
+
+
+def tune_classes(dtype="float64"):
+    """The class settings of one precision, each a dict with the class key
+    (``l_blocks``, ``bands``, ``log2n``) and the setting's fields."""
+    lib = L.lib()
+    code = _dtype_code(dtype)
+    n = lib.adt_tune_class_count(code)
+    L.check(min(n, 0), "tune_class_count")
+    out = []
+    for i in range(n):
+        c = L.AdtTuneClass()
+        L.check(lib.adt_tune_class_get(code, i, ctypes.byref(c)), "tune_class_get")
+        out.append(c.as_dict())
+    return out
+
+
+def clear_tune_classes(dtype="float64"):
+    L.check(L.lib().adt_tune_class_clear(_dtype_code(dtype)), "tune_class_clear")
+
+
+def save_tune(path):
+    """Write every launch setting (global and per class, both precisions)
+    to a JSON file."""
+    L.check(L.lib().adt_tune_save(os.fsencode(path)), "tune_save")
+
+
+def load_tune(path):
+    """Read settings written by save_tune (replacing the global settings it
+    holds and every class setting)."""
+    L.check(L.lib().adt_tune_load(os.fsencode(path)), "tune_load")
+
+
+def autotune(u0s, fs, hs, tols, grad_us=None, reps=3, compute_f64=False, cache=None):
     """Time the launch settings on these events and keep the fastest for
-    their precision (forward kernel split, adjoint workgroup size, event
-    order, largest event whose forward and adjoint are fused in one kernel).
-    Returns the chosen setting with its measured times."""
+    their precision: per box-shape class among them, the forward kernel and
+    its segment count and the adjoint traversal; over all of them, the event
+    order, the node-level adjoint's workgroup size and the fusion threshold.
+    ``cache``: a JSON file of settings; read instead of timing when it
+    exists, written after timing when it does not.  Returns the global
+    setting with its measured times."""
+    dtype = "float32" if fs[0].dtype == _torch().float32 else "float64"
+    if cache is not None and os.path.exists(cache):
+        load_tune(cache)
+        return get_tune(dtype)
     torch = _torch()
     us = [torch.empty_like(f) for f in fs]
     gu0s = [torch.empty_like(f) for f in fs] if grad_us is not None else None
@@ -450,6 +510,8 @@ def autotune(u0s, fs, hs, tols, grad_us=None, reps=3, compute_f64=False):
     best = L.AdtTune()
     L.check(L.lib().adt_autotune(ctypes.byref(tab), len(fs), flags, _dtype_code(dtype), int(reps),
                                  ctypes.byref(best), _stream()), "autotune")
+    if cache is not None:
+        save_tune(cache)
     return best.as_dict()
 
 

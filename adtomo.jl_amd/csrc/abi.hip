@@ -9,6 +9,7 @@
 #include <cstring>
 #include <algorithm>
 #include <list>
+#include <map>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -70,12 +71,66 @@ struct DevBuf {
 // adt_set_tune / adt_autotune under a mutex.
 std::mutex g_tune_mu;
 constexpr int32_t kFuseMaxDefault = 1 << 21;
-adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f, 0},
-                      {adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f, 0}};
+constexpr int kSegsMax = 8;  // c segments of the diagonal forward (its waves)
+adt_tune g_tune[2] = {{adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f, 0, 0},
+                      {adt::kDiagAll, 1024, 1, kFuseMaxDefault, 0.f, 0.f, 0, 0}};
+
+// Box-shape classes: events whose k extent needs the same number of
+// 8-node segments, whose shorter horizontal side needs the same number of
+// 64-row bands, and whose node count has the same binary order share a
+// class.  A class may carry its own setting of the per-event fields
+// (diag_max_l, fwd_segs, adj_algo); the launch-wide fields (adj_wg, order,
+// fuse_max_n) always come from the global setting.
+struct ClassKey {
+  int l_blocks, bands, log2n;
+  bool operator<(const ClassKey& o) const {
+    return l_blocks != o.l_blocks ? l_blocks < o.l_blocks
+                                  : (bands != o.bands ? bands < o.bands : log2n < o.log2n);
+  }
+  bool operator==(const ClassKey& o) const {
+    return l_blocks == o.l_blocks && bands == o.bands && log2n == o.log2n;
+  }
+};
+ClassKey class_of(int64_t m, int64_t n, int64_t l) {
+  ClassKey k;
+  k.l_blocks = (int)((l + 7) / 8);
+  k.bands = (int)((std::min(m, n) + 63) / 64);
+  int64_t N = m * n * l;
+  int lg = 0;
+  while (N > 1) {
+    N >>= 1;
+    ++lg;
+  }
+  k.log2n = lg;
+  return k;
+}
+std::map<ClassKey, adt_tune> g_classes[2];
 
 template <typename T>
 constexpr int tune_index() {
   return std::is_same<T, float>::value ? 1 : 0;
+}
+// The settings a launch uses, taken together under the lock.
+struct TuneSnap {
+  adt_tune global;
+  std::map<ClassKey, adt_tune> classes;
+  // the setting of an event of box (m, n, l): its class's per-event fields
+  // over the global setting
+  adt_tune for_box(int64_t m, int64_t n, int64_t l) const {
+    adt_tune t = global;
+    auto it = classes.find(class_of(m, n, l));
+    if (it != classes.end()) {
+      t.diag_max_l = it->second.diag_max_l;
+      t.fwd_segs = it->second.fwd_segs;
+      t.adj_algo = it->second.adj_algo;
+    }
+    return t;
+  }
+};
+template <typename T>
+TuneSnap current_snap() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return TuneSnap{g_tune[tune_index<T>()], g_classes[tune_index<T>()]};
 }
 template <typename T>
 adt_tune current_tune() {
@@ -85,7 +140,91 @@ adt_tune current_tune() {
 bool tune_valid(const adt_tune& t) {
   return t.diag_max_l >= 0 && t.diag_max_l <= adt::kDiagAll &&
          (t.adj_wg == 256 || t.adj_wg == 512 || t.adj_wg == 1024) && (t.order == 0 || t.order == 1) &&
-         t.fuse_max_n >= 0 && (t.adj_algo == 0 || t.adj_algo == 1);
+         t.fuse_max_n >= 0 && (t.adj_algo == 0 || t.adj_algo == 1) && t.fwd_segs >= 0 &&
+         t.fwd_segs <= kSegsMax;
+}
+
+// ---- the settings as JSON text (adt_tune_save / adt_tune_load): one
+// object per setting, {"dtype": d, <field>: value, ...}, global ones in
+// "global" and class ones (with "l_blocks", "bands", "log2n") in "classes".
+const char* const kTuneFields[] = {"diag_max_l", "adj_wg", "order", "fuse_max_n",
+                                   "fwd_ms", "adj_ms", "adj_algo", "fwd_segs"};
+
+std::string tune_json(int dtype, const adt_tune& t, const ClassKey* k) {
+  char buf[512];
+  int o = snprintf(buf, sizeof buf, "{\"dtype\": %d, ", dtype);
+  if (k)
+    o += snprintf(buf + o, sizeof buf - o, "\"l_blocks\": %d, \"bands\": %d, \"log2n\": %d, ",
+                  k->l_blocks, k->bands, k->log2n);
+  snprintf(buf + o, sizeof buf - o,
+           "\"diag_max_l\": %d, \"adj_wg\": %d, \"order\": %d, \"fuse_max_n\": %d, "
+           "\"fwd_ms\": %.9g, \"adj_ms\": %.9g, \"adj_algo\": %d, \"fwd_segs\": %d}",
+           t.diag_max_l, t.adj_wg, t.order, t.fuse_max_n, (double)t.fwd_ms, (double)t.adj_ms,
+           t.adj_algo, t.fwd_segs);
+  return buf;
+}
+
+// Reads the objects of the text written by tune_json (keys in any order,
+// numbers only); every field must be present.
+int parse_tune_json(const std::string& txt, adt_tune glob[2], bool have_glob[2],
+                    std::map<ClassKey, adt_tune> cls[2]) {
+  size_t i = 0;
+  std::string section;
+  while (i < txt.size()) {
+    const char ch = txt[i];
+    if (ch == '"') {  // a key outside the setting objects names the section
+      const size_t e = txt.find('"', i + 1);
+      if (e == std::string::npos) return fail(ADT_EINVAL, "tune file: unterminated string");
+      section = txt.substr(i + 1, e - i - 1);
+      i = e + 1;
+      continue;
+    }
+    if (ch != '{' || (section != "global" && section != "classes")) {
+      ++i;
+      continue;
+    }
+    const size_t e = txt.find('}', i);
+    if (e == std::string::npos) return fail(ADT_EINVAL, "tune file: unterminated object");
+    const std::string obj = txt.substr(i + 1, e - i - 1);
+    i = e + 1;
+    std::map<std::string, double> kv;
+    size_t j = 0;
+    while ((j = obj.find('"', j)) != std::string::npos) {
+      const size_t k = obj.find('"', j + 1);
+      const size_t c = k == std::string::npos ? k : obj.find(':', k);
+      if (c == std::string::npos) return fail(ADT_EINVAL, "tune file: malformed object");
+      char* end = nullptr;
+      const double v = strtod(obj.c_str() + c + 1, &end);
+      if (end == obj.c_str() + c + 1) return fail(ADT_EINVAL, "tune file: value is not a number");
+      kv[obj.substr(j + 1, k - j - 1)] = v;
+      j = (size_t)(end - obj.c_str());
+    }
+    auto get = [&](const char* key, double* v) {
+      auto it = kv.find(key);
+      if (it == kv.end()) return false;
+      *v = it->second;
+      return true;
+    };
+    double d = 0, f[8];
+    if (!get("dtype", &d) || (d != 0 && d != 1)) return fail(ADT_EINVAL, "tune file: bad dtype");
+    for (int q = 0; q < 8; ++q)
+      if (!get(kTuneFields[q], &f[q]))
+        return fail(ADT_EINVAL, std::string("tune file: missing ") + kTuneFields[q]);
+    adt_tune t{(int32_t)f[0], (int32_t)f[1], (int32_t)f[2], (int32_t)f[3],
+               (float)f[4], (float)f[5], (int32_t)f[6], (int32_t)f[7]};
+    if (!tune_valid(t)) return fail(ADT_EINVAL, "tune file: setting out of range");
+    const int di = (int)d;
+    if (section == "global") {
+      glob[di] = t;
+      have_glob[di] = true;
+    } else {
+      double lb, bd, lg;
+      if (!get("l_blocks", &lb) || !get("bands", &bd) || !get("log2n", &lg))
+        return fail(ADT_EINVAL, "tune file: class without l_blocks / bands / log2n");
+      cls[di][ClassKey{(int)lb, (int)bd, (int)lg}] = t;
+    }
+  }
+  return ADT_OK;
 }
 
 // ------------------------------------------------------------- adjoint ---
@@ -193,24 +332,31 @@ struct Staged {
 
 // Forward launches; mix (FP32 only): FP64 local solves on FP32 storage.
 template <typename T>
-void launch_fwd_diag(bool mix, int grid, const adt::EventDesc* d, int max_l, hipStream_t s) {
+void launch_fwd_diag(bool mix, int grid, const adt::EventDesc* d, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (mix) {
-      adt::fwd3_diag_kernel<float, true><<<grid, kDiagWG, 0, s>>>(d, kMaxIter3, max_l);
+      adt::fwd3_diag_kernel<float, true><<<grid, kDiagWG, 0, s>>>(d, kMaxIter3);
       return;
     }
   }
-  adt::fwd3_diag_kernel<T, false><<<grid, kDiagWG, 0, s>>>(d, kMaxIter3, max_l);
+  adt::fwd3_diag_kernel<T, false><<<grid, kDiagWG, 0, s>>>(d, kMaxIter3);
 }
 template <typename T>
-void launch_fwd_stream(bool mix, int grid, const adt::EventDesc* d, int min_l, hipStream_t s) {
+void launch_fwd_stream(bool mix, int grid, const adt::EventDesc* d, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (mix) {
-      adt::fwd3_stream_kernel<float, true><<<grid, kStreamWG, 0, s>>>(d, kMaxIter3, min_l);
+      adt::fwd3_stream_kernel<float, true><<<grid, kStreamWG, 0, s>>>(d, kMaxIter3);
       return;
     }
   }
-  adt::fwd3_stream_kernel<T, false><<<grid, kStreamWG, 0, s>>>(d, kMaxIter3, min_l);
+  adt::fwd3_stream_kernel<T, false><<<grid, kStreamWG, 0, s>>>(d, kMaxIter3);
+}
+
+// A single event's descriptor takes its box class's setting.
+void set_event_setting(adt::EventDesc& d, const adt_tune& t) {
+  d.fwd_diag = d.l <= t.diag_max_l ? 1 : 0;
+  d.fwd_segs = (int8_t)(t.fwd_segs > 0 ? t.fwd_segs : kSegsMax);
+  d.adj_algo = (int8_t)adj_algo_for(t.adj_algo, tiles_of(d.m, d.n, d.l));
 }
 
 template <typename T>
@@ -246,14 +392,14 @@ int fwd3_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t l, T t
   d.u = du;
   d.uold = base;
   d.info = (adt_info*)(base + off_info);
+  set_event_setting(d, current_snap<T>().for_box(m, n, l));
   ADT_HIP(hipMemcpyAsync(base + off_desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
   adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
-  const adt_tune tn = current_tune<T>();
   const bool mix = flags & ADT_F32_COMPUTE_F64;
-  if (l <= tn.diag_max_l)
-    launch_fwd_diag<T>(mix, 1, (const adt::EventDesc*)(base + off_desc), tn.diag_max_l, s);
+  if (d.fwd_diag)
+    launch_fwd_diag<T>(mix, 1, (const adt::EventDesc*)(base + off_desc), s);
   else
-    launch_fwd_stream<T>(mix, 1, (const adt::EventDesc*)(base + off_desc), tn.diag_max_l, s);
+    launch_fwd_stream<T>(mix, 1, (const adt::EventDesc*)(base + off_desc), s);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -329,13 +475,13 @@ int fwd3_noalloc_impl(const double* u0, const double* f, double h, int64_t m, in
   d.u = u;
   d.uold = b;  // written before it is read (u0 is the first snapshot)
   d.info = (adt_info*)(b + kNoAllocInfoOff);
+  set_event_setting(d, current_snap<double>().for_box(m, n, l));
   ADT_HIP(hipMemcpyAsync(b + kNoAllocDescOff, &d, sizeof(d), hipMemcpyHostToDevice, s));
   adt::info_clear_kernel<<<1, 64, 0, s>>>(d.info, 1);
-  const adt_tune tn = current_tune<double>();
-  if (l <= tn.diag_max_l)
-    launch_fwd_diag<double>(false, 1, (const adt::EventDesc*)(b + kNoAllocDescOff), tn.diag_max_l, s);
+  if (d.fwd_diag)
+    launch_fwd_diag<double>(false, 1, (const adt::EventDesc*)(b + kNoAllocDescOff), s);
   else
-    launch_fwd_stream<double>(false, 1, (const adt::EventDesc*)(b + kNoAllocDescOff), tn.diag_max_l, s);
+    launch_fwd_stream<double>(false, 1, (const adt::EventDesc*)(b + kNoAllocDescOff), s);
   ADT_HIP(hipGetLastError());
   adt_info hinf{};
   ADT_HIP(hipMemcpyAsync(&hinf, d.info, sizeof(adt_info), hipMemcpyDeviceToHost, s));
@@ -367,7 +513,8 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
     return rc;
   DevBuf ws;
   const int64_t tiles = tiles_of(m, n, l);
-  const adt_tune tn = current_tune<T>();
+  const TuneSnap snap = current_snap<T>();
+  const adt_tune tn = snap.for_box(m, n, l);
   const int algo = adj_algo_for(tn.adj_algo, tiles);
   size_t a, c, q;
   adj_ws_bytes<T>(algo, N, tiles, &a, &c, &q);
@@ -391,6 +538,7 @@ int bwd3_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_t m, i
   d.queue = (int32_t*)(base + a + c);
   d.grad_u0 = dgu0;
   d.grad_f = dgf;
+  set_event_setting(d, tn);
   const size_t od = a + c + q;
   d.info = (adt_info*)(base + od + 256);
   ADT_HIP(hipMemcpyAsync(base + od, &d, sizeof(d), hipMemcpyHostToDevice, s));
@@ -846,15 +994,20 @@ int check_misfit_item(const adt_misfit3d& it, int i, const double* rx, const dou
   return ADT_OK;
 }
 
+// Which kernels a set of events needs (from each event's setting).
+struct KernelMix {
+  bool diag = false, stream = false;  // forward kernels
+  bool tile = false, node = false;    // adjoint traversals
+  int64_t tile_max = 0;  // largest tile count among the tile-traversal events
+  int64_t max_n = 0;     // largest node count
+};
+
 template <typename T>
 struct BatchPlan {
   std::vector<adt::EventDesc> descs;
   size_t ws_bytes = 0;
   size_t off_desc = 0, off_info = 0;
-  int max_l = 0;      // largest k extent of the batch
-  int64_t max_n = 0;  // largest node count of the batch
-  int64_t max_tiles = 0;  // largest 8x8x8 tile count of the batch
-  int adj_algo = 0;       // adjoint traversal the workspace is laid out for
+  KernelMix mix;
   // attached receiver misfit (attach_misfit): table and coordinates in the
   // workspace
   const MisfitArgs* mis = nullptr;
@@ -924,14 +1077,10 @@ struct BatchPlan {
                      dc + mis->nrec, dc + 2 * mis->nrec, (int)mis->nrec};
   }
 
-  int build(const adt_event3d* ev, int nev, int flags, int adj_algo_req) {
+  // Each event takes the setting of its box class (or the global one).
+  int build(const adt_event3d* ev, int nev, int flags, const TuneSnap& snap) {
     if (nev < 0) return fail(ADT_EINVAL, "batched: nev < 0");
     if (nev > 0 && !ev) return fail(ADT_EINVAL, "batched: NULL events");
-    for (int e = 0; e < nev; ++e) {
-      if (check_dims3(ev[e].m, ev[e].n, ev[e].l)) continue;  // reported below
-      max_tiles = std::max(max_tiles, tiles_of(ev[e].m, ev[e].n, ev[e].l));
-    }
-    adj_algo = adj_algo_for(adj_algo_req, max_tiles);
     descs.resize(nev);
     size_t off = 0;
     auto take = [&](size_t b) {
@@ -951,11 +1100,25 @@ struct BatchPlan {
       const size_t N = size_t(x.m * x.n * x.l);
       if (N * sizeof(T) >= kMaxArrayBytes)
         return fail(ADT_EINVAL, "event " + std::to_string(e) + ": arrays of 4 GiB or more are not supported");
-      if (x.l > max_l) max_l = (int)x.l;
-      if ((int64_t)N > max_n) max_n = (int64_t)N;
+      const adt_tune t = snap.for_box(x.m, x.n, x.l);
+      const int64_t tiles = tiles_of(x.m, x.n, x.l);
+      descs[e] = adt::EventDesc{};
+      descs[e].fwd_diag = x.l <= t.diag_max_l ? 1 : 0;
+      descs[e].fwd_segs = (int8_t)(t.fwd_segs > 0 ? t.fwd_segs : kSegsMax);
+      descs[e].adj_algo = (int8_t)adj_algo_for(t.adj_algo, tiles);
+      if (flags & ADT_FORWARD) (descs[e].fwd_diag ? mix.diag : mix.stream) = true;
+      if (flags & ADT_ADJOINT) {
+        if (descs[e].adj_algo == 0) {
+          mix.tile = true;
+          mix.tile_max = std::max(mix.tile_max, tiles);
+        } else {
+          mix.node = true;
+        }
+      }
+      if ((int64_t)N > mix.max_n) mix.max_n = (int64_t)N;
       o_uold[e] = (flags & ADT_FORWARD) ? take(N * sizeof(T)) : 0;
       size_t wl, wc, wq;
-      adj_ws_bytes<T>(adj_algo, (int64_t)N, tiles_of(x.m, x.n, x.l), &wl, &wc, &wq);
+      adj_ws_bytes<T>(descs[e].adj_algo, (int64_t)N, tiles, &wl, &wc, &wq);
       o_lam[e] = (flags & ADT_ADJOINT) ? take(wl) : 0;
       o_code[e] = (flags & ADT_ADJOINT) ? take(wc) : 0;
       o_indeg[e] = 0;
@@ -967,7 +1130,6 @@ struct BatchPlan {
     for (int e = 0; e < nev; ++e) {
       const adt_event3d& x = ev[e];
       adt::EventDesc& d = descs[e];
-      d = adt::EventDesc{};
       d.m = (int)x.m;
       d.n = (int)x.n;
       d.l = (int)x.l;
@@ -1017,37 +1179,39 @@ struct BatchPlan {
   }
 };
 
+// The kernels of a batch: each event's forward on the kernel its setting
+// names (each kernel skips the other's events), the misfit, then each
+// event's adjoint traversal likewise.  tn: the launch-wide fields.
 template <typename T>
-int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, int64_t max_n,
-                 int adj_algo, int64_t max_tiles, const adt_tune& tn, adt_info* infos, hipStream_t s,
+int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, const KernelMix& km,
+                 const adt_tune& tn, adt_info* infos, hipStream_t s,
                  const MisfitRun& mr = MisfitRun{}) {
   if (nev == 0) return ADT_OK;
   adt::info_clear_kernel<<<1, 256, 0, s>>>(infos, nev);
   ADT_HIP(hipGetLastError());
-  // forward and adjoint of each event in one kernel (every event diagonal)
-  if (adj_algo == 1 && max_n <= tn.fuse_max_n && (flags & ADT_FORWARD) && (flags & ADT_ADJOINT) &&
-      max_l <= tn.diag_max_l) {
-    const bool mix = flags & ADT_F32_COMPUTE_F64;
+  const bool mix = flags & ADT_F32_COMPUTE_F64;
+  // forward and adjoint of each event in one kernel (every event on the
+  // diagonal forward and the node-level adjoint)
+  if ((flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && km.diag && !km.stream && km.node &&
+      !km.tile && km.max_n <= tn.fuse_max_n) {
     if constexpr (std::is_same<T, float>::value) {
       if (mix) {
-        adt::fwd_adj3_diag_kernel<float, true><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
+        adt::fwd_adj3_diag_kernel<float, true><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3);
         ADT_HIP(hipGetLastError());
         return ADT_OK;
       }
     }
-    adt::fwd_adj3_diag_kernel<T, false><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3, tn.diag_max_l);
+    adt::fwd_adj3_diag_kernel<T, false><<<nev, kDiagWG, 0, s>>>(ddesc, kMaxIter3);
     ADT_HIP(hipGetLastError());
     return ADT_OK;
   }
   if (flags & ADT_FORWARD) {
-    // events with l <= diag_max_l: diagonal streaming; longer ones: k-streaming
-    const bool mix = flags & ADT_F32_COMPUTE_F64;
-    if (tn.diag_max_l > 0) {
-      launch_fwd_diag<T>(mix, nev, ddesc, tn.diag_max_l, s);
+    if (km.diag) {
+      launch_fwd_diag<T>(mix, nev, ddesc, s);
       ADT_HIP(hipGetLastError());
     }
-    if (max_l > tn.diag_max_l) {
-      launch_fwd_stream<T>(mix, nev, ddesc, tn.diag_max_l, s);
+    if (km.stream) {
+      launch_fwd_stream<T>(mix, nev, ddesc, s);
       ADT_HIP(hipGetLastError());
     }
   }
@@ -1056,8 +1220,14 @@ int launch_batch(const adt::EventDesc* ddesc, int nev, int flags, int max_l, int
     ADT_HIP(hipGetLastError());
   }
   if (flags & ADT_ADJOINT) {
-    launch_adj<T>(adj_algo, max_tiles, nev, ddesc, tn.adj_wg, s);
-    ADT_HIP(hipGetLastError());
+    if (km.tile) {
+      launch_adj<T>(0, km.tile_max, nev, ddesc, tn.adj_wg, s);
+      ADT_HIP(hipGetLastError());
+    }
+    if (km.node) {
+      launch_adj<T>(1, 0, nev, ddesc, tn.adj_wg, s);
+      ADT_HIP(hipGetLastError());
+    }
   }
   return ADT_OK;
 }
@@ -1070,9 +1240,10 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   if (mis && !((flags & ADT_FORWARD) && (flags & ADT_ADJOINT)))
     return fail(ADT_EINVAL, "misfit batched: needs ADT_FORWARD and ADT_ADJOINT");
   hipStream_t s = (hipStream_t)stream;
-  const adt_tune tn = current_tune<T>();
+  const TuneSnap snap = current_snap<T>();
+  const adt_tune& tn = snap.global;
   BatchPlan<T> plan;
-  int rc = plan.build(ev, nev, flags, tn.adj_algo);
+  int rc = plan.build(ev, nev, flags, snap);
   if (rc) return rc;
   if (mis && (rc = plan.attach_misfit(ev, nev, mis))) return rc;
   if (nev == 0) return ADT_OK;
@@ -1084,9 +1255,8 @@ int batched_impl(const adt_event3d* ev, int nev, int flags, adt_info* infos, voi
   ADT_HIP(hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
                          hipMemcpyHostToDevice, s));
   if ((rc = plan.upload_misfit(base, nev, s))) return rc;
-  if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags,
-                            plan.max_l, plan.max_n, plan.adj_algo, plan.max_tiles, tn, (adt_info*)(base + plan.off_info), s,
-                            plan.misfit_run(base))))
+  if ((rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.mix,
+                            tn, (adt_info*)(base + plan.off_info), s, plan.misfit_run(base))))
     return rc;
   // the misfit table and coordinates were copied from the caller's host
   // memory: wait before returning
@@ -1222,12 +1392,18 @@ int bwd3_plain_impl(const T* g, const T* u, const T* u0, const T* f, T h, int64_
 
 // -------------------------------------------------------------- autotune ---
 
-// Times the launch settings on the caller's events and keeps the fastest:
-// the forward kernel split (diagonal streaming up to diag_max_l, or
-// k-streaming for every event), the adjoint workgroup size, and the event
-// order.  Every setting computes the same results (the forward bit for bit,
-// the adjoint to rounding), so the caller's outputs hold a valid solve
-// afterwards.
+// Times launch settings on the caller's events and keeps the fastest.
+// First, for each box-shape class among the events and on that class's
+// events alone: the forward kernel (k-streaming, or diagonal streaming with
+// 8, 4 or ceil(l / 8) c segments) and then the adjoint traversal (tiles or
+// node by node), stored as the class's setting.  Then, on all events with
+// those class settings: the event order, the node-level adjoint's
+// workgroup size, and whether the fused forward + adjoint kernel beats the
+// separate ones (which moves fuse_max_n; a loss lowers it only when the
+// batch fills the device, one event per compute unit, since a small batch
+// says little about a full one), stored as the global setting.  Every
+// setting computes the same results (the forward bit for bit, the adjoint
+// to rounding), so the caller's outputs hold a valid solve afterwards.
 template <typename T>
 int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune* best,
                   hipStream_t s) {
@@ -1235,17 +1411,16 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     return fail(ADT_EINVAL, "autotune: no ADT_FORWARD/ADT_ADJOINT flag");
   if (flags & ADT_HOST_POINTERS) return fail(ADT_EINVAL, "autotune: device pointers only");
   if (nev < 1) return fail(ADT_EINVAL, "autotune: no events");
+  if (!ev) return fail(ADT_EINVAL, "autotune: NULL events");
   reps = reps < 1 ? 1 : (reps > 50 ? 50 : reps);
-  const adt_tune cur = current_tune<T>();
-  int min_l = 1 << 30;
-  for (int e = 0; e < nev; ++e) min_l = std::min<int>(min_l, (int)ev[e].l);
-  int max_l_ev = 0;
-  for (int e = 0; e < nev; ++e) max_l_ev = std::max<int>(max_l_ev, (int)ev[e].l);
-  // diagonal kernel for every event, k-streaming for every event, and
-  // (mixed batches) diagonal up to one k chunk of 64, k-streaming above
-  std::vector<int> dmls = {adt::kDiagAll, 0};
-  if (min_l <= 64 && max_l_ev > 64) dmls.push_back(64);
-  const int wgs[3] = {256, 512, 1024};
+  const TuneSnap cur = current_snap<T>();
+  const int fmix = flags & ADT_F32_COMPUTE_F64;
+  std::map<ClassKey, std::vector<adt_event3d>> groups;
+  for (int e = 0; e < nev; ++e) {
+    int rc = check_dims3(ev[e].m, ev[e].n, ev[e].l);
+    if (rc) return fail(rc, "autotune: event " + std::to_string(e) + ": " + g_err);
+    groups[class_of(ev[e].m, ev[e].n, ev[e].l)].push_back(ev[e]);
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ADT_HIP(hipEventCreate(&e0));
   hipError_t he = hipEventCreate(&e1);
@@ -1253,98 +1428,150 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
     (void)hipEventDestroy(e0);
     return fail(ADT_EHIP, std::string("autotune: hipEventCreate: ") + hipGetErrorString(he));
   }
-  int rc = ADT_OK;
-  adt_tune win = cur;
-  float win_total = 3.4e38f;
-  for (int order = 0; order < 2 && rc == ADT_OK; ++order) {
+  // mean time of `reps` launches of `evs` under snapshot `sn` with flags
+  // `fl`, after one untimed launch
+  auto time_set = [&](const std::vector<adt_event3d>& evs, const TuneSnap& sn, int fl,
+                      float* ms) -> int {
+    const int n = (int)evs.size();
     BatchPlan<T> plan;
-    if ((rc = plan.build(ev, nev, flags, cur.adj_algo))) break;
-    if (order) plan.order_largest_first();
+    int r = plan.build(evs.data(), n, fl, sn);
+    if (r) return r;
+    if (sn.global.order) plan.order_largest_first();
     DevBuf ws;
-    if ((rc = ws.alloc(plan.ws_bytes, s))) break;
+    if ((r = ws.alloc(plan.ws_bytes, s))) return r;
     plan.bind((char*)ws.p);
     char* base = (char*)ws.p;
     const adt::EventDesc* dd = (const adt::EventDesc*)(base + plan.off_desc);
-    he = hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * nev,
-                        hipMemcpyHostToDevice, s);
-    if (he != hipSuccess) {
-      rc = fail(ADT_EHIP, std::string("autotune: hipMemcpyAsync: ") + hipGetErrorString(he));
-      break;
-    }
-    // time `reps` launches of one setting after one untimed launch
-    auto time_it = [&](int fl, const adt_tune& t, float* ms) -> int {
-      int r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, plan.adj_algo, plan.max_tiles, t, (adt_info*)(base + plan.off_info), s);
-      if (r) return r;
-      ADT_HIP(hipEventRecord(e0, s));
-      for (int k = 0; k < reps; ++k)
-        if ((r = launch_batch<T>(dd, nev, fl, plan.max_l, plan.max_n, plan.adj_algo, plan.max_tiles, t, (adt_info*)(base + plan.off_info), s)))
-          return r;
-      ADT_HIP(hipEventRecord(e1, s));
-      ADT_HIP(hipEventSynchronize(e1));
-      ADT_HIP(hipEventElapsedTime(ms, e0, e1));
-      *ms /= (float)reps;
-      return ADT_OK;
-    };
-    adt_tune t = cur;
-    t.order = order;
+    adt_info* inf = (adt_info*)(base + plan.off_info);
+    ADT_HIP(hipMemcpyAsync(base + plan.off_desc, plan.descs.data(), sizeof(adt::EventDesc) * n,
+                           hipMemcpyHostToDevice, s));
+    if ((r = launch_batch<T>(dd, n, fl, plan.mix, sn.global, inf, s))) return r;
+    ADT_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k)
+      if ((r = launch_batch<T>(dd, n, fl, plan.mix, sn.global, inf, s))) return r;
+    ADT_HIP(hipEventRecord(e1, s));
+    ADT_HIP(hipEventSynchronize(e1));
+    ADT_HIP(hipEventElapsedTime(ms, e0, e1));
+    *ms /= (float)reps;
+    return ADT_OK;
+  };
+  int rc = ADT_OK;
+  std::map<ClassKey, adt_tune> classes = cur.classes;
+  // --- per class: forward kernel and segments, then adjoint traversal
+  for (auto it = groups.begin(); it != groups.end() && rc == ADT_OK; ++it) {
+    const std::vector<adt_event3d>& evs = it->second;
+    const adt_event3d& x0 = evs.front();
+    TuneSnap sc;
+    sc.global = cur.for_box(x0.m, x0.n, x0.l);
+    sc.global.fuse_max_n = 0;  // the class's kernels are timed apart
     float fbest = 0.f, abest = 0.f;
-    t.fuse_max_n = 0;  // the two kernels are timed apart first
     if (flags & ADT_FORWARD) {
+      std::vector<adt_tune> cands;
+      adt_tune c = sc.global;
+      c.diag_max_l = 0;  // k-streaming
+      cands.push_back(c);
+      std::vector<int> segs = {kSegsMax, 4};
+      if (it->first.l_blocks >= 1 && it->first.l_blocks < kSegsMax) segs.push_back(it->first.l_blocks);
+      std::sort(segs.begin(), segs.end());
+      segs.erase(std::unique(segs.begin(), segs.end()), segs.end());
+      for (int k : segs) {
+        c = sc.global;
+        c.diag_max_l = adt::kDiagAll;
+        c.fwd_segs = k;
+        cands.push_back(c);
+      }
       fbest = 3.4e38f;
-      for (int dml : dmls) {
-        adt_tune c = t;
-        c.diag_max_l = dml;
+      for (const adt_tune& cand : cands) {
+        TuneSnap t1 = sc;
+        t1.global = cand;
         float ms = 0.f;
-        if ((rc = time_it(ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), c, &ms))) break;
+        if ((rc = time_set(evs, t1, ADT_FORWARD | fmix, &ms))) break;
         if (ms < fbest) {
           fbest = ms;
-          t.diag_max_l = dml;
+          sc.global.diag_max_l = cand.diag_max_l;
+          sc.global.fwd_segs = cand.fwd_segs;
         }
       }
       if (rc) break;
-      // leave this order's best forward in the outputs for the adjoint
-      if ((rc = launch_batch<T>(dd, nev, ADT_FORWARD | (flags & ADT_F32_COMPUTE_F64), plan.max_l, plan.max_n,
-                                plan.adj_algo, plan.max_tiles, t,
-                                (adt_info*)(base + plan.off_info), s)))
-        break;
+    }
+    if (flags & ADT_ADJOINT) {
+      abest = 3.4e38f;
+      for (int algo = 0; algo < 2; ++algo) {
+        if (algo == 0 && tiles_of(x0.m, x0.n, x0.l) > adt::kTileLargeMax) continue;
+        TuneSnap t1 = sc;
+        t1.global.adj_algo = algo;
+        float ms = 0.f;
+        if ((rc = time_set(evs, t1, ADT_ADJOINT, &ms))) break;
+        if (ms < abest) {
+          abest = ms;
+          sc.global.adj_algo = algo;
+        }
+      }
+      if (rc) break;
+    }
+    adt_tune ct = sc.global;
+    ct.fuse_max_n = cur.global.fuse_max_n;
+    ct.fwd_ms = fbest;
+    ct.adj_ms = abest;
+    classes[it->first] = ct;
+  }
+  // --- all events: order, node-level workgroup size, fusion
+  adt_tune win = cur.global;
+  float win_total = 3.4e38f;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) ncu = 0;
+  std::vector<adt_event3d> all(ev, ev + nev);
+  const int wgs[3] = {256, 512, 1024};
+  for (int order = 0; order < 2 && rc == ADT_OK; ++order) {
+    TuneSnap sn{cur.global, classes};
+    sn.global.order = order;
+    sn.global.fuse_max_n = 0;
+    float fbest = 0.f, abest = 0.f;
+    if (flags & ADT_FORWARD) {
+      if ((rc = time_set(all, sn, ADT_FORWARD | fmix, &fbest))) break;
+    }
+    bool any_node = false, all_fusable = (flags & ADT_FORWARD) && (flags & ADT_ADJOINT);
+    int64_t max_n = 0;
+    for (const adt_event3d& x : all) {
+      const adt_tune t = sn.for_box(x.m, x.n, x.l);
+      const bool node = adj_algo_for(t.adj_algo, tiles_of(x.m, x.n, x.l)) == 1;
+      any_node = any_node || node;
+      all_fusable = all_fusable && node && x.l <= t.diag_max_l;
+      max_n = std::max<int64_t>(max_n, x.m * x.n * x.l);
     }
     if (flags & ADT_ADJOINT) {
       abest = 3.4e38f;
       for (int wg : wgs) {
-        if (plan.adj_algo == 0 && wg != wgs[0]) break;  // the tile traversal has one shape
-        adt_tune c = t;
-        c.adj_wg = wg;
+        if (!any_node && wg != wgs[0]) break;  // the tile traversal has one shape
+        TuneSnap t1 = sn;
+        t1.global.adj_wg = wg;
         float ms = 0.f;
-        if ((rc = time_it(ADT_ADJOINT, c, &ms))) break;
+        if ((rc = time_set(all, t1, ADT_ADJOINT, &ms))) break;
         if (ms < abest) {
           abest = ms;
-          t.adj_wg = wg;
+          sn.global.adj_wg = wg;
         }
       }
       if (rc) break;
     }
-    // both kernels requested and every event diagonal: the fused kernel
-    // against the best pair of separate ones
-    // (the threshold moves to include this batch's largest event if fusing
-    // wins, below it if not, and stays as it was when fusion does not apply)
     float total = fbest + abest;
-    t.fuse_max_n = cur.fuse_max_n;
-    if ((flags & ADT_FORWARD) && (flags & ADT_ADJOINT) && plan.max_l <= t.diag_max_l &&
-        plan.adj_algo == 1) {
-      adt_tune c = t;
-      c.fuse_max_n = (int32_t)plan.max_n;
+    sn.global.fuse_max_n = cur.global.fuse_max_n;
+    if (all_fusable) {
+      TuneSnap t1 = sn;
+      t1.global.fuse_max_n = (int32_t)std::min<int64_t>(max_n, INT32_MAX);
       float ms = 0.f;
-      if ((rc = time_it(flags & (ADT_FORWARD | ADT_ADJOINT | ADT_F32_COMPUTE_F64), c, &ms))) break;
+      if ((rc = time_set(all, t1, flags & (ADT_FORWARD | ADT_ADJOINT | ADT_F32_COMPUTE_F64), &ms)))
+        break;
       if (ms < total) {
         total = ms;
-        t.fuse_max_n = std::max(cur.fuse_max_n, (int32_t)plan.max_n);
-      } else {
-        t.fuse_max_n = std::min(cur.fuse_max_n, (int32_t)(plan.max_n - 1));
+        sn.global.fuse_max_n = std::max(cur.global.fuse_max_n, t1.global.fuse_max_n);
+      } else if (ncu > 0 && nev >= ncu) {
+        sn.global.fuse_max_n = std::min(cur.global.fuse_max_n, (int32_t)(t1.global.fuse_max_n - 1));
       }
     }
     if (total < win_total) {
       win_total = total;
-      win = t;
+      win = sn.global;
       win.fwd_ms = fbest;
       win.adj_ms = abest;
     }
@@ -1355,6 +1582,7 @@ int autotune_impl(const adt_event3d* ev, int nev, int flags, int reps, adt_tune*
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[tune_index<T>()] = win;
+    g_classes[tune_index<T>()] = classes;
   }
   if (best) *best = win;
   return ADT_OK;
@@ -1377,9 +1605,10 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   if (!(flags & (ADT_FORWARD | ADT_ADJOINT))) return fail(ADT_EINVAL, "graph: no ADT_FORWARD/ADT_ADJOINT flag");
   if (mis && !((flags & ADT_FORWARD) && (flags & ADT_ADJOINT)))
     return fail(ADT_EINVAL, "misfit graph: needs ADT_FORWARD and ADT_ADJOINT");
-  const adt_tune tn = current_tune<T>();
+  const TuneSnap snap = current_snap<T>();
+  const adt_tune& tn = snap.global;
   BatchPlan<T> plan;
-  int rc = plan.build(ev, nev, flags, tn.adj_algo);
+  int rc = plan.build(ev, nev, flags, snap);
   if (rc) return rc;
   if (mis && (rc = plan.attach_misfit(ev, nev, mis))) return rc;
   if (tn.order) plan.order_largest_first();
@@ -1409,8 +1638,7 @@ int graph_create_impl(const adt_event3d* ev, int nev, int flags, GraphBase** out
   e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.max_l, plan.max_n,
-                         plan.adj_algo, plan.max_tiles, tn,
+    rc = launch_batch<T>((const adt::EventDesc*)(base + plan.off_desc), nev, flags, plan.mix, tn,
                          (adt_info*)(base + plan.off_info), cs, plan.misfit_run(base));
     hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
     if (rc == ADT_OK && e2 != hipSuccess) e = e2;
@@ -1449,7 +1677,7 @@ int adt_get_tune(int32_t dtype, adt_tune* out) {
 int adt_set_tune(int32_t dtype, const adt_tune* t) {
   if ((dtype != 0 && dtype != 1) || !t) return fail(ADT_EINVAL, "set_tune: bad arguments");
   if (!tune_valid(*t))
-    return fail(ADT_EINVAL, "set_tune: diag_max_l must be in [0, 2^30], adj_wg 256/512/1024, order 0/1, fuse_max_n >= 0");
+    return fail(ADT_EINVAL, "set_tune: diag_max_l must be in [0, 2^30], adj_wg 256/512/1024, order 0/1, fuse_max_n >= 0, adj_algo 0/1, fwd_segs 0..8");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[dtype] = *t;
   return ADT_OK;
@@ -1459,6 +1687,85 @@ int adt_autotune(const adt_event3d* events, int32_t nev, int32_t flags, int32_t 
   if (dtype == 0) return autotune_impl<double>(events, nev, flags, reps, best, (hipStream_t)stream);
   if (dtype == 1) return autotune_impl<float>(events, nev, flags, reps, best, (hipStream_t)stream);
   return fail(ADT_EINVAL, "autotune: dtype must be 0 (FP64) or 1 (FP32)");
+}
+
+int adt_tune_class_set(int32_t dtype, int64_t m, int64_t n, int64_t l, const adt_tune* t) {
+  if (dtype != 0 && dtype != 1) return fail(ADT_EINVAL, "tune_class_set: dtype must be 0 or 1");
+  int rc = check_dims3(m, n, l);
+  if (rc) return rc;
+  if (t && !tune_valid(*t)) return fail(ADT_EINVAL, "tune_class_set: setting out of range");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (t)
+    g_classes[dtype][class_of(m, n, l)] = *t;
+  else
+    g_classes[dtype].erase(class_of(m, n, l));
+  return ADT_OK;
+}
+int adt_tune_class_count(int32_t dtype) {
+  if (dtype != 0 && dtype != 1) return fail(ADT_EINVAL, "tune_class_count: dtype must be 0 or 1");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return (int)g_classes[dtype].size();
+}
+int adt_tune_class_get(int32_t dtype, int32_t i, adt_tune_class* out) {
+  if ((dtype != 0 && dtype != 1) || !out) return fail(ADT_EINVAL, "tune_class_get: bad arguments");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (i < 0 || i >= (int)g_classes[dtype].size())
+    return fail(ADT_EINVAL, "tune_class_get: index out of range");
+  auto it = g_classes[dtype].begin();
+  std::advance(it, i);
+  out->l_blocks = it->first.l_blocks;
+  out->bands = it->first.bands;
+  out->log2n = it->first.log2n;
+  out->tune = it->second;
+  return ADT_OK;
+}
+int adt_tune_class_clear(int32_t dtype) {
+  if (dtype != 0 && dtype != 1) return fail(ADT_EINVAL, "tune_class_clear: dtype must be 0 or 1");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_classes[dtype].clear();
+  return ADT_OK;
+}
+int adt_tune_save(const char* path) {
+  if (!path) return fail(ADT_EINVAL, "tune_save: NULL path");
+  std::string txt = "{\"format\": \"adtomo_hip tune 1\",\n \"global\": [";
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int d = 0; d < 2; ++d) txt += std::string(d ? ",\n  " : "\n  ") + tune_json(d, g_tune[d], nullptr);
+    txt += "],\n \"classes\": [";
+    bool first = true;
+    for (int d = 0; d < 2; ++d)
+      for (const auto& kv : g_classes[d]) {
+        txt += std::string(first ? "\n  " : ",\n  ") + tune_json(d, kv.second, &kv.first);
+        first = false;
+      }
+  }
+  txt += "]}\n";
+  FILE* fp = fopen(path, "w");
+  if (!fp) return fail(ADT_EINVAL, std::string("tune_save: cannot open ") + path);
+  const bool ok = fwrite(txt.data(), 1, txt.size(), fp) == txt.size();
+  if (fclose(fp) != 0 || !ok) return fail(ADT_EINVAL, std::string("tune_save: cannot write ") + path);
+  return ADT_OK;
+}
+int adt_tune_load(const char* path) {
+  if (!path) return fail(ADT_EINVAL, "tune_load: NULL path");
+  FILE* fp = fopen(path, "r");
+  if (!fp) return fail(ADT_EINVAL, std::string("tune_load: cannot open ") + path);
+  std::string txt;
+  char buf[4096];
+  size_t k;
+  while ((k = fread(buf, 1, sizeof buf, fp)) > 0) txt.append(buf, k);
+  fclose(fp);
+  adt_tune glob[2];
+  bool have[2] = {false, false};
+  std::map<ClassKey, adt_tune> cls[2];
+  int rc = parse_tune_json(txt, glob, have, cls);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int d = 0; d < 2; ++d) {
+    if (have[d]) g_tune[d] = glob[d];
+    g_classes[d] = cls[d];
+  }
+  return ADT_OK;
 }
 
 int adt_eikonal2d_fwd_f64(const double* f, int64_t m, int64_t n, int64_t srcx, int64_t srcy,

@@ -14,7 +14,11 @@ struct MisfitNodes;
 
 // One 3D event as seen by the persistent kernels (device resident).
 struct EventDesc {
-  int m, n, l, pad;
+  int m, n, l;
+  // launch setting of this event (its box class's, or the global one):
+  // forward on the diagonal-streaming kernel (else k-streaming), c segments
+  // of that kernel (1..8), adjoint traversal (0 tiles, 1 node by node)
+  int8_t fwd_diag, fwd_segs, adj_algo, pad;
   double h, tol;
   const void* u0;
   const void* f;
@@ -104,7 +108,7 @@ __global__ void fwd3_plane_kernel(T* u, const T* f, T h, int m, int n, int l, in
 // MIX (T = float only): FP32 storage with every local solve in FP64,
 // rounded once to FP32.
 template <typename T, bool MIX>
-__global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter, int min_l);
+__global__ void fwd3_stream_kernel(const EventDesc* evs, int max_iter);
 // Bytes of one node's record in the 3D adjoint workspace (lambda and three
 // coefficients).
 template <typename T>
@@ -113,11 +117,11 @@ constexpr size_t kAdjRecBytes = 4 * sizeof(T);
 // (it sweeps the k axis in chunks of 64 nodes).
 constexpr int kDiagAll = 1 << 30;
 template <typename T, bool MIX>
-__global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
+__global__ void fwd3_diag_kernel(const EventDesc* evs, int max_iter);
 // Forward then adjoint of each event in one workgroup (every event on the
 // diagonal forward).
 template <typename T, bool MIX>
-__global__ void fwd_adj3_diag_kernel(const EventDesc* evs, int max_iter, int max_l);
+__global__ void fwd_adj3_diag_kernel(const EventDesc* evs, int max_iter);
 __global__ void info_clear_kernel(adt_info* info, int n);
 template <typename T>
 __global__ void strided_copy_kernel(const T* src, T* dst, int64_t N, int64_t d1, int64_t d2,

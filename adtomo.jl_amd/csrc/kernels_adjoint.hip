@@ -24,14 +24,20 @@ __device__ __forceinline__ unsigned dec_byte(uint8_t* base, int64_t idx) {
 
 // ----------------------------------------------------------------- 3D ---
 
+// Each adjoint kernel takes the events whose launch setting names its
+// traversal and leaves the others to the other kernel.
 template <typename T>
 __global__ void __launch_bounds__(1024) adj3_kahn_kernel(const EventDesc* __restrict__ evs) {
-  adj3_kahn_event<T, 2, 2>(evs[blockIdx.x]);
+  const EventDesc& ev = evs[blockIdx.x];
+  if (ev.adj_algo != 1) return;
+  adj3_kahn_event<T, 2, 2>(ev);
 }
 
 template <typename T, int NW, int MAXT>
 __global__ void __launch_bounds__(NW * 64) adj3_tile_kernel(const EventDesc* __restrict__ evs) {
-  adj3_tile_event<T, NW, MAXT>(evs[blockIdx.x]);
+  const EventDesc& ev = evs[blockIdx.x];
+  if (ev.adj_algo != 0) return;
+  adj3_tile_event<T, NW, MAXT>(ev);
 }
 
 // ----------------------------------------------------------------- 2D ---

@@ -15,9 +15,11 @@
 // or lane H-1), (a-1, b) lies on diagonal v - delta and (a+1, b) on
 // v + delta, delta = N_b - H + 1 >= 33.
 //
-// The c axis is cut into kDW segments of Lw <= kLW nodes, one per wave.  At
-// step T wave w processes diagonal v = T - w over its segment, node by node
-// in increasing c.  Each node's neighbours then come from:
+// The c axis is cut into S <= kDW segments of Lw <= kLW nodes, one per wave
+// (S is the event's launch setting, kDW by default; waves beyond S idle and
+// the k axis is swept in chunks of S * kLW nodes).  At step T wave w
+// processes diagonal v = T - w over its segment, node by node in
+// increasing c.  Each node's neighbours then come from:
 //   (a-1, b), (a, b-1): the new values of diagonal v-1 that lanes p-1 and p
 //       left in the wave's LDS plane n, or, across a band edge, memory
 //       (stored delta - 1 >= 32 steps before);
@@ -105,8 +107,9 @@ struct DSweep {
   bool flipA, flipB, flipC;
   T h;
   double hd;    // h for the FP64 local solve of the FP32-storage variant
-  int cs;       // canonical c of this pass's first node (k chunks of <= 64)
+  int cs;       // canonical c of this pass's first node (k chunks)
   int Lfull;    // k extent of the event (g.L is the chunk's)
+  int nseg;     // c segments (active waves), 1..kDW
   rsrc_t u;     // travel times: new values written (and read back) here
   rsrc_t src;   // old values: u, or u0 in the very first sweep (u = u0,
                 // Eikonal3D.cpp:72, folded into that sweep)
@@ -432,15 +435,16 @@ __device__ __forceinline__ void edge_land(const u32x4& pc, lptr<T> pn, lptr<T> p
 // kernel runs to its end and reports a fault.
 constexpr int kSpinCap = 1 << 16;
 
-__device__ __forceinline__ void wait_neighbours(int* prog, int* abort_flag, int w, int t) {
+__device__ __forceinline__ void wait_neighbours(int* prog, int* abort_flag, int w, int nseg,
+                                                int t) {
   if (t == 0) return;
   for (int spin = 0;; ++spin) {
     const int a = w > 0 ? __hip_atomic_load(&prog[w - 1], __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_WORKGROUP)
                         : t;
-    const int b = w + 1 < kDW ? __hip_atomic_load(&prog[w + 1], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
-                              : t;
+    const int b = w + 1 < nseg ? __hip_atomic_load(&prog[w + 1], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP)
+                               : t;
     if (a >= t && b >= t) break;
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
     if (spin >= kSpinCap) {
@@ -477,7 +481,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   const bool fwd = !S.flipC;  // the segment runs forward in memory
   const bool lane_first = (p == 0);
   const bool lane_last = (p == g.H - 1);
-  const bool has_x = c0 + Lw < L && w + 1 < kDW;  // a node exists one past the segment
+  const bool has_x = c0 + Lw < L && w + 1 < S.nseg;  // a node exists one past the segment
   // planes of this wave: new values of v-1 (then v), old values of v+1,
   // slowness of v
   lptr<T> pn = planes + w * 3 * P::PLANE;
@@ -558,7 +562,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   if (p == 0) __hip_atomic_store(&prog[w], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   __syncthreads();
 
-  const int steps = g.V + kDW - 1;
+  const int steps = g.V + S.nseg - 1;
   int seg_prev = -1;  // segment of the previous step's new values (-1: none)
   for (int t = 0; t < steps; ++t) {
     const int v = v0 + t;
@@ -601,7 +605,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     // v+2, slowness and band edges of v+1) are issued with the first
     // nodes, every step, so they have nearly a whole step to arrive.
     T xo = T(0);
-    wait_neighbours(prog, abort_flag, w, t);
+    wait_neighbours(prog, abort_flag, w, S.nseg, t);
     {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : XCM;
       const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : XCP;
@@ -684,8 +688,10 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
 }
 
 // One sweep of one event by the whole workgroup.  Waves whose segment has
-// kLW nodes take the fixed-length variant; the others (and idle waves) the
-// general one.  Both run the same number of steps and barriers.
+// kLW nodes take the fixed-length variant; the others (and empty segments)
+// the general one.  Both run the same number of steps and barriers.  Waves
+// beyond the event's segment count only meet the sweep's two workgroup
+// barriers.
 // ERR (last sweep of an iteration): returns this lane's max |u - snapshot|
 // key and writes the next snapshot.
 template <typename T, bool MIX, bool ERR>
@@ -694,9 +700,14 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
   // the wave index is uniform across the wave: keep it (and everything
   // derived from it) in scalar registers
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (w >= S.nseg) {  // the prologue's and the end's barriers of the active waves
+    __syncthreads();
+    __syncthreads();
+    return 0;
+  }
   int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
-  if (S.Lfull > kDW * kLW) {
+  if (S.Lfull > S.nseg * kLW) {
     if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, true>(S, w, cnt, xch, xold, planes, prog, abort_flag);
     return diag_sweep_wave<T, false, MIX, ERR, true>(S, w, cnt, xch, xold, planes, prog, abort_flag);
   }
@@ -706,7 +717,7 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
 
 // The whole forward of one event, run by one workgroup of kDW waves.
 template <typename T, bool MIX>
-__device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_iter, int max_l) {
+__device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_iter) {
   __shared__ T xch[2 * kDW * 64];
   __shared__ T xold[2 * kDW * 64];
   __shared__ T planes[kDW * 3 * Pieces<T>::PLANE];
@@ -714,7 +725,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   __shared__ int prog[kDW];
   __shared__ int abort_flag;
   const int m = ev.m, n = ev.n, l = ev.l;
-  if (l > max_l) return;  // handled by the k-streaming kernel
+  if (!ev.fwd_diag) return;  // handled by the k-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
   if (threadIdx.x == 0) abort_flag = 0;
@@ -722,10 +733,12 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   DSweep<T> S;
   S.g = make_dgeom(m, n, l);
   S.Lfull = l;
-  // the k axis is swept in chunks of kDW * kLW nodes (the last one shorter),
-  // in sweep order: a chunk's nodes read the previous chunk's as new values
-  // and the next chunk's as old ones, as the lexicographic sweep does
-  constexpr int kChunk = kDW * kLW;
+  S.nseg = (ev.fwd_segs >= 1 && ev.fwd_segs <= kDW) ? ev.fwd_segs : kDW;
+  // the k axis is swept in chunks of nseg * kLW nodes (the last one
+  // shorter), in sweep order: a chunk's nodes read the previous chunk's as
+  // new values and the next chunk's as old ones, as the lexicographic sweep
+  // does
+  const int kChunk = S.nseg * kLW;
   const int nchunk = (l + kChunk - 1) / kChunk;
   S.h = (T)ev.h;
   S.hd = ev.h;
@@ -755,7 +768,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
         S.cs = ch * kChunk;
         const int Lc = l - S.cs < kChunk ? l - S.cs : kChunk;
         S.g.L = Lc;
-        S.g.Lw = (Lc + kDW - 1) / kDW;
+        S.g.Lw = (Lc + S.nseg - 1) / S.nseg;
         S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
                   (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) +
                   (S.flipC ? (int64_t)(l - 1 - S.cs) : (int64_t)S.cs);
@@ -799,8 +812,8 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
 
 template <typename T, bool MIX>
 __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __restrict__ evs,
-                                                             int max_iter, int max_l) {
-  fwd3_diag_event<T, MIX>(evs[blockIdx.x], max_iter, max_l);
+                                                             int max_iter) {
+  fwd3_diag_event<T, MIX>(evs[blockIdx.x], max_iter);
 }
 
 // Forward and then adjoint of one event per workgroup (with the event's
@@ -813,9 +826,9 @@ __global__ void __launch_bounds__(kDW * 64) fwd3_diag_kernel(const EventDesc* __
 // thread (the forward's register budget leaves room for them).
 template <typename T, bool MIX>
 __global__ void __launch_bounds__(kDW * 64) fwd_adj3_diag_kernel(const EventDesc* __restrict__ evs,
-                                                                 int max_iter, int max_l) {
+                                                                 int max_iter) {
   const EventDesc& ev = evs[blockIdx.x];
-  fwd3_diag_event<T, MIX>(ev, max_iter, max_l);
+  fwd3_diag_event<T, MIX>(ev, max_iter);
   // the next phase reads the travel times the waves of this workgroup stored
   __threadfence();
   __syncthreads();
@@ -831,11 +844,11 @@ __global__ void __launch_bounds__(kDW * 64) fwd_adj3_diag_kernel(const EventDesc
   adj3_kahn_event<T, 4, 4>(ev);
 }
 
-template __global__ void fwd3_diag_kernel<double, false>(const EventDesc*, int, int);
-template __global__ void fwd3_diag_kernel<float, false>(const EventDesc*, int, int);
-template __global__ void fwd3_diag_kernel<float, true>(const EventDesc*, int, int);
-template __global__ void fwd_adj3_diag_kernel<double, false>(const EventDesc*, int, int);
-template __global__ void fwd_adj3_diag_kernel<float, false>(const EventDesc*, int, int);
-template __global__ void fwd_adj3_diag_kernel<float, true>(const EventDesc*, int, int);
+template __global__ void fwd3_diag_kernel<double, false>(const EventDesc*, int);
+template __global__ void fwd3_diag_kernel<float, false>(const EventDesc*, int);
+template __global__ void fwd3_diag_kernel<float, true>(const EventDesc*, int);
+template __global__ void fwd_adj3_diag_kernel<double, false>(const EventDesc*, int);
+template __global__ void fwd_adj3_diag_kernel<float, false>(const EventDesc*, int);
+template __global__ void fwd_adj3_diag_kernel<float, true>(const EventDesc*, int);
 
 }  // namespace adt

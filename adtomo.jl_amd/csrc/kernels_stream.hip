@@ -339,13 +339,13 @@ __device__ void stream_sweep(const Sweep<T>& S_in, lptr<T> lds_new, lptr<T> lds_
 
 template <typename T, bool MIX>
 __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* __restrict__ evs,
-                                                              int max_iter, int min_l) {
+                                                              int max_iter) {
   __shared__ T lds_new[2 * kW * 64];
   __shared__ T lds_old[2 * kW * 64];
   __shared__ uint64_t red[kW];
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
-  if (l <= min_l) return;  // handled by the diagonal-streaming kernel
+  if (ev.fwd_diag) return;  // handled by the diagonal-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
   Sweep<T> S;
@@ -409,8 +409,8 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
   }
 }
 
-template __global__ void fwd3_stream_kernel<double, false>(const EventDesc*, int, int);
-template __global__ void fwd3_stream_kernel<float, false>(const EventDesc*, int, int);
-template __global__ void fwd3_stream_kernel<float, true>(const EventDesc*, int, int);
+template __global__ void fwd3_stream_kernel<double, false>(const EventDesc*, int);
+template __global__ void fwd3_stream_kernel<float, false>(const EventDesc*, int);
+template __global__ void fwd3_stream_kernel<float, true>(const EventDesc*, int);
 
 }  // namespace adt

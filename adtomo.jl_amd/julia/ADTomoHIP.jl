@@ -22,7 +22,8 @@ using Libdl
 
 export eikonal, eikonal3d, eikonal3d_batched!, EikonalGraph, launch!, close!, source_u0!,
        misfit!, model!, model_grad!, slowness!, slowness_grad!, smooth_l1!, sum_arrays!,
-       AdtTune, get_tune, set_tune!, autotune!, eikonal3d_misfit_batched!
+       AdtTune, AdtTuneClass, get_tune, set_tune!, set_tune_class!, tune_classes, save_tune,
+       load_tune, autotune!, eikonal3d_misfit_batched!
 
 const libadt = normpath(joinpath(@__DIR__, "..", "libadtomo_hip.so"))
 const ADT_HOST_POINTERS = Int32(0x1)
@@ -457,6 +458,15 @@ struct AdtTune
     fwd_ms::Float32
     adj_ms::Float32
     adj_algo::Int32
+    fwd_segs::Int32
+end
+
+# Mirrors `adt_tune_class` in include/adtomo_hip.h.
+struct AdtTuneClass
+    l_blocks::Int32
+    bands::Int32
+    log2n::Int32
+    tune::AdtTune
 end
 
 dtype_code(::Type{Float32}) = Int32(1)
@@ -468,7 +478,7 @@ dtype_code(::Type{Float64}) = Int32(0)
 Launch setting of the 3D kernels for element type `T`.
 """
 function get_tune(::Type{T}=Float64) where {T}
-    t = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0, 0))
+    t = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0, 0, 0))
     check(ccall((:adt_get_tune, libadt), Cint, (Int32, Ref{AdtTune}), dtype_code(T), t),
           "get_tune")
     return t[]
@@ -486,19 +496,69 @@ function set_tune!(t::AdtTune, ::Type{T}=Float64) where {T}
 end
 
 """
+    set_tune_class!(dims, t::Union{AdtTune,Nothing}, T=Float64)
+
+Gives the box-shape class of `dims` = (m, n, l) the per-event fields of `t`
+(diag_max_l, fwd_segs, adj_algo), or removes its setting (`nothing`).
+"""
+function set_tune_class!(dims, t::Union{AdtTune,Nothing}, ::Type{T}=Float64) where {T}
+    m, n, l = Int64.(dims)
+    if t === nothing
+        rc = ccall((:adt_tune_class_set, libadt), Cint, (Int32, Int64, Int64, Int64, Ptr{AdtTune}),
+                   dtype_code(T), m, n, l, C_NULL)
+    else
+        rc = ccall((:adt_tune_class_set, libadt), Cint,
+                   (Int32, Int64, Int64, Int64, Ref{AdtTune}), dtype_code(T), m, n, l, Ref(t))
+    end
+    check(rc, "tune_class_set")
+    return t
+end
+
+"""
+    tune_classes(T=Float64)
+
+The class settings of element type `T` (`AdtTuneClass` values).
+"""
+function tune_classes(::Type{T}=Float64) where {T}
+    n = ccall((:adt_tune_class_count, libadt), Cint, (Int32,), dtype_code(T))
+    check(min(n, 0), "tune_class_count")
+    out = AdtTuneClass[]
+    for i in 0:n-1
+        c = Ref(AdtTuneClass(0, 0, 0, AdtTune(0, 0, 0, 0, 0f0, 0f0, 0, 0)))
+        check(ccall((:adt_tune_class_get, libadt), Cint, (Int32, Int32, Ref{AdtTuneClass}),
+                    dtype_code(T), Int32(i), c), "tune_class_get")
+        push!(out, c[])
+    end
+    return out
+end
+
+"""
+    save_tune(path), load_tune(path)
+
+Writes every launch setting (global and per class) to a JSON file, or reads
+one back.
+"""
+save_tune(path::AbstractString) =
+    check(ccall((:adt_tune_save, libadt), Cint, (Cstring,), path), "tune_save")
+load_tune(path::AbstractString) =
+    check(ccall((:adt_tune_load, libadt), Cint, (Cstring,), path), "tune_load")
+
+"""
     autotune!(us, u0s, fs, hs, tols; grad_us, grad_u0s, grad_fs, reps=3,
               compute_f64=false, stream=C_NULL)
 
 Times the launch settings on these events (device arrays, as for
-`eikonal3d_batched!`), keeps the fastest for their element type and
-returns it; the events' outputs hold a valid solve afterwards.
+`eikonal3d_batched!`): per box-shape class the forward kernel, its segment
+count and the adjoint traversal, over all events the launch-wide fields;
+keeps the fastest for their element type and returns the global setting;
+the events' outputs hold a valid solve afterwards.
 """
 function autotune!(us, u0s, fs, hs, tols; grad_us=nothing, grad_u0s=nothing, grad_fs=nothing,
                    reps::Integer=3, compute_f64::Bool=false, stream::Ptr{Cvoid}=C_NULL)
     tab = event_table(u0s, fs, us, hs, tols, grad_us, grad_u0s, grad_fs)
     flags = ADT_FORWARD | (grad_us === nothing ? Int32(0) : ADT_ADJOINT) |
             (compute_f64 ? ADT_F32_COMPUTE_F64 : Int32(0))
-    best = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0, 0))
+    best = Ref(AdtTune(0, 0, 0, 0, 0f0, 0f0, 0, 0))
     rc = ccall((:adt_autotune, libadt), Cint,
                (Ptr{AdtEvent3d}, Int32, Int32, Int32, Int32, Ref{AdtTune}, Ptr{Cvoid}),
                tab, length(tab), flags, dtype_code(eltype(fs[1])), Int32(reps), best, stream)

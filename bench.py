@@ -55,6 +55,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _class_key(shape):
+    m, n, l = shape
+    return (-(-l // 8), -(-min(m, n) // 64), (m * n * l).bit_length() - 1)
+
+
+def effective_setting(A, dtype, shape):
+    """The setting events of this box run with: the global setting with its
+    per-event fields (diag_max_l, fwd_segs, adj_algo) taken from the box's
+    class setting when there is one."""
+    tune = A.get_tune(dtype)
+    for c in A.tune_classes(dtype):
+        if (c["l_blocks"], c["bands"], c["log2n"]) == _class_key(shape):
+            tune.update({k: c[k] for k in ("diag_max_l", "fwd_segs", "adj_algo")})
+            tune["class"] = {k: c[k] for k in ("l_blocks", "bands", "log2n", "fwd_ms", "adj_ms")}
+    return tune
+
+
+def class_report(A, dtype):
+    return [{k: c[k] for k in ("l_blocks", "bands", "log2n", "diag_max_l", "fwd_segs", "adj_algo",
+                               "fwd_ms", "adj_ms")} for c in A.tune_classes(dtype)]
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -147,12 +169,12 @@ def main():
     del u0s, gs
 
     # launch setting: timed on this workload by the library's autotuner
-    if args.no_autotune:
-        tune = A.get_tune("float64")
-    else:
+    if not args.no_autotune:
         t0 = time.time()
-        tune = A.autotune(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd, reps=1)
-        log(f"autotune ({time.time() - t0:.1f}s): {tune}")
+        A.autotune(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd, reps=1)
+        log(f"autotune ({time.time() - t0:.1f}s): {A.get_tune('float64')}, "
+            f"classes {class_report(A, 'float64')}")
+    tune = effective_setting(A, "float64", shape)
 
     G = A.EikonalGraph(u0d, fs, [h] * nev, [tol] * nev, grad_us=gd)
     for _ in range(args.warmup):
@@ -317,7 +339,8 @@ def main():
                    "serial_depth_per_sweep": int(sum(shape) - 2),
                    "adj_sweeps_mean": float(sweeps.mean()),
                    "launch_setting": {k: tune[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n",
-                                                           "adj_algo")}},
+                                                           "adj_algo", "fwd_segs") + (
+                                                               ("class",) if "class" in tune else ())}},
         "roofline": dominant,
         "roofline_kernels": roofs,
         "cpu_baseline": cpu,
@@ -381,6 +404,7 @@ def run_uneven(args):
         ud = [torch.from_numpy(a).to("cuda", dt) for a in u0s]
         gd = [torch.from_numpy(a).to("cuda", dt) for a in gs]
         tune = A.autotune(ud, fd, [h] * nev, [tol] * nev, grad_us=gd, reps=1, compute_f64=mix)
+        tune["classes"] = class_report(A, "float32" if name != "f64" else "float64")
         G = A.EikonalGraph(ud, fd, [h] * nev, [tol] * nev, grad_us=gd, compute_f64=mix)
         for _ in range(args.warmup):
             G.launch()
@@ -402,7 +426,8 @@ def run_uneven(args):
           {"workload": "256 events, boxes uniform in 48x48x24..128x128x64, h=0.5 km, layered "
                        "3.5-7.5 km/s + 3% GRF, tol 1e-3, fwd+adjoint, one hipGraph",
            "events": nev, "nodes_total": nodes, "K_mean": float(K64.mean()),
-           "launch_setting": {k: tune64[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n")}},
+           "launch_setting": {k: tune64[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n",
+                                                     "classes")}},
           fp32=variant("f32"), fp32_storage_fp64_solve=variant("mixed"))
 
 

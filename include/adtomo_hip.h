@@ -209,10 +209,13 @@ int adt_graph_launch(adt_graph *g, void *stream);
 int adt_graph_infos(adt_graph *g, adt_info *infos, void *stream);
 int adt_graph_destroy(adt_graph *g);
 
-/* ---- Launch tuning of the 3D kernels (one setting per precision; dtype 0
- * = FP64, 1 = FP32).  Read when a batched call runs or a graph is created;
- * every setting gives the same travel times (bit for bit) and gradients
- * (to rounding). */
+/* ---- Launch tuning of the 3D kernels (one global setting per precision;
+ * dtype 0 = FP64, 1 = FP32, and optionally one per box-shape class, see
+ * adt_tune_class_set).  Read when a call runs or a graph is created (one
+ * graph may hold events of several classes, each with its class's
+ * setting); every setting gives the same travel times (bit for bit) and
+ * gradients (to rounding).  The reference has no counterpart (its op is
+ * single-threaded CPU code). */
 typedef struct adt_tune {
   int32_t diag_max_l; /* forward: events with l <= diag_max_l use the
                          diagonal-streaming kernel (any l, in k chunks of
@@ -236,15 +239,46 @@ typedef struct adt_tune {
                          most 65536 tiles, larger ones take 1), 1 = node by
                          node over the whole event (the only one that fuses
                          with the forward, see fuse_max_n) */
+  int32_t fwd_segs;   /* diagonal forward: the k axis is cut into fwd_segs
+                         segments of up to 8 nodes, one per wave (1..8; 0 =
+                         8), and swept in chunks of 8 * fwd_segs nodes */
 } adt_tune;
 int adt_get_tune(int32_t dtype, adt_tune *out);
 int adt_set_tune(int32_t dtype, const adt_tune *t);
-/* Times every setting on the given events (device pointers, as for
- * adt_eikonal3d_batched_*; `reps` timed launches each), stores the fastest
- * for `dtype` and returns it in `best` (may be NULL).  The events' outputs
- * hold a valid solve afterwards.  With ADT_ADJOINT but not ADT_FORWARD, u
- * must already hold the forward travel times.  The reference has no
- * counterpart (its op is single-threaded CPU code). */
+
+/* Box-shape classes: events whose k extent l needs the same number of
+ * 8-node blocks (l_blocks = ceil(l / 8)), whose shorter horizontal side
+ * min(m, n) the same number of 64-row bands (bands = ceil(min(m, n) / 64))
+ * and whose node count the same binary order (log2n = floor(log2(m n l)))
+ * share a class.  A class setting replaces the global one's per-event
+ * fields (diag_max_l, fwd_segs, adj_algo) for the events of that class;
+ * adj_wg, order and fuse_max_n stay global. */
+typedef struct adt_tune_class {
+  int32_t l_blocks, bands, log2n;
+  adt_tune tune;
+} adt_tune_class;
+/* Sets (t != NULL) or removes (t == NULL) the setting of the class of box
+ * (m, n, l). */
+int adt_tune_class_set(int32_t dtype, int64_t m, int64_t n, int64_t l, const adt_tune *t);
+/* Number of class settings of dtype; adt_tune_class_get returns the i-th
+ * (in class order). */
+int adt_tune_class_count(int32_t dtype);
+int adt_tune_class_get(int32_t dtype, int32_t i, adt_tune_class *out);
+int adt_tune_class_clear(int32_t dtype);
+/* Writes every setting (global and class, both precisions) to a JSON file,
+ * or reads such a file back (replacing the settings it holds: the global
+ * ones it names and all class ones). */
+int adt_tune_save(const char *path);
+int adt_tune_load(const char *path);
+/* Times the settings on the given events (device pointers, as for
+ * adt_eikonal3d_batched_*; `reps` timed launches each): for each box-shape
+ * class among them, on that class's events, the forward kernel and its
+ * segment count and the adjoint traversal, stored as the class's setting;
+ * then, on all events, the event order, the node-level adjoint's workgroup
+ * size and the fusion threshold, stored as the global setting for `dtype`
+ * and returned in `best` (may be NULL).  The events' outputs hold a valid
+ * solve afterwards.  With ADT_ADJOINT but not ADT_FORWARD, u must already
+ * hold the forward travel times. */
 int adt_autotune(const adt_event3d *events, int32_t nev, int32_t flags, int32_t dtype,
                  int32_t reps, adt_tune *best, void *stream);
 

@@ -1,6 +1,7 @@
 """CPU checks: golden fixtures against the oracle, and the C-ABI library's
 exports against include/adtomo_hip.h (no device calls)."""
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -140,12 +141,56 @@ def test_tune_get_set_validation_without_device():
         assert A.get_tune("float32")["adj_wg"] == base["adj_wg"]  # per precision
         assert A.get_tune("float32")["fuse_max_n"] == base["fuse_max_n"]
         for bad in ({"diag_max_l": (1 << 30) + 1}, {"diag_max_l": -1}, {"adj_wg": 300},
-                    {"order": 2}, {"fuse_max_n": -1}):
+                    {"order": 2}, {"fuse_max_n": -1}, {"adj_algo": 2}, {"fwd_segs": -1},
+                    {"fwd_segs": 9}):
             with pytest.raises(A.AdtError):
                 A.set_tune("float64", **bad)
+        with pytest.raises(TypeError):
+            A.set_tune("float64", tile=4)
     finally:
-        A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"], fuse_max_n=base["fuse_max_n"])
+        A.set_tune("float64", **{k: base[k] for k in ("diag_max_l", "adj_wg", "order",
+                                                      "fuse_max_n", "adj_algo", "fwd_segs")})
+
+
+def test_tune_classes_and_json_round_trip_without_device(tmp_path):
+    # class settings and the JSON file are host state too
+    import adtomo_hip as A
+    saved = str(tmp_path / "saved.json")
+    A.save_tune(saved)
+    try:
+        A.clear_tune_classes("float64")
+        A.clear_tune_classes("float32")
+        assert A.tune_classes("float64") == []
+        A.set_tune_class((128, 128, 64), "float64", fwd_segs=4, adj_algo=1)
+        A.set_tune_class((48, 100, 24), "float64", diag_max_l=0)
+        A.set_tune_class((128, 130, 64), "float64", fwd_segs=2)  # same class as the first
+        A.set_tune_class((40, 40, 40), "float32", fwd_segs=5)
+        c64 = A.tune_classes("float64")
+        assert [(c["l_blocks"], c["bands"], c["log2n"]) for c in c64] == [(3, 1, 16), (8, 2, 20)]
+        assert c64[1]["fwd_segs"] == 2 and c64[0]["diag_max_l"] == 0
+        assert [(c["l_blocks"], c["fwd_segs"]) for c in A.tune_classes("float32")] == [(5, 5)]
+        with pytest.raises(A.AdtError):
+            A.set_tune_class((16, 16, 16), "float64", fwd_segs=9)
+        with pytest.raises(A.AdtError):
+            A.set_tune_class((1, 16, 16), "float64", fwd_segs=2)
+        f = str(tmp_path / "t.json")
+        A.save_tune(f)
+        json.load(open(f))  # valid JSON
+        A.clear_tune_classes("float64")
+        A.set_tune("float64", order=1 - A.get_tune("float64")["order"])
+        A.load_tune(f)
+        assert A.tune_classes("float64") == c64
+        assert A.get_tune("float64")["order"] == json.load(open(f))["global"][0]["order"]
+        A.set_tune_class((128, 128, 64), "float64", clear=True)
+        assert len(A.tune_classes("float64")) == 1
+        bad = str(tmp_path / "bad.json")
+        open(bad, "w").write('{"global": [{"dtype": 0, "adj_wg": 256}]}')
+        with pytest.raises(A.AdtError):
+            A.load_tune(bad)
+        with pytest.raises(A.AdtError):
+            A.load_tune(str(tmp_path / "missing.json"))
+    finally:
+        A.load_tune(saved)
 
 
 def _c_struct_fields(name):
@@ -176,7 +221,7 @@ def _julia_struct_fields(name):
 @pytest.mark.parametrize("c_name,py_name,jl_name", [
     ("adt_info", "AdtInfo", "AdtInfo"), ("adt_event3d", "AdtEvent3d", "AdtEvent3d"),
     ("adt_source3d", "AdtSource3d", "AdtSource3d"), ("adt_misfit3d", "AdtMisfit3d", "AdtMisfit3d"),
-    ("adt_tune", "AdtTune", "AdtTune")])
+    ("adt_tune", "AdtTune", "AdtTune"), ("adt_tune_class", "AdtTuneClass", "AdtTuneClass")])
 def test_struct_layouts_agree_across_bindings(c_name, py_name, jl_name):
     # the C header, the ctypes twin and the Julia module declare the same
     # fields in the same order

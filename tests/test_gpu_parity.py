@@ -573,10 +573,20 @@ def _mixed_batch(seed=300):
     {"diag_max_l": 1 << 30, "adj_wg": 1024, "order": 1, "fuse_max_n": 1 << 30, "adj_algo": 0},
     {"diag_max_l": 0, "adj_wg": 256, "order": 0, "fuse_max_n": 0, "adj_algo": 0},
     {"diag_max_l": 32, "adj_wg": 512, "order": 1, "fuse_max_n": 0, "adj_algo": 0},
+    # segment counts of the diagonal forward (k chunks of 8, 24, 40, 56 nodes)
+    {"diag_max_l": 1 << 30, "adj_wg": 256, "order": 1, "fuse_max_n": 0, "adj_algo": 0,
+     "fwd_segs": 1},
+    {"diag_max_l": 1 << 30, "adj_wg": 512, "order": 0, "fuse_max_n": 1 << 30, "adj_algo": 1,
+     "fwd_segs": 3},
+    {"diag_max_l": 1 << 30, "adj_wg": 256, "order": 1, "fuse_max_n": 0, "adj_algo": 0,
+     "fwd_segs": 5},
+    {"diag_max_l": 64, "adj_wg": 1024, "order": 0, "fuse_max_n": 0, "adj_algo": 1,
+     "fwd_segs": 7},
 ])
-def test_every_tune_setting_matches_oracle(setting):
+def test_every_tune_setting_matches_oracle(setting, tmp_path):
     import adtomo_hip as A
-    base = A.get_tune("float64")
+    saved = str(tmp_path / "tune.json")
+    A.save_tune(saved)
     fs, u0s, gs = _mixed_batch()
     try:
         A.set_tune("float64", **setting)
@@ -597,8 +607,45 @@ def test_every_tune_setting_matches_oracle(setting):
                 assert rel(host(gfs[e]), gf_ref) <= GRAD_RTOL, (setting, e)
         G.close()
     finally:
-        A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"], fuse_max_n=base["fuse_max_n"], adj_algo=base["adj_algo"])
+        A.load_tune(saved)
+
+
+def test_class_settings_mix_in_one_graph(tmp_path):
+    # events of several box-shape classes, each class with its own forward
+    # kernel, segment count and adjoint traversal, in one call and one graph:
+    # travel times bit for bit and gradients to the bar of the oracle
+    import adtomo_hip as A
+    saved = str(tmp_path / "tune.json")
+    A.save_tune(saved)
+    fs, u0s, gs = _mixed_batch(310)
+    per_class = [{"diag_max_l": 0, "adj_algo": 1}, {"fwd_segs": 3, "adj_algo": 0},
+                 {"fwd_segs": 8, "adj_algo": 1}, {"fwd_segs": 2, "adj_algo": 0},
+                 {"diag_max_l": 1 << 30, "fwd_segs": 6, "adj_algo": 0}, {"fwd_segs": 1}]
+    try:
+        A.clear_tune_classes("float64")
+        A.set_tune("float64", diag_max_l=1 << 30, fwd_segs=0, adj_algo=0, fuse_max_n=0)
+        for f, cfg in zip(fs, per_class):
+            A.set_tune_class(f.shape, "float64", **cfg)
+        assert len(A.tune_classes("float64")) == len({(-(-f.shape[2] // 8), -(-min(f.shape[:2]) // 64),
+                                                         int(np.log2(f.size))) for f in fs})
+        res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
+                                  [1e-4] * len(fs), grad_us=[dev(a) for a in gs])
+        G = A.EikonalGraph([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
+                           [1e-4] * len(fs), grad_us=[dev(a) for a in gs])
+        G.launch()
+        G.launch()
+        ginfo = G.infos()
+        for e in range(len(fs)):
+            u_ref, it, _ = oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-4)
+            _, gf_ref, _ = oracle.eikonal3d_grad(gs[e], u_ref, u0s[e], fs[e], 0.5)
+            for us, gfs, info in ((res["u"], res["grad_f"], res["info"][e]),
+                                  (G.us, G.grad_fs, ginfo[e])):
+                assert info["iterations"] == it, e
+                assert bits_equal(host(us[e]), u_ref), e
+                assert rel(host(gfs[e]), gf_ref) <= GRAD_RTOL, e
+        G.close()
+    finally:
+        A.load_tune(saved)
 
 
 @pytest.mark.parametrize("dtype,compute_f64", [("float64", False), ("float32", False),
@@ -699,26 +746,42 @@ def test_tile_adjoint_large_tile_count_variant():
     assert rel(host(gf), gf_ref) <= GRAD_RTOL
 
 
-def test_autotune_stores_a_valid_setting():
+def test_autotune_stores_a_valid_setting(tmp_path):
+    # per-class settings for each class of the batch, a global setting, the
+    # JSON cache written after timing and read instead of timing
     import adtomo_hip as A
-    base = A.get_tune("float64")
+    saved = str(tmp_path / "tune.json")
+    cache = str(tmp_path / "cache.json")
+    A.save_tune(saved)
     fs, u0s, gs = _mixed_batch(400)
+    args = ([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs), [1e-3] * len(fs))
     try:
-        best = A.autotune([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
-                          [1e-3] * len(fs), grad_us=[dev(a) for a in gs], reps=2)
+        A.clear_tune_classes("float64")
+        best = A.autotune(*args, grad_us=[dev(a) for a in gs], reps=2, cache=cache)
         print("autotune:", best, flush=True)
-        assert best["diag_max_l"] in (0, 64, 1 << 30) and best["adj_wg"] in (256, 512, 1024)
+        assert best["adj_wg"] in (256, 512, 1024)
         assert best["order"] in (0, 1) and best["fwd_ms"] > 0 and best["adj_ms"] > 0
         assert best["fuse_max_n"] >= 0
         cur = A.get_tune("float64")
-        assert all(cur[k] == best[k] for k in ("diag_max_l", "adj_wg", "order", "fuse_max_n"))
-        res = A.eikonal3d_batched([dev(a) for a in u0s], [dev(a) for a in fs], [0.5] * len(fs),
-                                  [1e-3] * len(fs))
+        assert all(cur[k] == best[k] for k in ("adj_wg", "order", "fuse_max_n"))
+        classes = A.tune_classes("float64")
+        keys = {(-(-f.shape[2] // 8), -(-min(f.shape[:2]) // 64), int(np.log2(f.size))) for f in fs}
+        assert {(c["l_blocks"], c["bands"], c["log2n"]) for c in classes} == keys
+        for c in classes:
+            print("class:", c, flush=True)
+            assert c["diag_max_l"] in (0, 1 << 30) and c["adj_algo"] in (0, 1)
+            assert 1 <= c["fwd_segs"] <= 8 and c["fwd_ms"] > 0 and c["adj_ms"] > 0
+        res = A.eikonal3d_batched(*args)
         for e in range(len(fs)):
             assert bits_equal(host(res["u"][e]), oracle.eikonal3d(u0s[e], fs[e], 0.5, 1e-3)[0])
+        # the cache holds what was chosen and is read back instead of timing
+        assert os.path.exists(cache)
+        A.clear_tune_classes("float64")
+        again = A.autotune(*args, grad_us=[dev(a) for a in gs], reps=2, cache=cache)
+        assert all(again[k] == best[k] for k in ("adj_wg", "order", "fuse_max_n"))
+        assert A.tune_classes("float64") == classes
     finally:
-        A.set_tune("float64", diag_max_l=base["diag_max_l"], adj_wg=base["adj_wg"],
-                   order=base["order"], fuse_max_n=base["fuse_max_n"])
+        A.load_tune(saved)
 
 
 # ------------------------------------------- FP32 storage, FP64 solves ---

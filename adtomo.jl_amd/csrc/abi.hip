@@ -249,7 +249,7 @@ void adj_ws_bytes(int algo, int64_t N, int64_t tiles, size_t* lam, size_t* code,
   if (algo == 0) {
     *lam = sizeof(T) * size_t(N);  // multipliers
     *code = size_t(N);             // upwind codes
-    *queue = 4 * size_t(tiles);    // external-edge count per tile
+    *queue = 16 * size_t(tiles);   // per tile: external-edge count, rank, inverse, key
   } else {
     *lam = adt::kAdjRecBytes<T> * size_t(N);  // per-node records
     *code = 2 * size_t(N);                    // 16-bit node states

@@ -23,9 +23,17 @@
 // reaches zero resolves completely in one visit.  Tiles whose external
 // feeders depend on each other in a cycle (the node graph is acyclic, the
 // tile graph need not be) are marked dirty whenever one of their external
-// feeders resolves; when no tile is ready and no wave is visiting, the dirty
-// tiles are visited as they are, which always makes progress (the
-// unresolved node of largest u has all its feeders final).
+// feeders resolves; a wave that finds no ready tile visits a dirty one as
+// it is.  That always leaves some work reachable: a tile that is neither
+// ready nor dirty nor being visited has had a visit since its last
+// delivery, which resolved all it could then, and the unresolved node of
+// largest u (all its feeders final) lies in a tile that has not.
+//
+// Multipliers flow from large to small travel times, so tiles are taken in
+// decreasing order of the largest travel time at their corners (ranked
+// once, by a counting sort over 1024 bins): the bitmaps are indexed by
+// that rank and a wave takes the ready (else dirty) tile of lowest rank.
+// Most tiles then resolve in a single visit.
 //
 // One workgroup per event: the scheduling state (bitmaps, counters) lives
 // in LDS and global memory is shared only among the waves of one compute
@@ -89,14 +97,12 @@ struct WaveBuf {
 enum { kReady = 0, kDirty = 1, kBusy = 2, kDone = 3, kVisited = 4, kMaps = 5 };
 struct Sched {
   int remaining;   // non-Z nodes not final
-  int active;      // waves holding a tile
-  int lock;        // deadlock-round mover
+  int active;      // waves holding a tile (diagnostics)
   int fault;       // a bounded wait gave up
   int visits;      // tile visits (diagnostics)
-  int rounds;      // dirty rounds (diagnostics)
-  int cursor;      // bitmap word where the next pop starts
-  int pad;
+  unsigned kmin, kmax;  // order-preserving bits of the smallest / largest tile key
 };
+constexpr int kBins = 1024;  // counting-sort bins of the tile order (+1 for done tiles)
 
 // Bounded idle polling of a wave: past this many polls without any node
 // resolving, the kernel reports a fault instead of waiting on.
@@ -113,7 +119,11 @@ struct Ev {
   gptr<const T> u, u0, g, f;
   gptr<T> gu0, gf, lam;
   gptr<uint8_t> code;
-  gptr<int> ext;
+  // per tile (ntiles each): count of unresolved external feeding edges (-1:
+  // no non-Z node), rank in the visiting order and its inverse, and the
+  // ordering key (largest travel time at the tile's corners)
+  gptr<int> ext, t2r, r2t;
+  gptr<float> key;
 };
 
 // Node of slot s (0..13) of a lane in the tile box: slots 0-7 are the tile
@@ -249,19 +259,36 @@ __device__ __forceinline__ uint8_t code_at(const Ev<T>& E, int i, int j, int k) 
   return in ? E.code[(int64_t)i * E.sI + (int64_t)j * E.l + k] : kCA;
 }
 
+// Order-preserving unsigned bits of a float (NaN aside).
+__device__ __forceinline__ unsigned key_bits(float x) {
+  const unsigned b = __float_as_uint(x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
 template <typename T>
-__device__ void phase_counts(const Ev<T>& E, uint32_t* maps, int nw, int w) {
+__device__ void phase_counts(const Ev<T>& E, Sched& S, int nw, int w) {
   constexpr int kPT = 4;
   const int lane = lane_id();
   const int a = lane >> 3, b = lane & 7;
   for (int t0 = w * kPT; t0 < E.ntiles; t0 += nw * kPT) {
     uint8_t col[kPT][8], hal[kPT][6], inn[kPT][4];
+    float cu[kPT];
 #pragma unroll
     for (int x = 0; x < kPT; ++x) {
       const int t = t0 + x < E.ntiles ? t0 + x : t0;
       const int i0 = t / (E.ntj * E.ntk) * kT, j0 = (t / E.ntk) % E.ntj * kT, k0 = t % E.ntk * kT;
 #pragma unroll
       for (int s = 0; s < 8; ++s) col[x][s] = code_at(E, i0 + s, j0 + a, k0 + b);  // (s, a, b)
+      // ordering key: lanes 0-7 read the tile's corners (clamped into the grid)
+      {
+        const int ci = lane & 4 ? (i0 + kT - 1 < E.m ? i0 + kT - 1 : E.m - 1) : i0;
+        const int cj = lane & 2 ? (j0 + kT - 1 < E.n ? j0 + kT - 1 : E.n - 1) : j0;
+        const int ck = lane & 1 ? (k0 + kT - 1 < E.l ? k0 + kT - 1 : E.l - 1) : k0;
+        cu[x] = lane < 8 ? (float)E.u[(int64_t)ci * E.sI + (int64_t)cj * E.l + ck] : -3.4e38f;
+      }
       hal[x][0] = code_at(E, i0 - 1, j0 + a, k0 + b);   // i faces, position (a, b)
       hal[x][1] = code_at(E, i0 + kT, j0 + a, k0 + b);
       hal[x][2] = code_at(E, i0 + a, j0 - 1, k0 + b);   // j faces, position (a, b)
@@ -285,17 +312,83 @@ __device__ void phase_counts(const Ev<T>& E, uint32_t* maps, int nw, int w) {
         ext += (!(tn[q] & (kCZ | kCA)) && feeds(hal[x][q], q)) ? 1 : 0;
       nz = wave_sum(nz);
       ext = wave_sum(ext);
+      float kmax = cu[x];
+#pragma unroll
+      for (int off = 4; off > 0; off >>= 1) {
+        const float o = __shfl_xor(kmax, off);
+        kmax = o > kmax ? o : kmax;
+      }
       if (lane == 0 && t < E.ntiles) {
-        E.ext[t] = ext;
-        const uint32_t bit = 1u << (t & 31);
-        uint32_t* wd = maps + (t >> 5);
-        if (nz == 0) {
-          atomicOr(wd + kDone * E.nwords, bit);
-        } else {
-          atomicOr(wd + kDirty * E.nwords, bit);
-          if (ext == 0) atomicOr(wd + kReady * E.nwords, bit);
+        E.ext[t] = nz == 0 ? -1 : ext;
+        E.key[t] = kmax;
+        if (nz) {
+          atomicMin(&S.kmin, key_bits(kmax));
+          atomicMax(&S.kmax, key_bits(kmax));
         }
       }
+    }
+  }
+}
+
+// Counting-sort bin of a tile: tiles with non-Z nodes by decreasing key
+// over [kmin, kmax] in kBins bins, tiles without in bin kBins.
+__device__ __forceinline__ int tile_bin(float key, int ext, float lo, float hi) {
+  if (ext < 0) return kBins;
+  const float span = hi - lo;
+  const float x = span > 0.f ? (hi - key) / span * (float)(kBins - 1) : 0.f;
+  return x >= 0.f ? (x < (float)(kBins - 1) ? (int)x : kBins - 1) : 0;  // NaN -> 0
+}
+
+// Rank of every tile in the visiting order (rank2tile / tile2rank) and the
+// initial bitmaps at those ranks: done for tiles without non-Z nodes, dirty
+// for the others, ready for those without external feeders.  hist / cur:
+// kBins + 1 counters of LDS scratch each.
+template <typename T>
+__device__ void phase_rank(const Ev<T>& E, const Sched& S, uint32_t* maps, unsigned* hist,
+                           unsigned* cur) {
+  const float lo = key_float(S.kmin), hi = key_float(S.kmax);
+  for (int i = threadIdx.x; i <= kBins; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < E.ntiles; t += blockDim.x)
+    atomicAdd(&hist[tile_bin(E.key[t], E.ext[t], lo, hi)], 1u);
+  __syncthreads();
+  if (threadIdx.x < 64) {  // exclusive scan by one wave, 17 bins per lane
+    constexpr int kPer = (kBins + 1 + 63) / 64;
+    const int lane = lane_id();
+    unsigned v[kPer], sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int i = lane * kPer + q;
+      v[q] = i <= kBins ? hist[i] : 0u;
+      sum += v[q];
+    }
+    unsigned incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned o = __shfl_up(incl, off);
+      incl += lane >= off ? o : 0u;
+    }
+    unsigned run = incl - sum;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int i = lane * kPer + q;
+      if (i <= kBins) cur[i] = run;
+      run += v[q];
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < E.ntiles; t += blockDim.x) {
+    const int ext = E.ext[t];
+    const int r = (int)atomicAdd(&cur[tile_bin(E.key[t], ext, lo, hi)], 1u);
+    E.t2r[t] = r;
+    E.r2t[r] = t;
+    const uint32_t bit = 1u << (r & 31);
+    uint32_t* wd = maps + (r >> 5);
+    if (ext < 0) {
+      atomicOr(wd + kDone * E.nwords, bit);
+    } else {
+      atomicOr(wd + kDirty * E.nwords, bit);
+      if (ext == 0) atomicOr(wd + kReady * E.nwords, bit);
     }
   }
 }
@@ -510,35 +603,33 @@ __device__ __forceinline__ int tile_nb(const Ev<T>& E, int t, int q) {
   return (c[0] * E.ntj + c[1]) * E.ntk + c[2];
 }
 
-// Takes a ready tile that no wave is visiting: clears its ready and dirty
-// bits, marks it busy and counts the wave as active.  Returns -1 if none was
+// Takes the tile of lowest rank marked in map `which` (kReady or kDirty)
+// that no wave is visiting and that is not done: clears that bit (and the
+// dirty bit: the visit sees every delivery made so far), marks the tile
+// busy and counts the wave as active.  Returns its rank, or -1 if none was
 // found (wave-uniform).
-__device__ __forceinline__ int pop_tile(uint32_t* maps, int nwords, Sched& S) {
+__device__ __forceinline__ int pop_tile(uint32_t* maps, int nwords, Sched& S, int which) {
   const int lane = lane_id();
-  const int start = S.cursor;
   for (int base = 0; base < nwords; base += 64) {
-    int wi = start + base + lane;
-    wi = wi >= nwords ? wi - nwords : wi;
-    wi = wi >= nwords ? wi - nwords : wi;
+    const int wi = base + lane;
     uint32_t cand = 0;
-    if (base + lane < nwords)
-      cand = maps[kReady * nwords + wi] & ~maps[kBusy * nwords + wi] & ~maps[kDone * nwords + wi];
+    if (wi < nwords)
+      cand = maps[which * nwords + wi] & ~maps[kBusy * nwords + wi] & ~maps[kDone * nwords + wi];
     const uint64_t bal = __ballot(cand != 0);
     if (!bal) continue;
     const int src = __builtin_ctzll(bal);
     int t = -1;
     if (lane == src) {
       const uint32_t bit = cand & (0u - cand);
-      const uint32_t old = atomicAnd(&maps[kReady * nwords + wi], ~bit);
+      const uint32_t old = atomicAnd(&maps[which * nwords + wi], ~bit);
       if (old & bit) {
         const uint32_t ob = atomicOr(&maps[kBusy * nwords + wi], bit);
         if (ob & bit) {
-          atomicOr(&maps[kReady * nwords + wi], bit);  // being visited: leave it ready
+          atomicOr(&maps[which * nwords + wi], bit);  // being visited: leave the mark
         } else {
           atomicAdd(&S.active, 1);
-          atomicAnd(&maps[kDirty * nwords + wi], ~bit);
+          if (which != kDirty) atomicAnd(&maps[kDirty * nwords + wi], ~bit);
           t = wi * 32 + __builtin_ctz(bit);
-          S.cursor = wi;
         }
       }
     }
@@ -578,6 +669,9 @@ __device__ void adj3_tile_event(const EventDesc& ev) {
   E.lam = (gptr<T>)ev.lam;
   E.code = (gptr<uint8_t>)ev.code;
   E.ext = (gptr<int>)ev.queue;
+  E.t2r = E.ext + E.ntiles;
+  E.r2t = E.t2r + E.ntiles;
+  E.key = (gptr<float>)(E.r2t + E.ntiles);
   const int w = threadIdx.x >> 6;
   const int lane = lane_id();
   if (E.ntiles > MAXT) {  // the host routes such events elsewhere
@@ -588,17 +682,19 @@ __device__ void adj3_tile_event(const EventDesc& ev) {
   if (threadIdx.x == 0) {
     S.remaining = 0;
     S.active = 0;
-    S.lock = 0;
     S.fault = 0;
     S.visits = 0;
-    S.rounds = 0;
-    S.cursor = 0;
+    S.kmin = 0xffffffffu;
+    S.kmax = 0u;
   }
   __syncthreads();
   const int nz = phase_codes<T, 4>(E);
   atomicAdd(&S.remaining, nz);
   __syncthreads();
-  phase_counts<T>(E, maps, NW, w);
+  phase_counts<T>(E, S, NW, w);
+  __syncthreads();
+  // the waves' boxes are free until the traversal: counting-sort scratch
+  phase_rank<T>(E, S, maps, (unsigned*)wb[0].u, (unsigned*)wb[0].lam);
   __syncthreads();
   // --- tile visits until every non-Z node is final
   int idle = 0;
@@ -614,11 +710,13 @@ __device__ void adj3_tile_event(const EventDesc& ev) {
       if (lane == 0) atomicExch(&S.fault, 1);
       break;
     }
-    const int t = pop_tile(maps, E.nwords, S);
-    if (t >= 0) {
+    int r = pop_tile(maps, E.nwords, S, kReady);
+    if (r < 0) r = pop_tile(maps, E.nwords, S, kDirty);
+    if (r >= 0) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      const uint32_t bit = 1u << (t & 31);
-      const bool revisit = maps[kVisited * E.nwords + (t >> 5)] & bit;
+      const int t = E.r2t[r];
+      const uint32_t bit = 1u << (r & 31);
+      const bool revisit = maps[kVisited * E.nwords + (r >> 5)] & bit;
       int left = 0, face[6];
       const int nres = visit<T>(E, wb[w], t, revisit, left, face);
       // the visit's stores before the bits and counts that publish them
@@ -628,15 +726,16 @@ __device__ void adj3_tile_event(const EventDesc& ev) {
         if (t2 >= 0) {
           const int old = __hip_atomic_fetch_sub(&E.ext[t2], face[lane], __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-          const uint32_t b2 = 1u << (t2 & 31);
-          atomicOr(&maps[kDirty * E.nwords + (t2 >> 5)], b2);
-          if (old == face[lane]) atomicOr(&maps[kReady * E.nwords + (t2 >> 5)], b2);
+          const int r2 = E.t2r[t2];
+          const uint32_t b2 = 1u << (r2 & 31);
+          atomicOr(&maps[kDirty * E.nwords + (r2 >> 5)], b2);
+          if (old == face[lane]) atomicOr(&maps[kReady * E.nwords + (r2 >> 5)], b2);
         }
       }
       if (lane == 0) {
-        atomicOr(&maps[kVisited * E.nwords + (t >> 5)], bit);
-        if (left == 0) atomicOr(&maps[kDone * E.nwords + (t >> 5)], bit);
-        atomicAnd(&maps[kBusy * E.nwords + (t >> 5)], ~bit);
+        atomicOr(&maps[kVisited * E.nwords + (r >> 5)], bit);
+        if (left == 0) atomicOr(&maps[kDone * E.nwords + (r >> 5)], bit);
+        atomicAnd(&maps[kBusy * E.nwords + (r >> 5)], ~bit);
         atomicAdd(&S.visits, 1);
         if (nres) atomicSub(&S.remaining, nres);
         atomicSub(&S.active, 1);
@@ -645,24 +744,7 @@ __device__ void adj3_tile_event(const EventDesc& ev) {
       idle = 0;
       continue;
     }
-    // nothing ready: when no wave holds a tile, every dirty tile is visited
-    // as it is (some node is always resolvable; see the file comment)
-    if (__hip_atomic_load(&S.active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
-      int got = 0;
-      if (lane == 0) got = atomicCAS(&S.lock, 0, 1) == 0;
-      got = __shfl(got, 0);
-      if (got) {
-        if (__hip_atomic_load(&S.active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
-          for (int i = lane; i < E.nwords; i += 64) {
-            const uint32_t d = maps[kDirty * E.nwords + i] & ~maps[kDone * E.nwords + i];
-            if (d) atomicOr(&maps[kReady * E.nwords + i], d);
-          }
-          if (lane == 0) atomicAdd(&S.rounds, 1);
-        }
-        wave_sync();
-        if (lane == 0) atomicExch(&S.lock, 0);
-      }
-    }
+    // nothing ready or dirty: other waves hold the work; poll
     if (rem != last_rem) {
       last_rem = rem;
       idle = 0;

@@ -40,15 +40,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "adtomo.jl_amd")]
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes (read + write) per launch of each kernel on this workload (256
-# events), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy
-# kernel of known size: profiles/pmc_traffic_fwd3_diag_adj3_kahn_256x128x128x64.md
 FWD_KERNEL = "fwd3_diag_kernel<double, false>"
 ADJ_KERNEL = "adj3_kahn_kernel<double>"
 ADJ_TILE_KERNEL = "adj3_tile_kernel<double, 8, 8192>"
 FUSED_KERNEL = "fwd_adj3_diag_kernel<double, false>"
-TRAFFIC_BYTES_256 = {FWD_KERNEL: 117.9e9 + 57.2e9, ADJ_KERNEL: 129.4e9 + 49.3e9,
-                     FUSED_KERNEL: 243.9e9 + 105.0e9}
+# HBM bytes (read + write) per launch of each kernel on this workload (256
+# events, default setting), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# scaled by a copy kernel of known size and checked against the TCC EA
+# request counters; kernels not profiled at the current code report null
+TRAFFIC_SOURCE = "profiles/pmc_traffic_fwd3_diag_adj3_tile_256x128x128x64.md"
+TRAFFIC_BYTES_256 = {FWD_KERNEL: 97.98e9 + 58.45e9, ADJ_TILE_KERNEL: 47.04e9 + 9.04e9}
 
 
 def log(*a):
@@ -309,7 +310,8 @@ def main():
     def roof(kernel, gbs, kms):
         return {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "launch_ms": round(kms, 3),
-                "traffic": TRAFFIC_BYTES_256.get(kernel) if nev == 256 else None}
+                "traffic": TRAFFIC_BYTES_256.get(kernel) if nev == 256 else None,
+                "traffic_source": TRAFFIC_SOURCE if nev == 256 and kernel in TRAFFIC_BYTES_256 else None}
 
     adj_kernel = ADJ_TILE_KERNEL if tune.get("adj_algo", 1) == 0 else ADJ_KERNEL
     roofs = [roof(FWD_KERNEL, fwd_gbs, fwd_ms), roof(adj_kernel, adj_gbs, adj_ms)]

@@ -39,6 +39,24 @@
 // instructions are never skipped: masked lanes use an out-of-range buffer
 // offset.
 //
+// Chained sweeps.  A sweep's first H-1 diagonals leave lanes p > v idle and
+// its last H-1 leave lanes p <= v - bands*N_b idle, exactly complementary:
+// when the next sweep keeps the c direction (it flips the a or the b axis),
+// sweep k+1's diagonal v runs in the same step as sweep k's diagonal
+// v + bands*N_b, lanes p <= v on sweep k+1 and the others on sweep k.
+// Every global access of wave w touches k-segment w only, so the only
+// ordering needed is the wave's own program order: a column's old values
+// for sweep k+1 must have been stored by sweep k at least two steps before
+// they are loaded, and sweep k's last columns must be read before sweep k+1
+// stores over their neighbours.  With diagonal indices both hold when the
+// next sweep flips a and N_b >= H + 3 (its first columns are sweep k's
+// columns of diagonals <= (bands-1) N_b + H), or flips b and bands >= 2
+// (they are sweep k's first band).  In-plane neighbours across the lane
+// boundary of the two sweeps are grid edges, whose one-sided minimum never
+// reads them.  Chains stop at the last sweep of an iteration (it takes the
+// stop rule), at the first sweep of the first iteration (it reads u0), at
+// a flip of c, and for events swept in several k chunks.
+//
 // Iteration (Eikonal3D.cpp:71-88): u = u0 is a copy pass at the start;
 // err = max|u - u_old| is a coalesced pass after the eighth sweep against
 // the snapshot (u0 for the first iteration), which also writes the next
@@ -84,6 +102,7 @@ __device__ inline DGeom make_dgeom(int m, int n, int l) {
 struct DCol {
   int a, b;
   bool valid;
+  bool fa, fb;    // the column's sweep flips a / b
   int64_t node0;  // flat index of canonical (a, b, c = 0)
 };
 
@@ -100,11 +119,20 @@ __device__ __forceinline__ DCol dcol(const DGeom& g, int v, int p, int64_t base0
   return c;
 }
 
+constexpr int kChainMax = 4;  // sweeps of one chain (the c direction fixed)
+
 template <typename T>
 struct DSweep {
   DGeom g;
-  int64_t tA, tB, tC, base0;
-  bool flipA, flipB, flipC;
+  int64_t tC;
+  bool flipC;
+  // the chained sweeps (1..kChainMax): bits 2q, 2q+1 of cfl = sweep q
+  // flips a, b; a sweep's canonical (a, b, c = 0) lies at
+  // (fa ? aoff : 0) + (fb ? boff : 0) + coff with strides -+sA, -+sB
+  int nch;
+  int V0;  // bands * N_b: diagonal offset between consecutive chained sweeps
+  int cfl;
+  int64_t aoff, boff, coff;
   T h;
   double hd;    // h for the FP64 local solve of the FP32-storage variant
   int cs;       // canonical c of this pass's first node (k chunks)
@@ -118,6 +146,29 @@ struct DSweep {
   rsrc_t snap_src;  // last sweep of an iteration: snapshot at its start
   rsrc_t snap_dst;  // ... and the snapshot for the next iteration
 };
+
+// Column of lane p on chain diagonal t, whose chained sweep kk (uniform,
+// kk = min(t / V0, nch - 1)) owns local diagonal v = t - kk V0: lanes p > v
+// of a sweep's first diagonals take the previous sweep's diagonal v + V0.
+// Stride of the a axis in chained sweep k (uniform).
+template <typename T>
+__device__ __forceinline__ int64_t chain_tA(const DSweep<T>& S, int k) {
+  return ((S.cfl >> (2 * k)) & 1) ? -S.g.sA : S.g.sA;
+}
+
+template <typename T>
+__device__ __forceinline__ DCol chain_col(const DSweep<T>& S, int t, int kk, int p) {
+  const int v = t - kk * S.V0;
+  const bool prev = kk > 0 && p > v;
+  const int fl = (S.cfl >> (2 * (prev ? kk - 1 : kk))) & 3;
+  const bool fa = fl & 1, fb = fl & 2;
+  const int64_t base = (fa ? S.aoff : 0) + (fb ? S.boff : 0) + S.coff;
+  DCol c = dcol(S.g, prev ? v + S.V0 : v, p, base, fa ? -S.g.sA : S.g.sA,
+                fb ? -S.g.sB : S.g.sB);
+  c.fa = fa;
+  c.fb = fb;
+  return c;
+}
 
 // cnt values of column `col` from canonical c0 on (element j <- c0 + j),
 // one load per element.
@@ -495,12 +546,16 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   };
   // Band-edge neighbour of lane 0 ((a-1, b), new) and lane H-1 ((a+1, b),
   // old) of column `col`, one load per node (canonical order).
-  auto load_edge = [&](T(&buf)[kLW], const DCol& col) {
+  // (lanes 0 and H-1 always take the diagonal's own sweep kk: a lane in
+  // the previous sweep's tail lies in its last band, where lane H-1 is the
+  // grid edge and lane 0 never is)
+  auto load_edge = [&](T(&buf)[kLW], const DCol& col, int kk) {
     const bool eu = lane_first && col.valid && col.a > 0;
     const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
     T bo[kLW];
-    load_seg(buf, S.u, col, -S.tA, eu, c0, cnt, L, S.tC);
-    load_seg(bo, S.src, col, S.tA, ed, c0, cnt, L, S.tC);
+    const int64_t tA = chain_tA(S, kk);
+    load_seg(buf, S.u, col, -tA, eu, c0, cnt, L, S.tC);
+    load_seg(bo, S.src, col, tA, ed, c0, cnt, L, S.tC);
 #pragma unroll
     for (int j = 0; j < kLW; ++j) buf[j] = eu ? buf[j] : bo[j];
   };
@@ -531,42 +586,50 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   };
   T XCM, XCP;  // values for the current diagonal, landed at step start
   // band-edge segment of lane 0 / lane H-1 for column `col` (full variant)
-  auto eseg_of = [&](const DCol& col) -> int {
+  auto eseg_of = [&](const DCol& col, int kk) -> int {
     const bool eu = lane_first && col.valid && col.a > 0;
     const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
-    const int64_t d = eu ? -S.tA : S.tA;
+    const int64_t tA = chain_tA(S, kk);
+    const int64_t d = eu ? -tA : tA;
     const int64_t lo = fwd ? col.node0 + d + c0 : col.node0 + d - c0 - (kLW - 1);
     return (eu || ed) ? (int)lo : -1;
   };
-  auto issue_edge = [&](const DCol& col) {
+  auto issue_edge = [&](const DCol& col, int kk) {
     if constexpr (FULL)
-      EF = edge_issue<T>(S.u, S.src, eseg_of(col), g.H - 1, p);
+      EF = edge_issue<T>(S.u, S.src, eseg_of(col, kk), g.H - 1, p);
     else
-      load_edge(LE, col);
+      load_edge(LE, col, kk);
   };
 
+  // chained sweep of chain diagonal t (t < 0: the first)
+  auto chain_of = [&](int t) -> int {
+    const int k = t > 0 ? t / S.V0 : 0;
+    return k < S.nch - 1 ? k : S.nch - 1;
+  };
   const int v0 = -w;  // diagonal of step 0
-  DCol cv = dcol(g, v0, p, S.base0, S.tA, S.tB);
-  DCol cv1 = dcol(g, v0 + 1, p, S.base0, S.tA, S.tB);
-  DCol cv2 = dcol(g, v0 + 2, p, S.base0, S.tA, S.tB);
+  int kk0 = chain_of(v0), kk1 = chain_of(v0 + 1), kk2 = chain_of(v0 + 2);
+  DCol cv = chain_col(S, v0, kk0, p);
+  DCol cv1 = chain_col(S, v0 + 1, kk1, p);
+  DCol cv2 = chain_col(S, v0 + 2, kk2, p);
   // prologue: plane x = old(v0); in flight: old(v0+1), f(v0), edges(v0);
   // node 0 of old(v0+1) published for the wave upstream (its step 0)
   seg_issue<T, FULL>(FD, S.src, seg_of(cv), cnt, p);
   seg_land<T, FULL>(FD, px, cnt, fwd, p);
   seg_issue<T, FULL>(FD, S.src, seg_of(cv1), cnt, p);
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
-  issue_edge(cv);
+  issue_edge(cv, kk0);
   T XCM_n = load_cm(cv), XCP_n = load_cp(cv);
   if constexpr (ERR) seg_issue<T, FULL>(FS, S.snap_src, -1, cnt, p);
   seg_publish<T, FULL>(FD, xold + kDW * 64 + w * 64, cnt, fwd, p);
   if (p == 0) __hip_atomic_store(&prog[w], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   __syncthreads();
 
-  const int steps = g.V + S.nseg - 1;
+  const int steps = (S.nch - 1) * S.V0 + g.V + S.nseg - 1;
+  const int vend = (S.nch - 1) * S.V0 + g.V;  // chain diagonals
   int seg_prev = -1;  // segment of the previous step's new values (-1: none)
   for (int t = 0; t < steps; ++t) {
     const int v = v0 + t;
-    const bool active = v >= 0 && v < g.V;
+    const bool active = v >= 0 && v < vend;
     // new values of v-1 (plane n) to memory
     if constexpr (ERR) {
       seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, cnt, fwd, p, FS, key);
@@ -609,8 +672,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : XCM;
       const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : XCP;
-      const int ra = S.flipA ? g.Ma - 1 - cv.a : cv.a;
-      const int rb = S.flipB ? g.Nb - 1 - cv.b : cv.b;
+      const int ra = cv.fa ? g.Ma - 1 - cv.a : cv.a;
+      const int rb = cv.fb ? g.Nb - 1 - cv.b : cv.b;
       const bool af = ra == 0, al = ra == g.Ma - 1;
       const bool bf = rb == 0, bl = rb == g.Nb - 1;
       // The a and b neighbours are read straight from the planes: the sweep
@@ -619,8 +682,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       // b-1 (new) / b+1 (old) at position p+1 of planes n / x.
       const lptr<T> a_up = pn + p, a_dn = px + p + 2;
       const lptr<T> b_up = pn + p + 1, b_dn = px + p + 1;
-      const lptr<T> a_plus = S.flipA ? a_up : a_dn, a_minus = S.flipA ? a_dn : a_up;
-      const lptr<T> b_plus = S.flipB ? b_up : b_dn, b_minus = S.flipB ? b_dn : b_up;
+      const lptr<T> a_plus = cv.fa ? a_up : a_dn, a_minus = cv.fa ? a_dn : a_up;
+      const lptr<T> b_plus = cv.fb ? b_up : b_dn, b_minus = cv.fb ? b_dn : b_up;
       const lptr<T> x_plus0 = g.a_is_i ? a_plus : b_plus, x_minus0 = g.a_is_i ? a_minus : b_minus;
       const lptr<T> y_plus0 = g.a_is_i ? b_plus : a_plus, y_minus0 = g.a_is_i ? b_minus : a_minus;
       const bool xf = g.a_is_i ? af : bf, xl = g.a_is_i ? al : bl;
@@ -635,7 +698,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       for (int e = 0; e < kLW; ++e) {
         if (e == 0) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
         if (e == 1) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
-        if (e == 2) issue_edge(cv1);
+        if (e == 2) issue_edge(cv1, kk1);
         if (e == 2) {
           XCM_n = load_cm(cv1);
           XCP_n = load_cp(cv1);
@@ -676,7 +739,10 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     xch[(t & 1) * kDW * 64 + w * 64 + p] = xo;
     cv = cv1;
     cv1 = cv2;
-    cv2 = dcol(g, v + 3, p, S.base0, S.tA, S.tB);
+    kk0 = kk1;
+    kk1 = kk2;
+    if (kk2 < S.nch - 1 && v + 3 >= (kk2 + 1) * S.V0) ++kk2;  // chain_of(v + 3)
+    cv2 = chain_col(S, v + 3, kk2, p);
     if (p == 0) publish_progress(prog, w, t + 1);
   }
   if constexpr (ERR)
@@ -749,17 +815,24 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   const T tol = (T)ev.tol;
   int it = 0;
   T err = T(0);
+  S.V0 = S.g.bands * S.g.Nb;
+  // may sweep sw+1 run chained after sweep sw (see "Chained sweeps")?
+  auto chains = [&](int it_, int sw) -> bool {
+    if (sw + 1 >= 7 || (it_ == 0 && sw == 0) || nchunk != 1) return false;
+    const int flip = kDir3[sw] ^ kDir3[sw + 1];
+    if (flip & 4) return false;
+    const bool flips_a = (flip & 1) ? S.g.a_is_i : !S.g.a_is_i;
+    return flips_a ? S.g.Nb >= S.g.H + 3 : S.g.bands >= 2;
+  };
   for (it = 0; it < max_iter; ++it) {
     uint64_t key = 0;
-    for (int sw = 0; sw < 8; ++sw) {
-      const int dir = kDir3[sw];
-      const bool fi = dir & 1, fj = dir & 2, fk = dir & 4;
-      S.flipA = S.g.a_is_i ? fi : fj;
-      S.flipB = S.g.a_is_i ? fj : fi;
+    for (int sw = 0; sw < 8;) {
+      int nch = 1;
+      while (nch < kChainMax && chains(it, sw + nch - 1)) ++nch;
+      const int fk = kDir3[sw] & 4;
       S.flipC = fk;
-      S.tA = S.flipA ? -S.g.sA : S.g.sA;
-      S.tB = S.flipB ? -S.g.sB : S.g.sB;
       S.tC = S.flipC ? -1 : 1;
+      S.nch = nch;
       // u = u0 (Eikonal3D.cpp:72): the first sweep reads its old values
       // from u0; every node is written once per sweep
       S.src = (it == 0 && sw == 0) ? S.u0 : S.u;
@@ -769,9 +842,16 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
         const int Lc = l - S.cs < kChunk ? l - S.cs : kChunk;
         S.g.L = Lc;
         S.g.Lw = (Lc + S.nseg - 1) / S.nseg;
-        S.base0 = (S.flipA ? (int64_t)(S.g.Ma - 1) * S.g.sA : 0) +
-                  (S.flipB ? (int64_t)(S.g.Nb - 1) * S.g.sB : 0) +
-                  (S.flipC ? (int64_t)(l - 1 - S.cs) : (int64_t)S.cs);
+        S.cfl = 0;
+        for (int q = 0; q < nch; ++q) {
+          const int dir = kDir3[sw + q];
+          const bool fi = dir & 1, fj = dir & 2;
+          const bool fa = S.g.a_is_i ? fi : fj, fb = S.g.a_is_i ? fj : fi;
+          S.cfl |= ((fa ? 1 : 0) | (fb ? 2 : 0)) << (2 * q);
+        }
+        S.aoff = (int64_t)(S.g.Ma - 1) * S.g.sA;
+        S.boff = (int64_t)(S.g.Nb - 1) * S.g.sB;
+        S.coff = S.flipC ? (int64_t)(l - 1 - S.cs) : (int64_t)S.cs;
         if (sw < 7) {
           diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes, prog,
                                     &abort_flag);
@@ -785,6 +865,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
           key = kc > key ? kc : key;
         }
       }
+      sw += nch;
     }
     for (int off = 32; off > 0; off >>= 1) {
       uint64_t o = __shfl_xor(key, off);

@@ -54,8 +54,11 @@ constexpr int kTN = kT * kT * kT;      // nodes per tile
 constexpr int kB = kT + 2;             // box edge with a one-node halo
 constexpr int kBox = kB * kB * kB;     // box entries
 constexpr int kBI = kB * kB, kBJ = kB;  // box strides (k stride 1)
-constexpr uint8_t kCZ = 0x40;          // code flag: Z (lambda = 0, no upwind)
-constexpr uint8_t kCA = 0x80;          // code flag (LDS only): outside the grid
+constexpr uint8_t kCZ = 0x40;          // code flag: Z (lambda = 0, no upwind); in
+                                       // LDS also nodes outside the grid, which
+                                       // like Z nodes feed and release nothing
+constexpr uint8_t kCF = 0x80;          // code flag: lambda final (set by the visit
+                                       // that resolves the node)
 // Per-node record of a visit (16 bits): bits 0-3 the number of feeders not
 // yet final, 4-9 the feeder mask (neighbour order i-1, i+1, j-1, j+1, k-1,
 // k+1), 10-12 which upwind neighbours (x, y, z) are present and not Z (they
@@ -64,22 +67,6 @@ constexpr uint8_t kCA = 0x80;          // code flag (LDS only): outside the grid
 constexpr uint32_t kPCount = 0xFu;
 constexpr int kPFeedShift = 4, kPRelShift = 10;
 constexpr uint32_t kPNA = 1u << 14, kPDone = 1u << 15;
-
-// Multiplier of a node that is not final yet: a signalling-NaN pattern that
-// arithmetic never produces (computed NaNs are quiet).
-template <typename T> struct Sent;
-template <> struct Sent<double> {
-  using U = unsigned long long;
-  static constexpr U bits = 0x7FF4DEADBEEF0001ull;
-  static __device__ __forceinline__ double value() { return __longlong_as_double((long long)bits); }
-  static __device__ __forceinline__ bool is(double x) { return (U)__double_as_longlong(x) == bits; }
-};
-template <> struct Sent<float> {
-  using U = unsigned int;
-  static constexpr U bits = 0x7FA0BEEFu;
-  static __device__ __forceinline__ float value() { return __uint_as_float(bits); }
-  static __device__ __forceinline__ bool is(float x) { return __float_as_uint(x) == bits; }
-};
 
 // LDS of one wave: the tile and its halo (u, multipliers, codes), the
 // pending-feeder count of each tile node and the traversal queue.
@@ -163,7 +150,7 @@ __device__ __forceinline__ int box_nb(int b, int q) {
 // to r (a neighbour at -1 has its upwind at +1, and the reverse).
 __device__ __forceinline__ bool feeds(uint8_t cn, int q) {
   const int d = q >> 1;
-  return !(cn & (kCZ | kCA)) && (((cn >> (2 * d)) & 3u) == ((q & 1) ? 1u : 2u));
+  return !(cn & kCZ) && (((cn >> (2 * d)) & 3u) == ((q & 1) ? 1u : 2u));
 }
 
 // Wave-level ordering of LDS accesses by different lanes (the wave's LDS
@@ -195,11 +182,11 @@ __device__ __forceinline__ unsigned pend_dec(uint16_t* pend, int id, unsigned by
 // ---------------------------------------------------------------- phases ---
 
 // Phase 0 (whole workgroup, grid-stride over the event's nodes): upwind
-// codes (the reference's Z rule, Eikonal3D.cpp:105-171), grad_u0 = g where
-// u == u0 (:105-110), multipliers set to 0 on Z and to the "not final"
-// pattern elsewhere, and dL/df = -0 * (-2 f h h) on Z (the value the
-// reference's final loop gives those nodes, :194-196).  Returns this
-// thread's count of non-Z nodes.
+// codes (the reference's Z rule, Eikonal3D.cpp:105-171; no node final yet),
+// grad_u0 = g where u == u0 (:105-110; g is read only there), and
+// dL/df = -0 * (-2 f h h) on Z (the value the reference's final loop gives
+// those nodes, :194-196).  Z multipliers are 0 and never read: a Z node
+// feeds nothing.  Returns this thread's count of non-Z nodes.
 template <typename T, int kPrep>
 __device__ int phase_codes(const Ev<T>& E) {
   const int m = E.m, n = E.n, l = E.l;
@@ -213,13 +200,12 @@ __device__ int phase_codes(const Ev<T>& E) {
   for (int k = 0; k < kPrep; ++k) wk[k] = Walk(threadIdx.x + k * blockDim.x, stride, m, n, l);
   for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
     const PrepGroup<kPrep> G = prep_group<kPrep>(p0, N, wk);
-    T uc[kPrep], u0c[kPrep], gc[kPrep], nb[kPrep][6];
+    T uc[kPrep], u0c[kPrep], nb[kPrep][6];
 #pragma unroll
     for (int k = 0; k < kPrep; ++k) {
       const int64_t q = G.q[k];
       uc[k] = E.u[q];
       u0c[k] = E.u0[q];
-      gc[k] = E.g[q];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {  // clamped into the grid; edges use one side
         nb[k][2 * d] = E.u[G.co[k][d] > 0 ? q - st[d] : q + st[d]];
@@ -233,8 +219,7 @@ __device__ int phase_codes(const Ev<T>& E) {
         const int64_t q = G.q[k];
         const bool z = code & kZ;
         E.code[q] = (uint8_t)code;
-        if (E.gu0) E.gu0[q] = (uc[k] == u0c[k]) ? gc[k] : T(0);
-        E.lam[q] = z ? T(0) : Sent<T>::value();
+        if (E.gu0) E.gu0[q] = (uc[k] == u0c[k]) ? E.g[q] : T(0);
         if (z) {
           const T rhs = T(-2.0) * E.f[q] * E.h * E.h;
           E.gf[q] = -T(0) * rhs;
@@ -256,7 +241,7 @@ __device__ int phase_codes(const Ev<T>& E) {
 template <typename T>
 __device__ __forceinline__ uint8_t code_at(const Ev<T>& E, int i, int j, int k) {
   const bool in = i >= 0 && i < E.m && j >= 0 && j < E.n && k >= 0 && k < E.l;
-  return in ? E.code[(int64_t)i * E.sI + (int64_t)j * E.l + k] : kCA;
+  return in ? E.code[(int64_t)i * E.sI + (int64_t)j * E.l + k] : kCZ;
 }
 
 // Order-preserving unsigned bits of a float (NaN aside).
@@ -305,11 +290,11 @@ __device__ void phase_counts(const Ev<T>& E, Sched& S, int nw, int w) {
       const int t = t0 + x;
       int nz = 0, ext = 0;
 #pragma unroll
-      for (int s = 0; s < 8; ++s) nz += (col[x][s] & (kCZ | kCA)) ? 0 : 1;
+      for (int s = 0; s < 8; ++s) nz += (col[x][s] & kCZ) ? 0 : 1;
       const uint8_t tn[6] = {col[x][0], col[x][7], inn[x][0], inn[x][1], inn[x][2], inn[x][3]};
 #pragma unroll
       for (int q = 0; q < 6; ++q)
-        ext += (!(tn[q] & (kCZ | kCA)) && feeds(hal[x][q], q)) ? 1 : 0;
+        ext += (!(tn[q] & kCZ) && feeds(hal[x][q], q)) ? 1 : 0;
       nz = wave_sum(nz);
       ext = wave_sum(ext);
       float kmax = cu[x];
@@ -419,8 +404,15 @@ __device__ int visit(const Ev<T>& E, WaveBuf<T>& B, int t, bool revisit, int& le
       bx[s] = slot_node(s, lane, i0, j0, k0, gi, gj, gk);
       in[s] = gi >= 0 && gi < E.m && gj >= 0 && gj < E.n && gk >= 0 && gk < E.l;
       gx[s] = in[s] ? (int64_t)gi * E.sI + (int64_t)gj * E.l + gk : 0;
-      c[s] = in[s] ? E.code[gx[s]] : kCA;
-      lm[s] = (in[s] && (s >= 8 || revisit)) ? E.lam[gx[s]] : Sent<T>::value();
+      c[s] = in[s] ? E.code[gx[s]] : kCZ;
+    }
+    // multipliers of final nodes only (of the tile itself on revisits),
+    // loaded after their flags
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+    for (int s = 0; s < 14; ++s) {
+      const bool need = (s >= 8 || revisit) && (c[s] & kCF);
+      lm[s] = need ? E.lam[gx[s]] : T(0);
     }
 #pragma unroll
     for (int s = 0; s < 14; ++s) {
@@ -437,26 +429,22 @@ __device__ int visit(const Ev<T>& E, WaveBuf<T>& B, int t, bool revisit, int& le
     const int bi = bx[s];
     const uint8_t c = B.code[bi];
     uint8_t cn[6];
-    T ln[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      cn[q] = B.code[box_nb(bi, q)];
-      ln[q] = B.lam[box_nb(bi, q)];
-    }
+    for (int q = 0; q < 6; ++q) cn[q] = B.code[box_nb(bi, q)];
     uint32_t rec = kPNA;
-    if (!(c & (kCZ | kCA)) && Sent<T>::is(B.lam[bi])) {
+    if (!(c & (kCZ | kCF))) {
       uint32_t cnt = 0, fm = 0, rel = 0;
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         const bool fd = feeds(cn[q], q);
         fm |= fd ? (1u << q) : 0u;
-        cnt += (fd && Sent<T>::is(ln[q])) ? 1u : 0u;
+        cnt += (fd && !(cn[q] & kCF)) ? 1u : 0u;
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const uint32_t sd = (c >> (2 * d)) & 3u;
         const uint8_t cx = cn[2 * d + (sd == 2 ? 1 : 0)];
-        rel |= (sd && !(cx & (kCZ | kCA))) ? (1u << d) : 0u;
+        rel |= (sd && !(cx & kCZ)) ? (1u << d) : 0u;
       }
       rec = cnt | (fm << kPFeedShift) | (rel << kPRelShift);
       todo |= 1u << s;
@@ -573,8 +561,12 @@ __device__ int visit(const Ev<T>& E, WaveBuf<T>& B, int t, bool revisit, int& le
     qtail = qtail < kTN ? qtail : kTN;  // each node enters once: never clamps
     wave_sync();
   }
-  // --- results of the nodes resolved in this visit (owners store coalesced)
+  // --- results of the nodes resolved in this visit (owners store coalesced):
+  // multipliers and dL/df, then the final flags.  Another wave may read a
+  // code byte at any time and reads the multiplier only after it saw the
+  // flag; within the workgroup the release keeps the flag behind the value.
   int nres = 0;
+  uint32_t fin = 0;
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     if (((todo >> s) & 1u) && (B.pend[s * 64 + lane] & kPDone)) {
@@ -582,9 +574,14 @@ __device__ int visit(const Ev<T>& E, WaveBuf<T>& B, int t, bool revisit, int& le
       E.lam[gx[s]] = (lr == lr) ? lr : T(__builtin_nan(""));
       const T rhs = T(-2.0) * fr[s] * E.h * E.h;  // Eikonal3D.cpp:113-116, 194-196
       E.gf[gx[s]] = -lr * rhs;
+      fin |= 1u << s;
       ++nres;
     }
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+    if ((fin >> s) & 1u) E.code[gx[s]] = (uint8_t)(B.code[bx[s]] | kCF);
 #pragma unroll
   for (int q = 0; q < 6; ++q) face[q] = wave_sum(fcnt[q]);
   nres = wave_sum(nres);

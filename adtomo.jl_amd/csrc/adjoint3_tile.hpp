@@ -233,15 +233,41 @@ __device__ int phase_codes(const Ev<T>& E) {
 
 // Phase 1 (waves over tiles, kPT tiles per iteration with their loads in
 // flight together): the count of edges that enter each tile from non-Z
-// nodes of other tiles (its external feeders, all unresolved at the start),
-// and the initial bitmaps: done for tiles without non-Z nodes, dirty for the
-// others, ready for those without external feeders.  Lane (a, b) checks, on
-// each face, the halo node of its face position against the tile node next
-// to it, and counts the non-Z nodes of its column (a, b) of the tile.
+// nodes of other tiles (its external feeders, all unresolved at the
+// start), -1 for tiles without non-Z nodes, and the tile's ordering key.
+// Lane (s, a) holds the tile's k row (s, a, 0..7), eight codes in one
+// 64-bit word, checks its faces against the halo rows (i faces: s = 0, 7;
+// j faces: a = 0, 7) and the two k-face halo nodes, and counts its row's
+// non-Z nodes; lanes 0, 7, 56, 63 read the tile's corner travel times.
 template <typename T>
 __device__ __forceinline__ uint8_t code_at(const Ev<T>& E, int i, int j, int k) {
   const bool in = i >= 0 && i < E.m && j >= 0 && j < E.n && k >= 0 && k < E.l;
   return in ? E.code[(int64_t)i * E.sI + (int64_t)j * E.l + k] : kCZ;
+}
+
+// The codes (i, j, k0 .. k0+7) as bytes of one word (outside the grid: Z).
+template <typename T>
+__device__ __forceinline__ uint64_t code_row(const Ev<T>& E, int i, int j, int k0) {
+  if (i < 0 || i >= E.m || j < 0 || j >= E.n) return 0x4040404040404040ull;
+  const int64_t q = (int64_t)i * E.sI + (int64_t)j * E.l + k0;
+  if (k0 + kT <= E.l && !(q & 7)) return *(gptr<const uint64_t>)(E.code + q);
+  uint64_t r = 0;
+#pragma unroll
+  for (int b = 0; b < kT; ++b)
+    r |= (uint64_t)(k0 + b < E.l ? E.code[q + b] : kCZ) << (8 * b);
+  return r;
+}
+
+// Edges from the halo row `hal` into the tile row `row` across a face of
+// direction q (the halo node lies at q of the tile node, byte by byte).
+__device__ __forceinline__ int face_edges(uint64_t row, uint64_t hal, int q) {
+  int n = 0;
+#pragma unroll
+  for (int b = 0; b < kT; ++b) {
+    const uint8_t tn = (uint8_t)(row >> (8 * b)), hn = (uint8_t)(hal >> (8 * b));
+    n += (!(tn & kCZ) && feeds(hn, q)) ? 1 : 0;
+  }
+  return n;
 }
 
 // Order-preserving unsigned bits of a float (NaN aside).
@@ -257,49 +283,51 @@ template <typename T>
 __device__ void phase_counts(const Ev<T>& E, Sched& S, int nw, int w) {
   constexpr int kPT = 4;
   const int lane = lane_id();
-  const int a = lane >> 3, b = lane & 7;
+  const int si = lane >> 3, a = lane & 7;
   for (int t0 = w * kPT; t0 < E.ntiles; t0 += nw * kPT) {
-    uint8_t col[kPT][8], hal[kPT][6], inn[kPT][4];
-    float cu[kPT];
+    uint64_t row[kPT], hi[kPT], hj[kPT];
+    uint8_t hk[kPT][2];
+    float cu[kPT][2];
 #pragma unroll
     for (int x = 0; x < kPT; ++x) {
       const int t = t0 + x < E.ntiles ? t0 + x : t0;
       const int i0 = t / (E.ntj * E.ntk) * kT, j0 = (t / E.ntk) % E.ntj * kT, k0 = t % E.ntk * kT;
-#pragma unroll
-      for (int s = 0; s < 8; ++s) col[x][s] = code_at(E, i0 + s, j0 + a, k0 + b);  // (s, a, b)
-      // ordering key: lanes 0-7 read the tile's corners (clamped into the grid)
-      {
-        const int ci = lane & 4 ? (i0 + kT - 1 < E.m ? i0 + kT - 1 : E.m - 1) : i0;
-        const int cj = lane & 2 ? (j0 + kT - 1 < E.n ? j0 + kT - 1 : E.n - 1) : j0;
-        const int ck = lane & 1 ? (k0 + kT - 1 < E.l ? k0 + kT - 1 : E.l - 1) : k0;
-        cu[x] = lane < 8 ? (float)E.u[(int64_t)ci * E.sI + (int64_t)cj * E.l + ck] : -3.4e38f;
-      }
-      hal[x][0] = code_at(E, i0 - 1, j0 + a, k0 + b);   // i faces, position (a, b)
-      hal[x][1] = code_at(E, i0 + kT, j0 + a, k0 + b);
-      hal[x][2] = code_at(E, i0 + a, j0 - 1, k0 + b);   // j faces, position (a, b)
-      hal[x][3] = code_at(E, i0 + a, j0 + kT, k0 + b);
-      hal[x][4] = code_at(E, i0 + a, j0 + b, k0 - 1);   // k faces, position (a, b)
-      hal[x][5] = code_at(E, i0 + a, j0 + b, k0 + kT);
-      inn[x][0] = code_at(E, i0 + a, j0, k0 + b);       // tile nodes next to the j, k faces
-      inn[x][1] = code_at(E, i0 + a, j0 + kT - 1, k0 + b);
-      inn[x][2] = code_at(E, i0 + a, j0 + b, k0);
-      inn[x][3] = code_at(E, i0 + a, j0 + b, k0 + kT - 1);
+      const int i = i0 + si, j = j0 + a;
+      row[x] = code_row(E, i, j, k0);
+      // halo rows of the i faces (si = 0: i0 - 1; si = 7: i0 + 8) and of the
+      // j faces (a = 0: j0 - 1; a = 7: j0 + 8)
+      hi[x] = (si == 0 || si == kT - 1) ? code_row(E, si == 0 ? i0 - 1 : i0 + kT, j, k0)
+                                        : 0x4040404040404040ull;
+      hj[x] = (a == 0 || a == kT - 1) ? code_row(E, i, a == 0 ? j0 - 1 : j0 + kT, k0)
+                                      : 0x4040404040404040ull;
+      hk[x][0] = code_at(E, i, j, k0 - 1);
+      hk[x][1] = code_at(E, i, j, k0 + kT);
+      // corners (clamped into the grid) on lanes (0|7, 0|7)
+      const bool corner = (si == 0 || si == kT - 1) && (a == 0 || a == kT - 1);
+      const int ci = i < E.m ? i : E.m - 1, cj = j < E.n ? j : E.n - 1;
+      const int ck = k0 + kT - 1 < E.l ? k0 + kT - 1 : E.l - 1;
+      const int64_t cq = (int64_t)ci * E.sI + (int64_t)cj * E.l;
+      cu[x][0] = corner ? (float)E.u[cq + k0] : -3.4e38f;
+      cu[x][1] = corner ? (float)E.u[cq + ck] : -3.4e38f;
     }
 #pragma unroll
     for (int x = 0; x < kPT; ++x) {
       const int t = t0 + x;
-      int nz = 0, ext = 0;
-#pragma unroll
-      for (int s = 0; s < 8; ++s) nz += (col[x][s] & kCZ) ? 0 : 1;
-      const uint8_t tn[6] = {col[x][0], col[x][7], inn[x][0], inn[x][1], inn[x][2], inn[x][3]};
-#pragma unroll
-      for (int q = 0; q < 6; ++q)
-        ext += (!(tn[q] & kCZ) && feeds(hal[x][q], q)) ? 1 : 0;
+      // non-Z nodes of the row: bytes with bit 6 clear
+      int nz = __builtin_popcountll(~row[x] & 0x4040404040404040ull);
+      int ext = 0;
+      if (si == 0) ext += face_edges(row[x], hi[x], 0);
+      if (si == kT - 1) ext += face_edges(row[x], hi[x], 1);
+      if (a == 0) ext += face_edges(row[x], hj[x], 2);
+      if (a == kT - 1) ext += face_edges(row[x], hj[x], 3);
+      const uint8_t first = (uint8_t)row[x], last = (uint8_t)(row[x] >> 56);
+      ext += (!(first & kCZ) && feeds(hk[x][0], 4)) ? 1 : 0;
+      ext += (!(last & kCZ) && feeds(hk[x][1], 5)) ? 1 : 0;
       nz = wave_sum(nz);
       ext = wave_sum(ext);
-      float kmax = cu[x];
+      float kmax = cu[x][0] > cu[x][1] ? cu[x][0] : cu[x][1];
 #pragma unroll
-      for (int off = 4; off > 0; off >>= 1) {
+      for (int off = 32; off > 0; off >>= 1) {
         const float o = __shfl_xor(kmax, off);
         kmax = o > kmax ? o : kmax;
       }

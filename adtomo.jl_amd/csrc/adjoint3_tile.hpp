@@ -181,51 +181,90 @@ __device__ __forceinline__ unsigned pend_dec(uint16_t* pend, int id, unsigned by
 
 // ---------------------------------------------------------------- phases ---
 
-// Phase 0 (whole workgroup, grid-stride over the event's nodes): upwind
+// Phase 0 (whole workgroup, grid-stride over k rows of kT nodes): upwind
 // codes (the reference's Z rule, Eikonal3D.cpp:105-171; no node final yet),
 // grad_u0 = g where u == u0 (:105-110; g is read only there), and
 // dL/df = -0 * (-2 f h h) on Z (the value the reference's final loop gives
 // those nodes, :194-196).  Z multipliers are 0 and never read: a Z node
-// feeds nothing.  Returns this thread's count of non-Z nodes.
-template <typename T, int kPrep>
-__device__ int phase_codes(const Ev<T>& E) {
-  const int m = E.m, n = E.n, l = E.l;
-  const int64_t nl = E.sI, N = (int64_t)m * nl;
-  const int ex[3] = {m, n, l};
-  const int64_t st[3] = {nl, l, 1};
-  const int64_t stride = kPrep * (int64_t)blockDim.x;
-  int nonz = 0;
-  Walk wk[kPrep];
+// feeds nothing.  A thread reads its row of u and u0 and the four
+// neighbouring rows (i -+ 1, j -+ 1; on a grid edge the inner row twice,
+// whose value the edge rule ignores) in 16-byte pieces, and the two k
+// neighbours past the row's ends.  Returns this thread's count of non-Z
+// nodes.
+
+// kT values from flat index q on (k0 + b < l; the rest 0); rows that lie
+// inside the grid and start 16-byte aligned are read in 16-byte pieces.
+template <typename T>
+__device__ __forceinline__ void row_load(gptr<const T> p, int64_t q, int k0, int l, T (&x)[kT]) {
+  constexpr int kPer = 16 / (int)sizeof(T);
+  if (k0 + kT <= l && !(q & (kPer - 1))) {
+    using v4 = __attribute__((ext_vector_type(4))) unsigned int;
+    gptr<const v4> v = (gptr<const v4>)(p + q);
 #pragma unroll
-  for (int k = 0; k < kPrep; ++k) wk[k] = Walk(threadIdx.x + k * blockDim.x, stride, m, n, l);
-  for (int64_t p0 = threadIdx.x; p0 < N; p0 += stride) {
-    const PrepGroup<kPrep> G = prep_group<kPrep>(p0, N, wk);
-    T uc[kPrep], u0c[kPrep], nb[kPrep][6];
+    for (int c = 0; c < kT / kPer; ++c) {
+      const v4 w = v[c];
+      const unsigned ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int k = 0; k < kPrep; ++k) {
-      const int64_t q = G.q[k];
-      uc[k] = E.u[q];
-      u0c[k] = E.u0[q];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {  // clamped into the grid; edges use one side
-        nb[k][2 * d] = E.u[G.co[k][d] > 0 ? q - st[d] : q + st[d]];
-        nb[k][2 * d + 1] = E.u[G.co[k][d] < ex[d] - 1 ? q + st[d] : q - st[d]];
+      for (int e = 0; e < kPer; ++e) {
+        if constexpr (sizeof(T) == 8)
+          x[c * kPer + e] = __longlong_as_double((long long)(((uint64_t)ww[2 * e + 1] << 32) | ww[2 * e]));
+        else
+          x[c * kPer + e] = __uint_as_float(ww[e]);
       }
     }
+  } else {
 #pragma unroll
-    for (int k = 0; k < kPrep; ++k) {
-      const uint32_t code = code3<T>(uc[k], u0c[k], nb[k], G.co[k], ex);
-      if (G.valid[k]) {
-        const int64_t q = G.q[k];
-        const bool z = code & kZ;
-        E.code[q] = (uint8_t)code;
-        if (E.gu0) E.gu0[q] = (uc[k] == u0c[k]) ? E.g[q] : T(0);
-        if (z) {
+    for (int e = 0; e < kT; ++e) x[e] = k0 + e < l ? p[q + e] : T(0);
+  }
+}
+
+template <typename T>
+__device__ int phase_codes(const Ev<T>& E) {
+  const int m = E.m, n = E.n, l = E.l;
+  const int nkb = (l + kT - 1) / kT;
+  const int64_t nrows = (int64_t)m * n * nkb;
+  const int ex[3] = {m, n, l};
+  int nonz = 0;
+  for (int64_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+    const int kb = (int)(r % nkb);
+    const int64_t ij = r / nkb;
+    const int j = (int)(ij % n), i = (int)(ij / n), k0 = kb * kT;
+    const int64_t q0 = (int64_t)i * E.sI + (int64_t)j * l + k0;
+    T own[kT], u0r[kT], im[kT], ip[kT], jm[kT], jp[kT];
+    row_load(E.u, q0, k0, l, own);
+    row_load(E.u0, q0, k0, l, u0r);
+    row_load(E.u, i > 0 ? q0 - E.sI : q0 + E.sI, k0, l, im);
+    row_load(E.u, i < m - 1 ? q0 + E.sI : q0 - E.sI, k0, l, ip);
+    row_load(E.u, j > 0 ? q0 - l : q0 + l, k0, l, jm);
+    row_load(E.u, j < n - 1 ? q0 + l : q0 - l, k0, l, jp);
+    const T kprev = k0 > 0 ? E.u[q0 - 1] : T(0);
+    const T knext = k0 + kT < l ? E.u[q0 + kT] : T(0);
+    uint64_t codes = 0;
+#pragma unroll
+    for (int e = 0; e < kT; ++e) {
+      const int k = k0 + e;
+      const T nb[6] = {im[e], ip[e], jm[e], jp[e], e > 0 ? own[e - 1] : kprev,
+                       e + 1 < kT ? own[e + 1] : knext};
+      const int co[3] = {i, j, k};
+      const uint32_t code = k < l ? code3<T>(own[e], u0r[e], nb, co, ex) : kCZ;
+      codes |= (uint64_t)code << (8 * e);
+      if (k < l) {
+        const int64_t q = q0 + e;
+        if (E.gu0) E.gu0[q] = (own[e] == u0r[e]) ? E.g[q] : T(0);
+        if (code & kZ) {
           const T rhs = T(-2.0) * E.f[q] * E.h * E.h;
           E.gf[q] = -T(0) * rhs;
+        } else {
+          ++nonz;
         }
-        nonz += z ? 0 : 1;
       }
+    }
+    if (k0 + kT <= l && !(q0 & 7)) {
+      *(gptr<uint64_t>)(E.code + q0) = codes;
+    } else {
+#pragma unroll
+      for (int e = 0; e < kT; ++e)
+        if (k0 + e < l) E.code[q0 + e] = (uint8_t)(codes >> (8 * e));
     }
   }
   return nonz;
@@ -713,7 +752,7 @@ __device__ void adj3_tile_event(const EventDesc& ev) {
     S.kmax = 0u;
   }
   __syncthreads();
-  const int nz = phase_codes<T, 4>(E);
+  const int nz = phase_codes<T>(E);
   atomicAdd(&S.remaining, nz);
   __syncthreads();
   phase_counts<T>(E, S, NW, w);

@@ -591,9 +591,11 @@ def eikonal3d_grad_plain(grad_u, u, u0, f, h):
 
 def selftest_arith(op: int, a, b=None):
     """Runs one device arithmetic op of the path on torch device tensors
-    (op 6: plain copy, the reference kernel for memory counters)."""
+    (op 6: plain copy, the reference kernel for memory counters; ops 7, 8:
+    the square root and the 3D local solve as the forward's step loops
+    take them)."""
     torch = _torch()
-    n = a.numel() // (3 if op == 4 else 2 if op == 5 else 1)
+    n = a.numel() // (3 if op in (4, 8) else 2 if op == 5 else 1)
     out = torch.empty(n, dtype=a.dtype, device=a.device)
     fn = getattr(L.lib(), "adt_selftest_arith" + _sfx(a.dtype))
     rc = fn(op, a.data_ptr(), b.data_ptr() if b is not None else None, out.data_ptr(), n,

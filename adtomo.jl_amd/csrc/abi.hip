@@ -2197,7 +2197,7 @@ int adt_sum_arrays_f32(const float* const* arrays, int32_t narr, int64_t N, floa
 
 int adt_selftest_arith_f64(int32_t op, const double* a, const double* b, double* out, int64_t n,
                            void* stream) {
-  if (n < 0 || !a || !out || op < 0 || op > 6 || ((op == 3 || (op >= 4 && op <= 5)) && !b))
+  if (n < 0 || !a || !out || op < 0 || op > 8 || ((op == 3 || op == 4 || op == 5 || op == 8) && !b))
     return fail(ADT_EINVAL, "selftest: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   adt::arith_kernel<double><<<op == 6 ? 4096 : 256, 256, 0, s>>>(op, a, b, out, n);
@@ -2207,7 +2207,7 @@ int adt_selftest_arith_f64(int32_t op, const double* a, const double* b, double*
 }
 int adt_selftest_arith_f32(int32_t op, const float* a, const float* b, float* out, int64_t n,
                            void* stream) {
-  if (n < 0 || !a || !out || op < 0 || op > 6 || ((op == 3 || (op >= 4 && op <= 5)) && !b))
+  if (n < 0 || !a || !out || op < 0 || op > 8 || ((op == 3 || op == 4 || op == 5 || op == 8) && !b))
     return fail(ADT_EINVAL, "selftest: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   adt::arith_kernel<float><<<op == 6 ? 4096 : 256, 256, 0, s>>>(op, a, b, out, n);

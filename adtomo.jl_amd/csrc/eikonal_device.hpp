@@ -23,10 +23,37 @@ using gptr = __attribute__((address_space(1))) T*;
 template <typename T> struct Real;
 template <> struct Real<double> {
   static __device__ __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }
+  // Fast IEEE square root for the local solve's discriminants.  The
+  // compiler expands __builtin_sqrt into a reciprocal square root estimate
+  // refined by Newton steps (g ~ sqrt x, h ~ 1 / (2 sqrt x), then two
+  // residual corrections), wrapped in a rescaling for x < 2^-767 and a
+  // fix-up for +-0 and +inf.  refine_ runs the same sequence of operations
+  // without the wrapper: the same bits for every x with |x| >= 2^-767 below
+  // +inf (the wrapper is then the identity; negative x gives NaN in both).
+  // risky_ flags the other operands (|x| < 2^-767, +inf), for which the
+  // caller takes __builtin_sqrt instead.
+  static __device__ __forceinline__ double refine_(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+  }
+  static __device__ __forceinline__ bool risky_(double x) {
+    return __builtin_fabs(x) < 0x1p-767 || x == __builtin_inf();
+  }
+  static constexpr bool kFastSqrt = true;
   static __device__ __forceinline__ double fabs_(double x) { return __builtin_fabs(x); }
 };
 template <> struct Real<float> {
   static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ float refine_(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ bool risky_(float) { return false; }
+  static constexpr bool kFastSqrt = false;
   static __device__ __forceinline__ float fabs_(float x) { return __builtin_fabsf(x); }
 };
 
@@ -48,9 +75,11 @@ __device__ __forceinline__ T solution2d(T a, T b, T f, T h) {
 // lies at least (result quantum) / 6 from every rounding midpoint, while
 // q0 + r y differs from x / 3 by r (y - 1/3), below 2^-50 of that quantum;
 // the argument holds in gradual underflow too.  For x = +-inf the residual
-// is inf - inf = NaN, which the minimum with the largest finite number (IEEE
-// minNum returns the number) replaces by a finite value, so q = q0 = +-inf;
-// the minimum leaves every finite residual as it is, and NaN stays NaN.
+// is inf - inf = NaN, which the minimum with 2^1023 (IEEE minNum returns the
+// number) replaces by a finite value, so q = q0 = +-inf; the minimum leaves
+// every finite residual as it is (|r| is at most about 2^-52 |x| < 2^972),
+// and NaN stays NaN.  (2^1023 is an instruction literal; the largest finite
+// number would take a scalar register pair through the step loops.)
 // Checked against IEEE division on every binary32 input and on 2e8 binary64
 // inputs (half of them subnormal or in the lowest 60 binades).  The one
 // exception: -0 gives +0.
@@ -59,7 +88,7 @@ template <> struct Div3<double> {
   static __device__ __forceinline__ double go(double x) {
     const double y = 0x1.5555555555555p-2;
     const double q0 = x * y;
-    const double r = __builtin_fmin(__builtin_fma(-3.0, q0, x), 0x1.fffffffffffffp1023);
+    const double r = __builtin_fmin(__builtin_fma(-3.0, q0, x), 0x1p1023);
     return __builtin_fma(r, y, q0);
   }
 };
@@ -78,9 +107,11 @@ __device__ __forceinline__ T div3(T x) { return Div3<T>::go(x); }
 // The three candidates (1D, 2D, 3D) are evaluated as written and the first
 // that the reference would return is selected: nearly every update of a wave
 // needs the 3D one, so evaluating all of them costs nothing and lets the 2D
-// and 3D square roots run side by side.
-template <typename T>
-__device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h) {
+// and 3D square roots run side by side.  FAST (FP64): the square roots
+// through Real::refine_, and `unsafe` set for a lane whose discriminants
+// it may get wrong; the caller then redoes the update with FAST off.
+template <typename T, bool FAST = false>
+__device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h, bool& unsafe) {
   T temp;
   if (a1 > a2) { temp = a1; a1 = a2; a2 = temp; }
   if (a1 > a3) { temp = a1; a1 = a3; a3 = temp; }
@@ -88,14 +119,29 @@ __device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h) {
   const T x1 = a1 + f * h;
   const T B2 = -(a1 + a2);
   const T C2 = (a1 * a1 + a2 * a2 - f * f * h * h) / T(2.0);
-  const T x2 = (-B2 + Real<T>::sqrt_(B2 * B2 - T(4.0) * C2)) / T(2.0);
   // div3's one exception (-0 -> +0) leaves x3 unchanged: B3 enters x3 as
   // B3 * B3 and as -B3 + s with s = sqrt(...) never -0, and C3's argument is
   // never -0 (a difference of two non-negative numbers)
   const T B3 = div3(T(-2.0) * (a1 + a2 + a3));
   const T C3 = div3(a1 * a1 + a2 * a2 + a3 * a3 - f * f * h * h);
-  const T x3 = (-B3 + Real<T>::sqrt_(B3 * B3 - T(4.0) * C3)) / T(2.0);
+  const T D2 = B2 * B2 - T(4.0) * C2, D3 = B3 * B3 - T(4.0) * C3;
+  T s2, s3;
+  if constexpr (FAST && Real<T>::kFastSqrt) {
+    s2 = Real<T>::refine_(D2);
+    s3 = Real<T>::refine_(D3);
+    unsafe = unsafe || Real<T>::risky_(D2) || Real<T>::risky_(D3);
+  } else {
+    s2 = Real<T>::sqrt_(D2);
+    s3 = Real<T>::sqrt_(D3);
+  }
+  const T x2 = (-B2 + s2) / T(2.0);
+  const T x3 = (-B3 + s3) / T(2.0);
   return (x1 <= a2) ? x1 : ((x2 <= a3) ? x2 : x3);
+}
+template <typename T>
+__device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h) {
+  bool unused = false;
+  return solution3d<T, false>(a1, a2, a3, f, h, unused);
 }
 
 // Combines the canonical upwind (new) / downwind (old) values of one axis

@@ -61,6 +61,8 @@
 // err = max|u - u_old| is a coalesced pass after the eighth sweep against
 // the snapshot (u0 for the first iteration), which also writes the next
 // snapshot u_old = u.
+#include <type_traits>
+
 #include "adjoint3.hpp"
 #include "buffer_ops.hpp"
 #include "misfit3.hpp"
@@ -79,7 +81,7 @@ constexpr int kLW = 8;  // maximum segment length (nodes per lane per step)
 
 struct DGeom {
   int Ma, Nb, L, H, bands, V, Lw;
-  int64_t sA, sB;  // memory strides of the a and b axes
+  int sA, sB;  // memory strides of the a and b axes (offsets fit 32 bits)
   bool a_is_i;
 };
 
@@ -88,8 +90,8 @@ __device__ inline DGeom make_dgeom(int m, int n, int l) {
   g.a_is_i = (m <= n);
   g.Ma = g.a_is_i ? m : n;
   g.Nb = g.a_is_i ? n : m;
-  g.sA = g.a_is_i ? (int64_t)n * l : (int64_t)l;
-  g.sB = g.a_is_i ? (int64_t)l : (int64_t)n * l;
+  g.sA = g.a_is_i ? n * l : l;
+  g.sB = g.a_is_i ? l : n * l;
   g.L = l;
   g.Lw = (l + kDW - 1) / kDW;
   g.bands = (g.Ma + 63) / 64;
@@ -98,41 +100,33 @@ __device__ inline DGeom make_dgeom(int m, int n, int l) {
   return g;
 }
 
-// Column of lane p on diagonal v.
+// Column of lane p on a diagonal: canonical (a, b) of its sweep, whether
+// that sweep flips a / b, and the flat index of canonical (a, b, c = 0).
+// The flags are bits of an integer (a lane value in a vector register, not
+// a lane mask in scalar registers, which the step loop cannot spare).
 struct DCol {
   int a, b;
-  bool valid;
-  bool fa, fb;    // the column's sweep flips a / b
-  int64_t node0;  // flat index of canonical (a, b, c = 0)
+  uint32_t fl;  // bit 0: valid, bit 1: fa, bit 2: fb
+  int node0;
+  __device__ __forceinline__ bool valid() const { return fl & 1; }
+  __device__ __forceinline__ bool fa() const { return fl & 2; }
+  __device__ __forceinline__ bool fb() const { return fl & 4; }
 };
-
-__device__ __forceinline__ DCol dcol(const DGeom& g, int v, int p, int64_t base0, int64_t tA,
-                                     int64_t tB) {
-  DCol c;
-  const int q = v - p;
-  const int qq = q > 0 ? q : 0;
-  const int beta = qq / g.Nb;
-  c.b = qq - beta * g.Nb;
-  c.a = beta * g.H + p;
-  c.valid = (p < g.H) && (q >= 0) && (beta < g.bands) && (c.a < g.Ma);
-  c.node0 = base0 + (int64_t)c.a * tA + (int64_t)c.b * tB;
-  return c;
-}
 
 constexpr int kChainMax = 4;  // sweeps of one chain (the c direction fixed)
 
 template <typename T>
 struct DSweep {
   DGeom g;
-  int64_t tC;
+  int tC;
   bool flipC;
   // the chained sweeps (1..kChainMax): bits 2q, 2q+1 of cfl = sweep q
   // flips a, b; a sweep's canonical (a, b, c = 0) lies at
-  // (fa ? aoff : 0) + (fb ? boff : 0) + coff with strides -+sA, -+sB
+  // ra sA + rb sB + coff, ra = fa ? Ma-1-a : a, rb = fb ? Nb-1-b : b
   int nch;
   int V0;  // bands * N_b: diagonal offset between consecutive chained sweeps
   int cfl;
-  int64_t aoff, boff, coff;
+  int coff;
   T h;
   double hd;    // h for the FP64 local solve of the FP32-storage variant
   int cs;       // canonical c of this pass's first node (k chunks)
@@ -147,38 +141,69 @@ struct DSweep {
   rsrc_t snap_dst;  // ... and the snapshot for the next iteration
 };
 
-// Column of lane p on chain diagonal t, whose chained sweep kk (uniform,
-// kk = min(t / V0, nch - 1)) owns local diagonal v = t - kk V0: lanes p > v
-// of a sweep's first diagonals take the previous sweep's diagonal v + V0.
-// Stride of the a axis in chained sweep k (uniform).
+// Position of lane p on chain diagonal t: with q = t - p, the lane's
+// chained sweep is k = q div V0 (lanes p > t - kk V0 of diagonal t, kk =
+// t div V0, still run sweep kk-1's last diagonals), its band beta and
+// canonical b come from r = q mod V0 = beta N_b + b, and a = beta H + p.
+// Lanes keep (k, beta, b) and advance it by one diagonal per step; before
+// the chain's first column reaches the lane (q < 0) b holds q.
+struct DCur {
+  int k, beta, b;
+};
+
 template <typename T>
-__device__ __forceinline__ int64_t chain_tA(const DSweep<T>& S, int k) {
-  return ((S.cfl >> (2 * k)) & 1) ? -S.g.sA : S.g.sA;
+__device__ __forceinline__ DCur cur_at(const DSweep<T>& S, int t, int p) {
+  const int q = t - p;
+  DCur c;
+  if (q < 0) {
+    c.k = 0;
+    c.beta = 0;
+    c.b = q;
+  } else {
+    c.k = q / S.V0;
+    const int r = q - c.k * S.V0;
+    c.beta = r / S.g.Nb;
+    c.b = r - c.beta * S.g.Nb;
+  }
+  return c;
 }
 
 template <typename T>
-__device__ __forceinline__ DCol chain_col(const DSweep<T>& S, int t, int kk, int p) {
-  const int v = t - kk * S.V0;
-  const bool prev = kk > 0 && p > v;
-  const int fl = (S.cfl >> (2 * (prev ? kk - 1 : kk))) & 3;
-  const bool fa = fl & 1, fb = fl & 2;
-  const int64_t base = (fa ? S.aoff : 0) + (fb ? S.boff : 0) + S.coff;
-  DCol c = dcol(S.g, prev ? v + S.V0 : v, p, base, fa ? -S.g.sA : S.g.sA,
-                fb ? -S.g.sB : S.g.sB);
-  c.fa = fa;
-  c.fb = fb;
-  return c;
+__device__ __forceinline__ void cur_next(const DSweep<T>& S, DCur& c) {
+  const int b1 = c.b + 1;
+  const bool wb = b1 == S.g.Nb;
+  c.b = wb ? 0 : b1;
+  const int beta1 = c.beta + (wb ? 1 : 0);
+  const bool wk = beta1 == S.g.bands;
+  c.beta = wk ? 0 : beta1;
+  c.k += wk ? 1 : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ DCol cur_col(const DSweep<T>& S, const DCur& c, int p) {
+  const DGeom& g = S.g;
+  DCol col;
+  const int kc = c.k < kChainMax ? c.k : kChainMax;  // bits beyond the chain are 0
+  const uint32_t fl = (S.cfl >> (2 * kc)) & 3;
+  col.a = c.beta * g.H + p;
+  col.b = c.b;
+  const bool valid = (p < g.H) && (c.b >= 0) && (c.k < S.nch) && (col.a < g.Ma);
+  col.fl = (fl << 1) | (valid ? 1u : 0u);
+  const int ra = (fl & 1) ? g.Ma - 1 - col.a : col.a;
+  const int rb = (fl & 2) ? g.Nb - 1 - col.b : col.b;
+  col.node0 = S.coff + ra * g.sA + rb * g.sB;
+  return col;
 }
 
 // cnt values of column `col` from canonical c0 on (element j <- c0 + j),
 // one load per element.
 template <typename T, int NE>
-__device__ __forceinline__ void load_seg(T (&buf)[NE], rsrc_t r, const DCol& col, int64_t delta,
-                                         bool ok, int c0, int cnt, int L, int64_t tC) {
+__device__ __forceinline__ void load_seg(T (&buf)[NE], rsrc_t r, const DCol& col, int delta,
+                                         bool ok, int c0, int cnt, int L, int tC) {
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
     const int c = c0 + j;
-    buf[j] = bload(r, boff<T>(ok && j < cnt && c < L, col.node0 + delta + (int64_t)c * tC), T(0));
+    buf[j] = bload(r, boff<T>(ok && j < cnt && c < L, col.node0 + delta + c * tC), T(0));
   }
 }
 
@@ -541,19 +566,16 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
 
   // Memory index of the lowest node of the segment of column `col`.
   auto seg_of = [&](const DCol& col) -> int {
-    const int64_t lo = fwd ? col.node0 + c0 : col.node0 - c0 - (cnt - 1);
-    return (col.valid && cnt > 0) ? (int)lo : -1;
+    const int lo = fwd ? col.node0 + c0 : col.node0 - c0 - (cnt - 1);
+    return (col.valid() && cnt > 0) ? lo : -1;
   };
   // Band-edge neighbour of lane 0 ((a-1, b), new) and lane H-1 ((a+1, b),
   // old) of column `col`, one load per node (canonical order).
-  // (lanes 0 and H-1 always take the diagonal's own sweep kk: a lane in
-  // the previous sweep's tail lies in its last band, where lane H-1 is the
-  // grid edge and lane 0 never is)
-  auto load_edge = [&](T(&buf)[kLW], const DCol& col, int kk) {
-    const bool eu = lane_first && col.valid && col.a > 0;
-    const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
+  auto load_edge = [&](T(&buf)[kLW], const DCol& col) {
+    const bool eu = lane_first && col.valid() && col.a > 0;
+    const bool ed = lane_last && col.valid() && col.a + 1 < g.Ma;
     T bo[kLW];
-    const int64_t tA = chain_tA(S, kk);
+    const int tA = col.fa() ? -g.sA : g.sA;
     load_seg(buf, S.u, col, -tA, eu, c0, cnt, L, S.tC);
     load_seg(bo, S.src, col, tA, ed, c0, cnt, L, S.tC);
 #pragma unroll
@@ -578,46 +600,44 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   const bool need_cp = CH && !has_x && cnt > 0 && S.cs + L < S.Lfull;
   auto load_cm = [&](const DCol& col) -> T {
     if constexpr (!CH) return T(0);
-    return bload(S.u, boff<T>(need_cm && col.valid, col.node0 - S.tC), T(0));
+    return bload(S.u, boff<T>(need_cm && col.valid(), col.node0 - S.tC), T(0));
   };
   auto load_cp = [&](const DCol& col) -> T {
     if constexpr (!CH) return T(0);
-    return bload(S.src, boff<T>(need_cp && col.valid, col.node0 + (int64_t)L * S.tC), T(0));
+    return bload(S.src, boff<T>(need_cp && col.valid(), col.node0 + L * S.tC), T(0));
   };
   T XCM, XCP;  // values for the current diagonal, landed at step start
   // band-edge segment of lane 0 / lane H-1 for column `col` (full variant)
-  auto eseg_of = [&](const DCol& col, int kk) -> int {
-    const bool eu = lane_first && col.valid && col.a > 0;
-    const bool ed = lane_last && col.valid && col.a + 1 < g.Ma;
-    const int64_t tA = chain_tA(S, kk);
-    const int64_t d = eu ? -tA : tA;
-    const int64_t lo = fwd ? col.node0 + d + c0 : col.node0 + d - c0 - (kLW - 1);
-    return (eu || ed) ? (int)lo : -1;
+  auto eseg_of = [&](const DCol& col) -> int {
+    const bool eu = lane_first && col.valid() && col.a > 0;
+    const bool ed = lane_last && col.valid() && col.a + 1 < g.Ma;
+    const int tA = col.fa() ? -g.sA : g.sA;
+    const int d = eu ? -tA : tA;
+    const int lo = fwd ? col.node0 + d + c0 : col.node0 + d - c0 - (kLW - 1);
+    return (eu || ed) ? lo : -1;
   };
-  auto issue_edge = [&](const DCol& col, int kk) {
+  auto issue_edge = [&](const DCol& col) {
     if constexpr (FULL)
-      EF = edge_issue<T>(S.u, S.src, eseg_of(col, kk), g.H - 1, p);
+      EF = edge_issue<T>(S.u, S.src, eseg_of(col), g.H - 1, p);
     else
-      load_edge(LE, col, kk);
+      load_edge(LE, col);
   };
 
-  // chained sweep of chain diagonal t (t < 0: the first)
-  auto chain_of = [&](int t) -> int {
-    const int k = t > 0 ? t / S.V0 : 0;
-    return k < S.nch - 1 ? k : S.nch - 1;
-  };
   const int v0 = -w;  // diagonal of step 0
-  int kk0 = chain_of(v0), kk1 = chain_of(v0 + 1), kk2 = chain_of(v0 + 2);
-  DCol cv = chain_col(S, v0, kk0, p);
-  DCol cv1 = chain_col(S, v0 + 1, kk1, p);
-  DCol cv2 = chain_col(S, v0 + 2, kk2, p);
+  DCur cur = cur_at(S, v0, p);
+  DCol cv = cur_col(S, cur, p);
+  cur_next(S, cur);
+  DCol cv1 = cur_col(S, cur, p);
+  cur_next(S, cur);
+  DCol cv2 = cur_col(S, cur, p);
+  cur_next(S, cur);  // at diagonal v0 + 3
   // prologue: plane x = old(v0); in flight: old(v0+1), f(v0), edges(v0);
   // node 0 of old(v0+1) published for the wave upstream (its step 0)
   seg_issue<T, FULL>(FD, S.src, seg_of(cv), cnt, p);
   seg_land<T, FULL>(FD, px, cnt, fwd, p);
   seg_issue<T, FULL>(FD, S.src, seg_of(cv1), cnt, p);
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
-  issue_edge(cv, kk0);
+  issue_edge(cv);
   T XCM_n = load_cm(cv), XCP_n = load_cp(cv);
   if constexpr (ERR) seg_issue<T, FULL>(FS, S.snap_src, -1, cnt, p);
   seg_publish<T, FULL>(FD, xold + kDW * 64 + w * 64, cnt, fwd, p);
@@ -672,8 +692,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : XCM;
       const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : XCP;
-      const int ra = cv.fa ? g.Ma - 1 - cv.a : cv.a;
-      const int rb = cv.fb ? g.Nb - 1 - cv.b : cv.b;
+      const int ra = cv.fa() ? g.Ma - 1 - cv.a : cv.a;
+      const int rb = cv.fb() ? g.Nb - 1 - cv.b : cv.b;
       const bool af = ra == 0, al = ra == g.Ma - 1;
       const bool bf = rb == 0, bl = rb == g.Nb - 1;
       // The a and b neighbours are read straight from the planes: the sweep
@@ -682,8 +702,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       // b-1 (new) / b+1 (old) at position p+1 of planes n / x.
       const lptr<T> a_up = pn + p, a_dn = px + p + 2;
       const lptr<T> b_up = pn + p + 1, b_dn = px + p + 1;
-      const lptr<T> a_plus = cv.fa ? a_up : a_dn, a_minus = cv.fa ? a_dn : a_up;
-      const lptr<T> b_plus = cv.fb ? b_up : b_dn, b_minus = cv.fb ? b_dn : b_up;
+      const lptr<T> a_plus = cv.fa() ? a_up : a_dn, a_minus = cv.fa() ? a_dn : a_up;
+      const lptr<T> b_plus = cv.fb() ? b_up : b_dn, b_minus = cv.fb() ? b_dn : b_up;
       const lptr<T> x_plus0 = g.a_is_i ? a_plus : b_plus, x_minus0 = g.a_is_i ? a_minus : b_minus;
       const lptr<T> y_plus0 = g.a_is_i ? b_plus : a_plus, y_minus0 = g.a_is_i ? b_minus : a_minus;
       const bool xf = g.a_is_i ? af : bf, xl = g.a_is_i ? al : bl;
@@ -694,39 +714,54 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       // selects become one address choice per step.
       const lptr<T> x_plus = xl ? x_minus0 : x_plus0, x_minus = xf ? x_plus0 : x_minus0;
       const lptr<T> y_plus = yl ? y_minus0 : y_plus0, y_minus = yf ? y_plus0 : y_minus0;
+      // The node updates take the fast square roots (Real::refine_); if a
+      // lane met a discriminant they may get wrong, the whole wave redoes the
+      // step's updates from the old values (still in memory: this step's new
+      // values are stored at the next step) with the IEEE square root.
+      bool unsafe = false;
+      auto nodes = [&](auto fast, auto issue) {
+        constexpr bool FAST = decltype(fast)::value, ISSUE = decltype(issue)::value;
 #pragma unroll
-      for (int e = 0; e < kLW; ++e) {
-        if (e == 0) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
-        if (e == 1) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
-        if (e == 2) issue_edge(cv1, kk1);
-        if (e == 2) {
-          XCM_n = load_cm(cv1);
-          XCP_n = load_cp(cv1);
-        }
-        if (FULL || e < cnt) {
-          const int c = c0 + e;
-          const int rw = e * P::RS;
-          // std::min(u(+1), u(-1)) per axis, one-sided at the edges
-          // (Eikonal3D.cpp:47-52)
-          const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
-          const T fe = pf[rw + p + 1];
-          const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
-          const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
-          if (cv.valid) {
-            const int cg = S.cs + c;  // canonical c in the whole event
-            const int rc = S.flipC ? S.Lfull - 1 - cg : cg;
-            const T ux = smin(xp, xm);
-            const T uy = smin(yp, ym);
-            const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == S.Lfull - 1);
-            T cand;
-            if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
-              cand = (T)solution3d<double>(ux, uy, mc, fe, S.hd);
-            else
-              cand = solution3d(ux, uy, mc, fe, S.h);
-            OWN[e] = smin(cand, OWN[e]);
+        for (int e = 0; e < kLW; ++e) {
+          if constexpr (ISSUE) {
+            if (e == 0) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
+            if (e == 1) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
+            if (e == 2) issue_edge(cv1);
+            if (e == 2) {
+              XCM_n = load_cm(cv1);
+              XCP_n = load_cp(cv1);
+            }
           }
-          if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
+          if (FULL || e < cnt) {
+            const int c = c0 + e;
+            const int rw = e * P::RS;
+            // std::min(u(+1), u(-1)) per axis, one-sided at the edges
+            // (Eikonal3D.cpp:47-52)
+            const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
+            const T fe = pf[rw + p + 1];
+            const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
+            const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
+            if (cv.valid()) {
+              const int cg = S.cs + c;  // canonical c in the whole event
+              const int rc = S.flipC ? S.Lfull - 1 - cg : cg;
+              const T ux = smin(xp, xm);
+              const T uy = smin(yp, ym);
+              const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == S.Lfull - 1);
+              T cand;
+              if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
+                cand = (T)solution3d<double, FAST>(ux, uy, mc, fe, S.hd, unsafe);
+              else
+                cand = solution3d<T, FAST>(ux, uy, mc, fe, S.h, unsafe);
+              OWN[e] = smin(cand, OWN[e]);
+            }
+            if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
+          }
         }
+      };
+      nodes(std::true_type{}, std::true_type{});
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(unsafe) != 0, 0)) {
+        load_seg(OWN, S.src, cv, 0, cv.valid(), c0, cnt, L, S.tC);
+        nodes(std::false_type{}, std::false_type{});
       }
     }
     // plane n: new values of v (all reads of v-1 are done)
@@ -739,10 +774,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     xch[(t & 1) * kDW * 64 + w * 64 + p] = xo;
     cv = cv1;
     cv1 = cv2;
-    kk0 = kk1;
-    kk1 = kk2;
-    if (kk2 < S.nch - 1 && v + 3 >= (kk2 + 1) * S.V0) ++kk2;  // chain_of(v + 3)
-    cv2 = chain_col(S, v + 3, kk2, p);
+    cv2 = cur_col(S, cur, p);  // diagonal v + 3
+    cur_next(S, cur);
     if (p == 0) publish_progress(prog, w, t + 1);
   }
   if constexpr (ERR)
@@ -806,8 +839,13 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   // does
   const int kChunk = S.nseg * kLW;
   const int nchunk = (l + kChunk - 1) / kChunk;
-  S.h = (T)ev.h;
-  S.hd = ev.h;
+  // h in a scalar pair of its own: read straight from the descriptor it is
+  // one part of a wider scalar load, and the step loop would reload the
+  // whole wider tuple from spill slots for every node
+  double hd = ev.h;
+  asm("" : "+s"(hd));
+  S.h = (T)hd;
+  S.hd = hd;
   S.u = make_rsrc(ev.u, bytes);
   S.u0 = make_rsrc(ev.u0, bytes);
   S.f = make_rsrc(ev.f, bytes);
@@ -849,9 +887,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
           const bool fa = S.g.a_is_i ? fi : fj, fb = S.g.a_is_i ? fj : fi;
           S.cfl |= ((fa ? 1 : 0) | (fb ? 2 : 0)) << (2 * q);
         }
-        S.aoff = (int64_t)(S.g.Ma - 1) * S.g.sA;
-        S.boff = (int64_t)(S.g.Nb - 1) * S.g.sB;
-        S.coff = S.flipC ? (int64_t)(l - 1 - S.cs) : (int64_t)S.cs;
+        S.coff = S.flipC ? l - 1 - S.cs : S.cs;
         if (sw < 7) {
           diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes, prog,
                                     &abort_flag);

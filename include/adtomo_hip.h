@@ -396,7 +396,11 @@ int adt_sum_arrays_f32(const float *const *arrays, int32_t narr, int64_t N, floa
  * 4: 3D local solve (a = 3 values, b = f,h per element);
  * 5: 2D local solve (a = 2 values, b = f,h per element);
  * 6: plain copy out = a (a streaming kernel of known traffic, n elements
- *    read and written once, for calibrating memory counters). */
+ *    read and written once, for calibrating memory counters);
+ * 7: sqrt(a) as the forward's step loops take it (the fast refinement, the
+ *    IEEE routine where it flags the operand);
+ * 8: op 4 as the forward's step loops take it (fast square roots, the
+ *    update redone with IEEE ones where they flag a discriminant). */
 int adt_selftest_arith_f64(int32_t op, const double *a, const double *b, double *out,
                            int64_t n, void *stream);
 int adt_selftest_arith_f32(int32_t op, const float *a, const float *b, float *out, int64_t n,

@@ -1,10 +1,11 @@
 """The division by 3 of the 3D local solve (csrc/eikonal_device.hpp, div3):
 q0 = RN(x y) with y = RN(1/3), r = x - 3 q0, q = RN(q0 + r y), the residual
-replaced by the largest finite number when it is NaN.
+clamped to at most 2^1023 (IEEE minNum, so NaN becomes 2^1023).
 
 Pinned on the CPU in exact rational arithmetic against IEEE division
 (Python's float division is correctly rounded): the residual is exactly
-representable (so the FMA computing it is exact), and q equals x / 3 for
+representable (so the FMA computing it is exact) and far below the clamp,
+and q equals x / 3 for
 normal, subnormal and boundary binary64 inputs, +-inf and NaN; -0 gives +0.
 The device implementation is checked against IEEE division on the GPU
 (tests/test_gpu_parity.py::test_device_arith_is_ieee).
@@ -15,6 +16,7 @@ from fractions import Fraction
 import numpy as np
 
 Y = float.fromhex("0x1.5555555555555p-2")
+CLAMP = 2.0 ** 1023
 BIG = float.fromhex("0x1.fffffffffffffp+1023")
 
 
@@ -31,7 +33,8 @@ def div3(x):
         r_exact = Fraction(x) - 3 * Fraction(q0)
         r = float(r_exact)
         assert Fraction(r) == r_exact, x  # the FMA's residual is exact
-    r = BIG if math.isnan(r) else min(r, BIG)  # IEEE minNum: the number survives
+        assert abs(r) <= 2.0 ** 972, x  # the clamp never changes a finite residual
+    r = CLAMP if math.isnan(r) else min(r, CLAMP)  # IEEE minNum: the number survives
     if math.isinf(q0) or math.isnan(q0):
         return q0 + r * Y  # inf + finite, or NaN
     return rn(Fraction(q0) + Fraction(r) * Fraction(Y))

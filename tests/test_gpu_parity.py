@@ -70,7 +70,7 @@ def test_device_arith_is_ieee():
     x = np.concatenate([x, edges, -edges, bits, -bits])
     # the division by 3 of the local solve is IEEE's except that -0 gives +0
     div3 = lambda v: np.where(v == 0, np.zeros_like(v), v / 3.0)  # noqa: E731
-    for op, ref in ((0, np.sqrt), (1, div3), (2, lambda v: v / 2.0)):
+    for op, ref in ((0, np.sqrt), (7, np.sqrt), (1, div3), (2, lambda v: v / 2.0)):
         with np.errstate(invalid="ignore"):
             want = ref(x)
         got = host(A.selftest_arith(op, dev(x)))
@@ -107,9 +107,10 @@ def test_device_local_solvers_match_numpy():
     a[: n // 4, 1] = a[: n // 4, 0]  # ties
     a[n // 4: n // 2, 2] = a[n // 4: n // 2, 0] + r.uniform(0, 1e-3, n // 4)
     fh = np.stack([r.uniform(0.1, 2, n), r.uniform(0.1, 3, n)], 1)
-    got = host(A.selftest_arith(4, dev(a.reshape(-1)), dev(fh.reshape(-1))))
     want = hp.solution3d(a[:, 0], a[:, 1], a[:, 2], fh[:, 0], fh[:, 1])
-    assert bits_equal(got, want)
+    for op in (4, 8):  # 8: the fast square roots of the step loops
+        got = host(A.selftest_arith(op, dev(a.reshape(-1)), dev(fh.reshape(-1))))
+        assert bits_equal(got, want), op
     got2 = host(A.selftest_arith(5, dev(a[:, :2].reshape(-1)), dev(fh.reshape(-1))))
     want2 = hp.solution2d(a[:, 0], a[:, 1], fh[:, 0], fh[:, 1])
     assert bits_equal(got2, want2)
@@ -121,10 +122,24 @@ def test_device_local_solvers_match_numpy():
                    1.7976931348623157e308, np.inf, -np.inf, np.nan])
     fv = np.array([0.0, 5e-324, 1e-160, 0.5, 1.0, 1e155, 1e200, np.inf])
     g = np.stack(np.meshgrid(sv, sv, sv, fv, fv, indexing="ij"), -1).reshape(-1, 5)
-    got3 = host(A.selftest_arith(4, dev(g[:, :3].reshape(-1)), dev(g[:, 3:].reshape(-1))))
     with np.errstate(all="ignore"):
         want3 = hp.solution3d(g[:, 0], g[:, 1], g[:, 2], g[:, 3], g[:, 4])
-    assert bits_equal(got3, want3)
+    for op in (4, 8):
+        got3 = host(A.selftest_arith(op, dev(g[:, :3].reshape(-1)), dev(g[:, 3:].reshape(-1))))
+        assert bits_equal(got3, want3), op
+    # discriminants on both sides of the fast square root's range edge
+    # 2^-767: a1 = a2 = 0 gives D2 = 2 (f h)^2 (selected with a3 = 1, while
+    # D3 < 0), a1 = a2 = a3 = 0 gives D3 = 4 (f h)^2 / 3 (selected), up to
+    # rounding, so f h around 2^-384 puts them at the edge
+    fhv = np.ldexp(1.0, np.arange(-400, -370))
+    fhv = np.concatenate([fhv, np.nextafter(fhv, 0), np.nextafter(fhv, 1)])
+    z = np.zeros_like(fhv)
+    e = np.concatenate([np.stack([z, z, a3 + z, fhv, z + 1.0], 1) for a3 in (0.0, 1.0, 1e300)])
+    e = np.concatenate([e, e[:, [2, 0, 1, 3, 4]]])
+    with np.errstate(all="ignore"):
+        want4 = hp.solution3d(e[:, 0], e[:, 1], e[:, 2], e[:, 3], e[:, 4])
+    got4 = host(A.selftest_arith(8, dev(e[:, :3].reshape(-1)), dev(e[:, 3:].reshape(-1))))
+    assert bits_equal(got4, want4)
 
 
 # ------------------------------------------------------------ golden ---

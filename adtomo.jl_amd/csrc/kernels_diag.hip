@@ -544,7 +544,8 @@ __device__ __forceinline__ void publish_progress(int* prog, int w, int done) {
 // values are stored at the start of the next step, before any load is
 // waited for.  So the compiler's wait counts stay exact on every path into
 // the loop body: landing a load never waits for a store issued after it.
-template <typename T, bool FULL, bool MIX, bool ERR, bool CH>
+// FC: the sweep runs c downwards (flips the k axis).
+template <typename T, bool FULL, bool MIX, bool ERR, bool CH, bool FC>
 __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
                                                     lptr<T> xch, lptr<T> xold, lptr<T> planes,
                                                     int* prog, int* abort_flag) {
@@ -554,7 +555,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   const int L = g.L, Lw = g.Lw;
   if (FULL) cnt = kLW;
   const int c0 = w * Lw;      // canonical start of this wave's segment
-  const bool fwd = !S.flipC;  // the segment runs forward in memory
+  constexpr bool fwd = !FC;  // the segment runs forward in memory
   const bool lane_first = (p == 0);
   const bool lane_last = (p == g.H - 1);
   const bool has_x = c0 + Lw < L && w + 1 < S.nseg;  // a node exists one past the segment
@@ -692,6 +693,11 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : XCM;
       const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : XCP;
+      // canonical c-1 of the segment's first node / c+1 of its last node
+      // lie outside the grid: the axis minimum then takes the one existing
+      // neighbour (Eikonal3D.cpp:51-52), as smin(v, v) = v bit for bit
+      const bool c_first = w == 0 && S.cs == 0;
+      const bool c_last = !has_x && S.cs + L == S.Lfull;
       const int ra = cv.fa() ? g.Ma - 1 - cv.a : cv.a;
       const int rb = cv.fb() ? g.Nb - 1 - cv.b : cv.b;
       const bool af = ra == 0, al = ra == g.Ma - 1;
@@ -733,20 +739,21 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
             }
           }
           if (FULL || e < cnt) {
-            const int c = c0 + e;
             const int rw = e * P::RS;
             // std::min(u(+1), u(-1)) per axis, one-sided at the edges
             // (Eikonal3D.cpp:47-52)
             const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
             const T fe = pf[rw + p + 1];
-            const T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
-            const T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
+            T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
+            T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
+            if (e == 0 && c_first) uc = dc;
+            if ((FULL ? e == kLW - 1 : e == cnt - 1) && c_last) dc = uc;
             if (cv.valid()) {
-              const int cg = S.cs + c;  // canonical c in the whole event
-              const int rc = S.flipC ? S.Lfull - 1 - cg : cg;
               const T ux = smin(xp, xm);
               const T uy = smin(yp, ym);
-              const T mc = axis_min(uc, dc, S.flipC, rc == 0, rc == S.Lfull - 1);
+              // std::min(u(k+1), u(k-1)): canonical c+1 is k+1 unless the
+              // sweep runs k downwards
+              const T mc = FC ? smin(uc, dc) : smin(dc, uc);
               T cand;
               if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
                 cand = (T)solution3d<double, FAST>(ux, uy, mc, fe, S.hd, unsafe);
@@ -806,12 +813,17 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
   }
   int cnt = S.g.L - w * S.g.Lw;  // nodes in this wave's segment
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
-  if (S.Lfull > S.nseg * kLW) {
-    if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, true>(S, w, cnt, xch, xold, planes, prog, abort_flag);
-    return diag_sweep_wave<T, false, MIX, ERR, true>(S, w, cnt, xch, xold, planes, prog, abort_flag);
-  }
-  if (cnt == kLW) return diag_sweep_wave<T, true, MIX, ERR, false>(S, w, cnt, xch, xold, planes, prog, abort_flag);
-  return diag_sweep_wave<T, false, MIX, ERR, false>(S, w, cnt, xch, xold, planes, prog, abort_flag);
+  auto run = [&](auto full, auto ch, auto fc) {
+    return diag_sweep_wave<T, decltype(full)::value, MIX, ERR, decltype(ch)::value,
+                           decltype(fc)::value>(S, w, cnt, xch, xold, planes, prog, abort_flag);
+  };
+  auto by_c = [&](auto full, auto ch) {
+    return S.flipC ? run(full, ch, std::true_type{}) : run(full, ch, std::false_type{});
+  };
+  auto by_len = [&](auto ch) {
+    return cnt == kLW ? by_c(std::true_type{}, ch) : by_c(std::false_type{}, ch);
+  };
+  return S.Lfull > S.nseg * kLW ? by_len(std::true_type{}) : by_len(std::false_type{});
 }
 
 // The whole forward of one event, run by one workgroup of kDW waves.

@@ -44,7 +44,7 @@ template <> struct Real<double> {
     return __builtin_fma(d, h, g);
   }
   static __device__ __forceinline__ bool risky_(double x) {
-    return __builtin_fabs(x) < 0x1p-767 || x == __builtin_inf();
+    return (int)(__builtin_fabs(x) < 0x1p-767) | (int)(x == __builtin_inf());
   }
   static constexpr bool kFastSqrt = true;
   static __device__ __forceinline__ double fabs_(double x) { return __builtin_fabs(x); }
@@ -129,7 +129,7 @@ __device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h, bool& unsafe
   if constexpr (FAST && Real<T>::kFastSqrt) {
     s2 = Real<T>::refine_(D2);
     s3 = Real<T>::refine_(D3);
-    unsafe = unsafe || Real<T>::risky_(D2) || Real<T>::risky_(D3);
+    unsafe = (int)unsafe | (int)Real<T>::risky_(D2) | (int)Real<T>::risky_(D3);
   } else {
     s2 = Real<T>::sqrt_(D2);
     s3 = Real<T>::sqrt_(D3);

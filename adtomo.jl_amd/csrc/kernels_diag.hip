@@ -805,7 +805,10 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
                                                lptr<T> planes, int* prog, int* abort_flag) {
   // the wave index is uniform across the wave: keep it (and everything
   // derived from it) in scalar registers
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // with all kDW segments active, waves wv and wv + kDW/2 (which the
+  // hardware places on one SIMD) take neighbouring segments
+  const int w = S.nseg == kDW ? (wv % (kDW / 2)) * 2 + wv / (kDW / 2) : wv;
   if (w >= S.nseg) {  // the prologue's and the end's barriers of the active waves
     __syncthreads();
     __syncthreads();

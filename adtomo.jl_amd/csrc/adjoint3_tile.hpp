@@ -240,6 +240,7 @@ __device__ int phase_codes(const Ev<T>& E) {
     const T kprev = k0 > 0 ? E.u[q0 - 1] : T(0);
     const T knext = k0 + kT < l ? E.u[q0 + kT] : T(0);
     uint64_t codes = 0;
+    T gz[kT];  // grad_u0 of the row, stored in 16-byte pieces where aligned
 #pragma unroll
     for (int e = 0; e < kT; ++e) {
       const int k = k0 + e;
@@ -248,9 +249,10 @@ __device__ int phase_codes(const Ev<T>& E) {
       const int co[3] = {i, j, k};
       const uint32_t code = k < l ? code3<T>(own[e], u0r[e], nb, co, ex) : kCZ;
       codes |= (uint64_t)code << (8 * e);
+      gz[e] = T(0);
       if (k < l) {
         const int64_t q = q0 + e;
-        if (E.gu0) E.gu0[q] = (own[e] == u0r[e]) ? E.g[q] : T(0);
+        if (E.gu0 && own[e] == u0r[e]) gz[e] = E.g[q];
         if (code & kZ) {
           const T rhs = T(-2.0) * E.f[q] * E.h * E.h;
           E.gf[q] = -T(0) * rhs;
@@ -265,6 +267,37 @@ __device__ int phase_codes(const Ev<T>& E) {
 #pragma unroll
       for (int e = 0; e < kT; ++e)
         if (k0 + e < l) E.code[q0 + e] = (uint8_t)(codes >> (8 * e));
+    }
+    if (E.gu0) {
+      constexpr int kPer = 16 / (int)sizeof(T);
+      if (k0 + kT <= l && !(q0 & (kPer - 1))) {
+        using v4 = __attribute__((ext_vector_type(4))) unsigned int;
+        gptr<v4> dst = (gptr<v4>)(E.gu0 + q0);
+#pragma unroll
+        for (int c = 0; c < kT / kPer; ++c) {
+          unsigned ww[4];
+#pragma unroll
+          for (int e = 0; e < kPer; ++e) {
+            if constexpr (sizeof(T) == 8) {
+              const uint64_t b = (uint64_t)__double_as_longlong(gz[c * kPer + e]);
+              ww[2 * e] = (unsigned)b;
+              ww[2 * e + 1] = (unsigned)(b >> 32);
+            } else {
+              ww[e] = __float_as_uint(gz[c * kPer + e]);
+            }
+          }
+          v4 w;
+          w.x = ww[0];
+          w.y = ww[1];
+          w.z = ww[2];
+          w.w = ww[3];
+          dst[c] = w;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < kT; ++e)
+          if (k0 + e < l) E.gu0[q0 + e] = gz[e];
+      }
     }
   }
   return nonz;

@@ -206,6 +206,47 @@ def test_forward3d_column_mappings(shape):
     assert bits_equal(host(u), u_ref)
 
 
+def _tiny_slowness_events():
+    # discriminants below the fast square root's range (2^-767): a block of
+    # slowness 1e-170 (travel times flat across it, D = 2 (f h)^2 or a
+    # difference of nearly equal values: about half of all updates, counted
+    # on the CPU), and an all-zero field (every update); k segments of 2 and
+    # 8 nodes, chained and single sweeps
+    out = []
+    for shape, seed in (((20, 24, 16), 5), ((40, 48, 64), 6)):
+        r = synth.rng(seed)
+        f = r.uniform(0.3, 3.0, size=shape)
+        f[:, : shape[1] // 2, :] = 1e-170
+        out.append((synth.point_source_u0(shape, (3, 5, 7)), f))
+    out.append((np.zeros((24, 20, 64)), np.zeros((24, 20, 64))))
+    return out
+
+
+def test_forward_redo_on_tiny_discriminants():
+    # the step loops' exact redo (the wave recomputes a step with the IEEE
+    # square root when a lane's discriminant lies outside the fast range)
+    import adtomo_hip as A
+    ev = _tiny_slowness_events()
+    refs = [oracle.eikonal3d(u0, f, 0.5, 1e-4) for u0, f in ev]
+    for (u0, f), (u_ref, it, err) in zip(ev, refs):
+        u = A.eikonal3d(dev(u0), dev(f), 0.5, *u0.shape, 1e-4, False)
+        assert A.ops.last_info["iterations"] == it, u0.shape
+        assert bits_equal(host(u), u_ref), u0.shape
+    res = A.eikonal3d_batched([dev(u0) for u0, _ in ev], [dev(f) for _, f in ev],
+                              [0.5] * len(ev), [1e-4] * len(ev))
+    for e, (u_ref, it, _) in enumerate(refs):
+        assert res["info"][e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), u_ref), e
+    # FP32 storage, FP64 solves
+    u32 = [dev(u0, np.float32) for u0, _ in ev]
+    f32 = [dev(f, np.float32) for _, f in ev]
+    res = A.eikonal3d_batched(u32, f32, [0.5] * len(ev), [1e-4] * len(ev), compute_f64=True)
+    for e, (u0, f) in enumerate(ev):
+        um, it, _ = oracle.eikonal3d_m32(u0, f, 0.5, 1e-4)
+        assert res["info"][e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), um), e
+
+
 @pytest.mark.parametrize("tol", [1e-3, 1e-6])
 def test_config2_forward_adjoint(tol):
     # 64x64x32, h=1, 4->8 km/s layered x (1 +/- 5%) checkerboard (period 16),

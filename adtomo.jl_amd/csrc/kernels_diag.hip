@@ -53,9 +53,11 @@
 // columns of diagonals <= (bands-1) N_b + H), or flips b and bands >= 2
 // (they are sweep k's first band).  In-plane neighbours across the lane
 // boundary of the two sweeps are grid edges, whose one-sided minimum never
-// reads them.  Chains stop at the last sweep of an iteration (it takes the
-// stop rule), at the first sweep of the first iteration (it reads u0), at
-// a flip of c, and for events swept in several k chunks.
+// reads them.  Chains stop at the first sweep of the first iteration (it
+// reads u0), at a flip of c, and for events swept in several k chunks.  A
+// chain may end with the last sweep of an iteration (it takes the stop
+// rule): the steps that reach it store its lanes' values to the next
+// snapshot and fold their error, lane by lane.
 //
 // Iteration (Eikonal3D.cpp:71-88): u = u0 is a copy pass at the start;
 // err = max|u - u_old| is a coalesced pass after the eighth sweep against
@@ -106,11 +108,12 @@ __device__ inline DGeom make_dgeom(int m, int n, int l) {
 // a lane mask in scalar registers, which the step loop cannot spare).
 struct DCol {
   int a, b;
-  uint32_t fl;  // bit 0: valid, bit 1: fa, bit 2: fb
+  uint32_t fl;  // bit 0: valid, bit 1: fa, bit 2: fb, bit 3: the chain's last sweep
   int node0;
   __device__ __forceinline__ bool valid() const { return fl & 1; }
   __device__ __forceinline__ bool fa() const { return fl & 2; }
   __device__ __forceinline__ bool fb() const { return fl & 4; }
+  __device__ __forceinline__ bool last() const { return fl & 8; }
 };
 
 constexpr int kChainMax = 4;  // sweeps of one chain (the c direction fixed)
@@ -188,7 +191,7 @@ __device__ __forceinline__ DCol cur_col(const DSweep<T>& S, const DCur& c, int p
   col.a = c.beta * g.H + p;
   col.b = c.b;
   const bool valid = (p < g.H) && (c.b >= 0) && (c.k < S.nch) && (col.a < g.Ma);
-  col.fl = (fl << 1) | (valid ? 1u : 0u);
+  col.fl = (fl << 1) | (valid ? 1u : 0u) | (c.k == S.nch - 1 ? 8u : 0u);
   const int ra = (fl & 1) ? g.Ma - 1 - col.a : col.a;
   const int rb = (fl & 2) ? g.Nb - 1 - col.b : col.b;
   col.node0 = S.coff + ra * g.sA + rb * g.sB;
@@ -372,15 +375,17 @@ __device__ __forceinline__ void seg_store(lptr<const T> pl, rsrc_t r, int seg, i
   }
 }
 
-// seg_store for the last sweep of an iteration: also stores the values to
-// the next snapshot and folds max |new - snapshot| into `key` (the stop
-// rule's err, Eikonal3D.cpp:79-81), `snap` holding the snapshot's values at
-// the same offsets (loaded with seg_issue one step earlier).  The stores are
-// issued before the snapshot values are waited for.
+// seg_store for the last sweep of an iteration: the segments of lanes on
+// that sweep (segs: seg, or -1 for lanes still on the sweep before it in a
+// chain) also go to the next snapshot, and max |new - snapshot| over them is
+// folded into `key` (the stop rule's err, Eikonal3D.cpp:79-81), `snap`
+// holding the snapshot's values at the same offsets (loaded with seg_issue
+// one step earlier).  The stores are issued before the snapshot values are
+// waited for.
 template <typename T, bool FULL>
 __device__ __forceinline__ void seg_store_err(lptr<const T> pl, rsrc_t r, rsrc_t sdst, int seg,
-                                              int cnt, bool fwd, int p, const Flight<T>& snap,
-                                              uint64_t& key) {
+                                              int segs, int cnt, bool fwd, int p,
+                                              const Flight<T>& snap, uint64_t& key) {
   using P = Pieces<T>;
   if (FULL) cnt = kLW;
   T x[P::LPC][P::EP];
@@ -390,17 +395,20 @@ __device__ __forceinline__ void seg_store_err(lptr<const T> pl, rsrc_t r, rsrc_t
     const int q = i * P::CPI + p / P::LPC;
     const int m0 = (p % P::LPC) * P::EP;
     const int sq = __shfl(seg, q);
-    ok[i] = sq >= 0 && (FULL || m0 + P::EP <= cnt);
+    const int ss = __shfl(segs, q);
+    const bool whole = FULL || m0 + P::EP <= cnt;
+    ok[i] = ss >= 0 && whole;
 #pragma unroll
     for (int e = 0; e < P::EP; ++e) {
       const int m = m0 + e;
       const int j = (FULL || m < cnt) ? canon<FULL>(m, cnt, fwd) : kLW;
       x[i][e] = pl[j * P::RS + q + 1];
     }
-    const uint32_t off = ok[i] ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB;
+    const uint32_t off = (sq >= 0 && whole) ? (uint32_t)(sq + m0) * (uint32_t)sizeof(T) : kOOB;
+    const uint32_t offs = ok[i] ? (uint32_t)(ss + m0) * (uint32_t)sizeof(T) : kOOB;
     const u32x4 v = piece_make<T>(x[i]);
     bstore4(r, off, v);
-    bstore4(sdst, off, v);
+    bstore4(sdst, offs, v);
   }
   T xt[P::EP - 1];
   if (!FULL) {
@@ -410,9 +418,8 @@ __device__ __forceinline__ void seg_store_err(lptr<const T> pl, rsrc_t r, rsrc_t
       const int m = nfull + t;
       const int j = m < cnt ? canon<FULL>(m, cnt, fwd) : kLW;
       xt[t] = pl[j * P::RS + p + 1];
-      const uint32_t off = boff<T>(seg >= 0 && m < cnt, (int64_t)seg + m);
-      bstore(r, off, xt[t]);
-      bstore(sdst, off, xt[t]);
+      bstore(r, boff<T>(seg >= 0 && m < cnt, (int64_t)seg + m), xt[t]);
+      bstore(sdst, boff<T>(segs >= 0 && m < cnt, (int64_t)segs + m), xt[t]);
     }
   }
 #pragma unroll
@@ -428,7 +435,7 @@ __device__ __forceinline__ void seg_store_err(lptr<const T> pl, rsrc_t r, rsrc_t
 #pragma unroll
     for (int t = 0; t < P::EP - 1; ++t) {
       const uint64_t k = maxkey(Real<T>::fabs_(xt[t] - snap.tail[t]));
-      key = (seg >= 0 && nfull + t < cnt && k > key) ? k : key;
+      key = (segs >= 0 && nfull + t < cnt && k > key) ? k : key;
     }
   }
 }
@@ -647,15 +654,21 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
 
   const int steps = (S.nch - 1) * S.V0 + g.V + S.nseg - 1;
   const int vend = (S.nch - 1) * S.V0 + g.V;  // chain diagonals
-  int seg_prev = -1;  // segment of the previous step's new values (-1: none)
-  for (int t = 0; t < steps; ++t) {
+  int seg_prev = -1;   // segment of the previous step's new values (-1: none)
+  int segs_prev = -1;  // ... if its column is on the chain's last sweep
+  // ES: a step of the stop-rule sweep (ERR), which the chain reaches at
+  // diagonal (nch-1) V0 (this wave's step (nch-1) V0 + w); earlier steps
+  // of its chain take the plain step
+  auto step = [&](int t, auto es) {
+    constexpr bool ES = decltype(es)::value;
     const int v = v0 + t;
     const bool active = v >= 0 && v < vend;
     // new values of v-1 (plane n) to memory
-    if constexpr (ERR) {
-      seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, cnt, fwd, p, FS, key);
-      // snapshot values of diagonal v, stored at the next step
-      seg_issue<T, FULL>(FS, S.snap_src, active ? seg_of(cv) : -1, cnt, p);
+    if constexpr (ES) {
+      seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, segs_prev, cnt, fwd, p, FS, key);
+      // snapshot values of diagonal v (its last-sweep lanes), stored at the
+      // next step
+      seg_issue<T, FULL>(FS, S.snap_src, (active && cv.last()) ? seg_of(cv) : -1, cnt, p);
     } else {
       seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
     }
@@ -778,17 +791,24 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     // node 0 of old(v+2), which the wave upstream needs at step T+1
     seg_publish<T, FULL>(FD, xold + (t & 1) * kDW * 64 + w * 64, cnt, fwd, p);
     seg_prev = active ? seg_of(cv) : -1;
+    segs_prev = cv.last() ? seg_prev : -1;
     xch[(t & 1) * kDW * 64 + w * 64 + p] = xo;
     cv = cv1;
     cv1 = cv2;
     cv2 = cur_col(S, cur, p);  // diagonal v + 3
     cur_next(S, cur);
     if (p == 0) publish_progress(prog, w, t + 1);
-  }
-  if constexpr (ERR)
-    seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, cnt, fwd, p, FS, key);
-  else
+  };
+  int t = 0;
+  if constexpr (ERR) {
+    const int t1 = (S.nch - 1) * S.V0 + w;
+    for (; t < steps && t < t1; ++t) step(t, std::false_type{});
+    for (; t < steps; ++t) step(t, std::true_type{});
+    seg_store_err<T, FULL>(pn, S.u, S.snap_dst, seg_prev, segs_prev, cnt, fwd, p, FS, key);
+  } else {
+    for (; t < steps; ++t) step(t, std::false_type{});
     seg_store<T, FULL>(pn, S.u, seg_prev, cnt, fwd, p);
+  }
   __syncthreads();  // the next sweep reads these values
   return key;
 }
@@ -871,7 +891,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   S.V0 = S.g.bands * S.g.Nb;
   // may sweep sw+1 run chained after sweep sw (see "Chained sweeps")?
   auto chains = [&](int it_, int sw) -> bool {
-    if (sw + 1 >= 7 || (it_ == 0 && sw == 0) || nchunk != 1) return false;
+    if (sw + 1 > 7 || (it_ == 0 && sw == 0) || nchunk != 1) return false;
     const int flip = kDir3[sw] ^ kDir3[sw + 1];
     if (flip & 4) return false;
     const bool flips_a = (flip & 1) ? S.g.a_is_i : !S.g.a_is_i;
@@ -889,7 +909,8 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
       // u = u0 (Eikonal3D.cpp:72): the first sweep reads its old values
       // from u0; every node is written once per sweep
       S.src = (it == 0 && sw == 0) ? S.u0 : S.u;
-      if (sw == 7) S.snap_src = make_rsrc(it == 0 ? ev.u0 : ev.uold, bytes);
+      const bool last = sw + nch - 1 == 7;  // the chain ends with the stop-rule sweep
+      if (last) S.snap_src = make_rsrc(it == 0 ? ev.u0 : ev.uold, bytes);
       for (int ch = 0; ch < nchunk; ++ch) {
         S.cs = ch * kChunk;
         const int Lc = l - S.cs < kChunk ? l - S.cs : kChunk;
@@ -903,7 +924,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
           S.cfl |= ((fa ? 1 : 0) | (fb ? 2 : 0)) << (2 * q);
         }
         S.coff = S.flipC ? l - 1 - S.cs : S.cs;
-        if (sw < 7) {
+        if (!last) {
           diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes, prog,
                                     &abort_flag);
         } else {

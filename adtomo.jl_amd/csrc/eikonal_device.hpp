@@ -48,6 +48,22 @@ template <> struct Real<double> {
   }
   static constexpr bool kFastSqrt = true;
   static __device__ __forceinline__ double fabs_(double x) { return __builtin_fabs(x); }
+  // Minimum / maximum in one instruction (__builtin_fmin adds a NaN-quieting
+  // instruction per loaded operand).  They equal smin / smax bit for bit
+  // unless an operand is NaN or the operands are zeros of opposite signs.
+  static __device__ __forceinline__ double min1_(double x, double y) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+  }
+  static __device__ __forceinline__ double max1_(double x, double y) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+  }
+  // NaN or -0 (class bits sNaN, qNaN, -0): the values on which min1_ /
+  // max1_ may differ from the compare-and-select forms
+  static __device__ __forceinline__ bool odd_(double x) { return __builtin_amdgcn_class(x, 0x23); }
 };
 template <> struct Real<float> {
   static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
@@ -55,6 +71,17 @@ template <> struct Real<float> {
   static __device__ __forceinline__ bool risky_(float) { return false; }
   static constexpr bool kFastSqrt = false;
   static __device__ __forceinline__ float fabs_(float x) { return __builtin_fabsf(x); }
+  static __device__ __forceinline__ float min1_(float x, float y) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+  }
+  static __device__ __forceinline__ float max1_(float x, float y) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+  }
+  static __device__ __forceinline__ bool odd_(float x) { return __builtin_amdgcn_classf(x, 0x23); }
 };
 
 template <typename T> __device__ __forceinline__ T smin(T x, T y) { return (y < x) ? y : x; }
@@ -107,15 +134,27 @@ __device__ __forceinline__ T div3(T x) { return Div3<T>::go(x); }
 // The three candidates (1D, 2D, 3D) are evaluated as written and the first
 // that the reference would return is selected: nearly every update of a wave
 // needs the 3D one, so evaluating all of them costs nothing and lets the 2D
-// and 3D square roots run side by side.  FAST (FP64): the square roots
-// through Real::refine_, and `unsafe` set for a lane whose discriminants
-// it may get wrong; the caller then redoes the update with FAST off.
+// and 3D square roots run side by side.  FAST: the three values sorted by
+// minima and maxima (the reference's compare-and-swap order gives the same
+// bits when none of a1, a2, a3 is NaN or -0, which the caller checks), and
+// in FP64 the square roots through Real::refine_, with `unsafe` set for a
+// lane whose discriminants they may get wrong; the caller then redoes the
+// update with FAST off.
 template <typename T, bool FAST = false>
 __device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h, bool& unsafe) {
-  T temp;
-  if (a1 > a2) { temp = a1; a1 = a2; a2 = temp; }
-  if (a1 > a3) { temp = a1; a1 = a3; a3 = temp; }
-  if (a2 > a3) { temp = a2; a2 = a3; a3 = temp; }
+  if constexpr (FAST) {
+    // lo = min, hi = max, median = max(min(a1, a2), min(max(a1, a2), a3))
+    const T mn = Real<T>::min1_(a1, a2), mx = Real<T>::max1_(a1, a2);
+    const T md = Real<T>::max1_(mn, Real<T>::min1_(mx, a3));
+    a1 = Real<T>::min1_(mn, a3);
+    a3 = Real<T>::max1_(mx, a3);
+    a2 = md;
+  } else {
+    T temp;
+    if (a1 > a2) { temp = a1; a1 = a2; a2 = temp; }
+    if (a1 > a3) { temp = a1; a1 = a3; a3 = temp; }
+    if (a2 > a3) { temp = a2; a2 = a3; a3 = temp; }
+  }
   const T x1 = a1 + f * h;
   const T B2 = -(a1 + a2);
   const T C2 = (a1 * a1 + a2 * a2 - f * f * h * h) / T(2.0);

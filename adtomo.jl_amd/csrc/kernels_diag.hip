@@ -440,6 +440,23 @@ __device__ __forceinline__ void seg_store_err(lptr<const T> pl, rsrc_t r, rsrc_t
   }
 }
 
+// Does a transfer in flight (once arrived) hold NaN or -0?  Masked loads
+// read +0.
+template <typename T>
+__device__ __forceinline__ bool flight_odd(const Flight<T>& fl, bool full) {
+  using P = Pieces<T>;
+  bool odd = false;
+#pragma unroll
+  for (int i = 0; i < P::LPC; ++i)
+#pragma unroll
+    for (int e = 0; e < P::EP; ++e) odd = (int)odd | (int)Real<T>::odd_(piece_get<T>(fl.pc[i], e));
+  if (!full) {
+#pragma unroll
+    for (int t = 0; t < P::EP - 1; ++t) odd = (int)odd | (int)Real<T>::odd_(fl.tail[t]);
+  }
+  return odd;
+}
+
 // Publishes canonical node 0 of every lane's segment from a transfer in
 // flight (once it has arrived) to dst[lane]: the old value the wave
 // upstream along k needs one past its own segment.
@@ -643,6 +660,10 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   // node 0 of old(v0+1) published for the wave upstream (its step 0)
   seg_issue<T, FULL>(FD, S.src, seg_of(cv), cnt, p);
   seg_land<T, FULL>(FD, px, cnt, fwd, p);
+  // a step whose inputs hold NaN or -0 takes the exact node pass (the fast
+  // one orders such values differently); so does the step after it, which
+  // reads the values that step landed or produced
+  int redo_prev = __builtin_amdgcn_ballot_w64(flight_odd(FD, FULL)) != 0;
   seg_issue<T, FULL>(FD, S.src, seg_of(cv1), cnt, p);
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
   issue_edge(cv);
@@ -677,14 +698,20 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
 #pragma unroll
     for (int j = 0; j < kLW; ++j)
       if (FULL || j < cnt) OWN[j] = px[j * P::RS + p + 1];
+    bool odd = flight_odd(FD, FULL);
     seg_land<T, FULL>(FD, px, cnt, fwd, p);
     seg_land<T, FULL>(FF, pf, cnt, fwd, p);
     XCM = XCM_n;
     XCP = XCP_n;
     // band-edge neighbours of v at lanes -1 (plane n) and H (plane x)
     if constexpr (FULL) {
+#pragma unroll
+      for (int e = 0; e < P::EP; ++e) odd = (int)odd | (int)Real<T>::odd_(piece_get<T>(EF, e));
       edge_land<T>(EF, pn, px, g.H, fwd, p);
     } else {
+#pragma unroll
+      for (int j = 0; j < kLW; ++j)
+        if (j < cnt) odd = (int)odd | (int)Real<T>::odd_(LE[j]);
       if (lane_first) {
 #pragma unroll
         for (int j = 0; j < kLW; ++j)
@@ -733,11 +760,14 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       // selects become one address choice per step.
       const lptr<T> x_plus = xl ? x_minus0 : x_plus0, x_minus = xf ? x_plus0 : x_minus0;
       const lptr<T> y_plus = yl ? y_minus0 : y_plus0, y_minus = yf ? y_plus0 : y_minus0;
-      // The node updates take the fast square roots (Real::refine_); if a
-      // lane met a discriminant they may get wrong, the whole wave redoes the
-      // step's updates from the old values (still in memory: this step's new
-      // values are stored at the next step) with the IEEE square root.
-      bool unsafe = false;
+      // The node updates take single-instruction minima and the fast square
+      // roots (Real::refine_).  If a lane met a discriminant the square
+      // roots may get wrong, or NaN or -0 among the values the minima order
+      // (landed this step, produced by the previous step or by this one),
+      // the whole wave redoes the step's updates from the old values (still
+      // in memory: this step's new values are stored at the next step) with
+      // the reference's comparisons and the IEEE square root.
+      bool unsafe = (int)odd | redo_prev | (int)Real<T>::odd_(xc_in) | (int)Real<T>::odd_(xd_in);
       auto nodes = [&](auto fast, auto issue) {
         constexpr bool FAST = decltype(fast)::value, ISSUE = decltype(issue)::value;
 #pragma unroll
@@ -762,24 +792,26 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
             if (e == 0 && c_first) uc = dc;
             if ((FULL ? e == kLW - 1 : e == cnt - 1) && c_last) dc = uc;
             if (cv.valid()) {
-              const T ux = smin(xp, xm);
-              const T uy = smin(yp, ym);
+              const T ux = FAST ? Real<T>::min1_(xp, xm) : smin(xp, xm);
+              const T uy = FAST ? Real<T>::min1_(yp, ym) : smin(yp, ym);
               // std::min(u(k+1), u(k-1)): canonical c+1 is k+1 unless the
               // sweep runs k downwards
-              const T mc = FC ? smin(uc, dc) : smin(dc, uc);
+              const T mc = FAST ? Real<T>::min1_(uc, dc) : FC ? smin(uc, dc) : smin(dc, uc);
               T cand;
               if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
                 cand = (T)solution3d<double, FAST>(ux, uy, mc, fe, S.hd, unsafe);
               else
                 cand = solution3d<T, FAST>(ux, uy, mc, fe, S.h, unsafe);
               OWN[e] = smin(cand, OWN[e]);
+              if constexpr (FAST) unsafe = (int)unsafe | (int)Real<T>::odd_(OWN[e]);
             }
             if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
           }
         }
       };
       nodes(std::true_type{}, std::true_type{});
-      if (__builtin_expect(__builtin_amdgcn_ballot_w64(unsafe) != 0, 0)) {
+      redo_prev = __builtin_amdgcn_ballot_w64(unsafe) != 0;
+      if (__builtin_expect(redo_prev, 0)) {
         load_seg(OWN, S.src, cv, 0, cv.valid(), c0, cnt, L, S.tC);
         nodes(std::false_type{}, std::false_type{});
       }

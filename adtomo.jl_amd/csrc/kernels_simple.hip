@@ -325,7 +325,10 @@ __global__ void arith_kernel(int op, const T* __restrict__ a, const T* __restric
       case 5: r = solution2d(a[2 * p], a[2 * p + 1], b[2 * p], b[2 * p + 1]); break;
       case 7: r = Real<T>::risky_(x) ? Real<T>::sqrt_(x) : Real<T>::refine_(x); break;
       case 8: {
-        bool unsafe = false;
+        // inputs the fast sort may order differently are redone, as in the
+        // step loops
+        bool unsafe = (int)Real<T>::odd_(a[3 * p]) | (int)Real<T>::odd_(a[3 * p + 1]) |
+                      (int)Real<T>::odd_(a[3 * p + 2]);
         r = solution3d<T, true>(a[3 * p], a[3 * p + 1], a[3 * p + 2], b[2 * p], b[2 * p + 1],
                                 unsafe);
         if (unsafe)

@@ -399,8 +399,9 @@ int adt_sum_arrays_f32(const float *const *arrays, int32_t narr, int64_t N, floa
  *    read and written once, for calibrating memory counters);
  * 7: sqrt(a) as the forward's step loops take it (the fast refinement, the
  *    IEEE routine where it flags the operand);
- * 8: op 4 as the forward's step loops take it (fast square roots, the
- *    update redone with IEEE ones where they flag a discriminant). */
+ * 8: op 4 as the forward's step loops take it (the values sorted by
+ *    minima and maxima, fast square roots; the update redone as op 4 where
+ *    an input is NaN or -0 or a discriminant is flagged). */
 int adt_selftest_arith_f64(int32_t op, const double *a, const double *b, double *out,
                            int64_t n, void *stream);
 int adt_selftest_arith_f32(int32_t op, const float *a, const float *b, float *out, int64_t n,

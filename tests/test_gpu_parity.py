@@ -247,6 +247,65 @@ def test_forward_redo_on_tiny_discriminants():
         assert bits_equal(host(res["u"][e]), um), e
 
 
+def _odd_value_events():
+    # NaN and -0 in the fields: the step loops' single-instruction minima
+    # order such values differently from the reference's comparisons, so a
+    # step that lands, receives or produces one is redone exactly.  -0 at the
+    # source and at two other nodes (they stay -0); a NaN block of slowness
+    # (NaN travel times spread from it), a NaN and a +inf start value
+    out = []
+    r = synth.rng(31)
+    shape = (20, 24, 16)
+    f = r.uniform(0.3, 3.0, size=shape)
+    u0 = synth.point_source_u0(shape, (3, 5, 7))
+    u0[3, 5, 7] = u0[10, 10, 3] = u0[15, 2, 12] = -0.0
+    out.append((u0, f))
+    r = synth.rng(32)
+    shape = (40, 48, 64)
+    f = r.uniform(0.3, 3.0, size=shape)
+    u0 = synth.point_source_u0(shape, (30, 9, 40))
+    f[:6, :7, :9] = np.nan
+    u0[20, 30, 5] = np.nan
+    u0[35, 40, 60] = np.inf
+    u0[30, 9, 40] = -0.0
+    out.append((u0, f))
+    return out
+
+
+def test_forward_nan_and_negative_zero_inputs():
+    import adtomo_hip as A
+    ev = _odd_value_events()
+    refs = [oracle.eikonal3d(u0, f, 0.5, 1e-4) for u0, f in ev]
+    assert np.isnan(refs[1][0]).sum() > 100 and np.signbit(refs[0][0][refs[0][0] == 0]).sum() == 3
+    for (u0, f), (u_ref, it, err) in zip(ev, refs):
+        u = A.eikonal3d(dev(u0), dev(f), 0.5, *u0.shape, 1e-4, False)
+        assert A.ops.last_info["iterations"] == it, u0.shape
+        assert bits_equal(host(u), u_ref), u0.shape
+        assert np.array_equal(np.signbit(host(u)), np.signbit(u_ref)), u0.shape
+    res = A.eikonal3d_batched([dev(u0) for u0, _ in ev], [dev(f) for _, f in ev],
+                              [0.5] * len(ev), [1e-4] * len(ev))
+    for e, (u_ref, it, _) in enumerate(refs):
+        assert res["info"][e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), u_ref), e
+        assert np.array_equal(np.signbit(host(res["u"][e])), np.signbit(u_ref)), e
+    u32 = [dev(u0, np.float32) for u0, _ in ev]
+    f32 = [dev(f, np.float32) for _, f in ev]
+    # FP32 storage, FP64 solves
+    res = A.eikonal3d_batched(u32, f32, [0.5] * len(ev), [1e-4] * len(ev), compute_f64=True)
+    for e, (u0, f) in enumerate(ev):
+        um, it, _ = oracle.eikonal3d_m32(u0, f, 0.5, 1e-4)
+        assert res["info"][e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), um), e
+        assert np.array_equal(np.signbit(host(res["u"][e])), np.signbit(um)), e
+    # FP32
+    res = A.eikonal3d_batched(u32, f32, [0.5] * len(ev), [1e-4] * len(ev))
+    for e, (u0, f) in enumerate(ev):
+        u_32, it, _ = oracle.eikonal3d(u0, f, 0.5, 1e-4, dtype=np.float32)
+        assert res["info"][e]["iterations"] == it, e
+        assert bits_equal(host(res["u"][e]), u_32), e
+        assert np.array_equal(np.signbit(host(res["u"][e])), np.signbit(u_32)), e
+
+
 @pytest.mark.parametrize("tol", [1e-3, 1e-6])
 def test_config2_forward_adjoint(tol):
     # 64x64x32, h=1, 4->8 km/s layered x (1 +/- 5%) checkerboard (period 16),

@@ -31,7 +31,9 @@ template <> struct Real<double> {
   // without the wrapper: the same bits for every x with |x| >= 2^-767 below
   // +inf (the wrapper is then the identity; negative x gives NaN in both).
   // risky_ flags the other operands (|x| < 2^-767, +inf), for which the
-  // caller takes __builtin_sqrt instead.
+  // caller takes __builtin_sqrt instead, and NaN (the fast local solve's
+  // discriminant is NaN where its shortcuts could differ from the
+  // reference's expressions).
   static __device__ __forceinline__ double refine_(double x) {
     const double r = __builtin_amdgcn_rsq(x);
     double g = x * r, h = r * 0.5;
@@ -44,7 +46,8 @@ template <> struct Real<double> {
     return __builtin_fma(d, h, g);
   }
   static __device__ __forceinline__ bool risky_(double x) {
-    return (int)(__builtin_fabs(x) < 0x1p-767) | (int)(x == __builtin_inf());
+    // class bits sNaN, qNaN, +inf
+    return (int)(__builtin_fabs(x) < 0x1p-767) | (int)__builtin_amdgcn_class(x, 0x203);
   }
   static constexpr bool kFastSqrt = true;
   static __device__ __forceinline__ double fabs_(double x) { return __builtin_fabs(x); }
@@ -118,6 +121,12 @@ template <> struct Div3<double> {
     const double r = __builtin_fmin(__builtin_fma(-3.0, q0, x), 0x1p1023);
     return __builtin_fma(r, y, q0);
   }
+  // Without the minimum: the same for finite x, NaN for x = +-inf.
+  static __device__ __forceinline__ double raw(double x) {
+    const double y = 0x1.5555555555555p-2;
+    const double q0 = x * y;
+    return __builtin_fma(__builtin_fma(-3.0, q0, x), y, q0);
+  }
 };
 template <> struct Div3<float> {
   static __device__ __forceinline__ float go(float x) {
@@ -155,6 +164,31 @@ __device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h, bool& unsafe
     if (a1 > a3) { temp = a1; a1 = a3; a3 = temp; }
     if (a2 > a3) { temp = a2; a2 = a3; a3 = temp; }
   }
+  if constexpr (FAST && Real<T>::kFastSqrt) {
+    // The reference's expressions (below) with three shortcuts that keep
+    // every unflagged result (D2, D3 >= 2^-767, finite, not NaN):
+    //   D2 = B2 B2 - 4 ((q - fhh) / 2) as fma(-2, q - fhh, B2 B2): the same
+    //     whenever halving q - fhh is exact; it is not only for
+    //     |q - fhh| < 2^-1021, which moves the exact difference by less than
+    //     2^-1019 and so changes no rounding of a D2 >= 2^-767;
+    //   D3 = B3 B3 - 4 C3 as fma(-4, C3, B3 B3): 4 C3 is exact unless it
+    //     overflows, and then D3 is +inf (flagged) or negative in both;
+    //   div3 without its minimum: the same for finite arguments, and an
+    //     infinite one makes D3 NaN (flagged).
+    const T x1 = a1 + f * h;
+    const T fhh = f * f * h * h;
+    const T q = a1 * a1 + a2 * a2;
+    const T B2 = -(a1 + a2);
+    const T B3 = Div3<T>::raw(T(-2.0) * (a1 + a2 + a3));
+    const T C3 = Div3<T>::raw(q + a3 * a3 - fhh);
+    const T D2 = __builtin_fma(T(-2.0), q - fhh, B2 * B2);
+    const T D3 = __builtin_fma(T(-4.0), C3, B3 * B3);
+    const T s2 = Real<T>::refine_(D2), s3 = Real<T>::refine_(D3);
+    unsafe = (int)unsafe | (int)Real<T>::risky_(D2) | (int)Real<T>::risky_(D3);
+    const T x2 = (-B2 + s2) / T(2.0);
+    const T x3 = (-B3 + s3) / T(2.0);
+    return (x1 <= a2) ? x1 : ((x2 <= a3) ? x2 : x3);
+  }
   const T x1 = a1 + f * h;
   const T B2 = -(a1 + a2);
   const T C2 = (a1 * a1 + a2 * a2 - f * f * h * h) / T(2.0);
@@ -164,15 +198,7 @@ __device__ __forceinline__ T solution3d(T a1, T a2, T a3, T f, T h, bool& unsafe
   const T B3 = div3(T(-2.0) * (a1 + a2 + a3));
   const T C3 = div3(a1 * a1 + a2 * a2 + a3 * a3 - f * f * h * h);
   const T D2 = B2 * B2 - T(4.0) * C2, D3 = B3 * B3 - T(4.0) * C3;
-  T s2, s3;
-  if constexpr (FAST && Real<T>::kFastSqrt) {
-    s2 = Real<T>::refine_(D2);
-    s3 = Real<T>::refine_(D3);
-    unsafe = (int)unsafe | (int)Real<T>::risky_(D2) | (int)Real<T>::risky_(D3);
-  } else {
-    s2 = Real<T>::sqrt_(D2);
-    s3 = Real<T>::sqrt_(D3);
-  }
+  const T s2 = Real<T>::sqrt_(D2), s3 = Real<T>::sqrt_(D3);
   const T x2 = (-B2 + s2) / T(2.0);
   const T x3 = (-B3 + s3) / T(2.0);
   return (x1 <= a2) ? x1 : ((x2 <= a3) ? x2 : x3);

@@ -802,8 +802,10 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
                 cand = (T)solution3d<double, FAST>(ux, uy, mc, fe, S.hd, unsafe);
               else
                 cand = solution3d<T, FAST>(ux, uy, mc, fe, S.h, unsafe);
-              OWN[e] = smin(cand, OWN[e]);
-              if constexpr (FAST) unsafe = (int)unsafe | (int)Real<T>::odd_(OWN[e]);
+              // a NaN or -0 candidate (which the single-instruction minimum
+              // would order differently) sends the step to the exact pass
+              if constexpr (FAST) unsafe = (int)unsafe | (int)Real<T>::odd_(cand);
+              OWN[e] = FAST ? Real<T>::min1_(cand, OWN[e]) : smin(cand, OWN[e]);
             }
             if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
           }

@@ -770,28 +770,29 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       bool unsafe = (int)odd | redo_prev | (int)Real<T>::odd_(xc_in) | (int)Real<T>::odd_(xd_in);
       auto nodes = [&](auto fast, auto issue) {
         constexpr bool FAST = decltype(fast)::value, ISSUE = decltype(issue)::value;
+        if constexpr (ISSUE) {
+          seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
+          seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
+          issue_edge(cv1);
+          XCM_n = load_cm(cv1);
+          XCP_n = load_cp(cv1);
+        }
+        // one branch around the whole segment (a lane's nodes share their
+        // column), so the updates form one block and the plane reads of a
+        // node can be issued during the updates before it
+        if (cv.valid()) {
 #pragma unroll
-        for (int e = 0; e < kLW; ++e) {
-          if constexpr (ISSUE) {
-            if (e == 0) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
-            if (e == 1) seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
-            if (e == 2) issue_edge(cv1);
-            if (e == 2) {
-              XCM_n = load_cm(cv1);
-              XCP_n = load_cp(cv1);
-            }
-          }
-          if (FULL || e < cnt) {
-            const int rw = e * P::RS;
-            // std::min(u(+1), u(-1)) per axis, one-sided at the edges
-            // (Eikonal3D.cpp:47-52)
-            const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
-            const T fe = pf[rw + p + 1];
-            T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
-            T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
-            if (e == 0 && c_first) uc = dc;
-            if ((FULL ? e == kLW - 1 : e == cnt - 1) && c_last) dc = uc;
-            if (cv.valid()) {
+          for (int e = 0; e < kLW; ++e) {
+            if (FULL || e < cnt) {
+              const int rw = e * P::RS;
+              // std::min(u(+1), u(-1)) per axis, one-sided at the edges
+              // (Eikonal3D.cpp:47-52)
+              const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
+              const T fe = pf[rw + p + 1];
+              T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
+              T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
+              if (e == 0 && c_first) uc = dc;
+              if ((FULL ? e == kLW - 1 : e == cnt - 1) && c_last) dc = uc;
               const T ux = FAST ? Real<T>::min1_(xp, xm) : smin(xp, xm);
               const T uy = FAST ? Real<T>::min1_(yp, ym) : smin(yp, ym);
               // std::min(u(k+1), u(k-1)): canonical c+1 is k+1 unless the
@@ -807,9 +808,11 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
               if constexpr (FAST) unsafe = (int)unsafe | (int)Real<T>::odd_(cand);
               OWN[e] = FAST ? Real<T>::min1_(cand, OWN[e]) : smin(cand, OWN[e]);
             }
-            if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
           }
         }
+#pragma unroll
+        for (int e = 0; e < kLW; ++e)
+          if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];
       };
       nodes(std::true_type{}, std::true_type{});
       redo_prev = __builtin_amdgcn_ballot_w64(unsafe) != 0;

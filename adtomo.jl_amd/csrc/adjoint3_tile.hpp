@@ -225,10 +225,20 @@ __device__ int phase_codes(const Ev<T>& E) {
   const int64_t nrows = (int64_t)m * n * nkb;
   const int ex[3] = {m, n, l};
   int nonz = 0;
-  for (int64_t r = threadIdx.x; r < nrows; r += blockDim.x) {
-    const int kb = (int)(r % nkb);
-    const int64_t ij = r / nkb;
-    const int j = (int)(ij % n), i = (int)(ij / n), k0 = kb * kT;
+  // row r = (i n + j) nkb + kb, advanced by the stride blockDim.x =
+  // (di n + dj) nkb + dkb with one carry per digit (a 64-bit division per
+  // row would cost more than the row's codes)
+  const int stride = (int)blockDim.x;
+  const int dkb = stride % nkb, dij = stride / nkb;
+  const int dj = dij % n, di = dij / n;
+  int kb = (int)threadIdx.x % nkb, j, i;
+  {
+    const int ij = (int)threadIdx.x / nkb;
+    j = ij % n;
+    i = ij / n;
+  }
+  for (int64_t r = threadIdx.x; r < nrows; r += stride) {
+    const int k0 = kb * kT;
     const int64_t q0 = (int64_t)i * E.sI + (int64_t)j * l + k0;
     T own[kT], u0r[kT], im[kT], ip[kT], jm[kT], jp[kT];
     row_load(E.u, q0, k0, l, own);
@@ -299,6 +309,13 @@ __device__ int phase_codes(const Ev<T>& E) {
           if (k0 + e < l) E.gu0[q0 + e] = gz[e];
       }
     }
+    kb += dkb;
+    const int c1 = kb >= nkb ? 1 : 0;
+    kb -= c1 * nkb;
+    j += dj + c1;  // < 2n: one carry
+    const int c2 = j >= n ? 1 : 0;
+    j -= c2 * n;
+    i += di + c2;
   }
   return nonz;
 }

@@ -225,8 +225,11 @@ __device__ __forceinline__ uint64_t bits_of(float x) { return (uint32_t)__float_
 
 // Order-preserving key of a travel-time difference for max-reductions with
 // integer atomics.  Non-negative IEEE values order like their bit patterns;
-// NaN maps to 0 so that it never wins, matching the reference's
-// `err = std::max(fabs(u - u_old), err)` which ignores NaN (Eikonal3D.cpp:79-81).
+// NaN maps to 0, so NaN differences are left out of the maximum.  The
+// reference's fold `err = std::max(fabs(u - u_old), err)` (Eikonal3D.cpp:79-81,
+// in memory order) restarts at a NaN difference instead: its err is the
+// maximum over the nodes after the last NaN difference, which can be
+// smaller, so an event with NaN differences can take more iterations here.
 __device__ __forceinline__ uint64_t maxkey(double x) {
   return (x == x) ? (uint64_t)__double_as_longlong(x) : 0ull;
 }

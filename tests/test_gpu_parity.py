@@ -269,41 +269,84 @@ def _odd_value_events():
     u0[35, 40, 60] = np.inf
     u0[30, 9, 40] = -0.0
     out.append((u0, f))
+    # k in two chunks (l = 80: chunk-edge values come from memory) and
+    # segments of 5 nodes (l = 40, the general step loop)
+    for shape, seed, src, kb in (((20, 24, 80), 33, (4, 6, 70), 60),
+                                 ((26, 18, 40), 34, (20, 3, 2), 30)):
+        r = synth.rng(seed)
+        f = r.uniform(0.3, 3.0, size=shape)
+        u0 = synth.point_source_u0(shape, src)
+        u0[src] = -0.0
+        f[10:13, 9:12, kb:kb + 3] = np.nan
+        u0[15, 4, kb + 5] = np.nan
+        out.append((u0, f))
     return out
+
+
+# Events 2 and 3 have NaN differences late in memory order, where the
+# reference's stop rule and the step loops' differ (see the xfail test
+# below): their sweeps are checked against the oracle run for the GPU's own
+# number of iterations.
+_NAN_ORDER = (2, 3)
+
+
+def _odd_ref(kind, u0, f, gpu_it, e):
+    fixed = e in _NAN_ORDER
+    tol, mi = (0.0, gpu_it) if fixed else (1e-4, 20)
+    if kind == "f64":
+        r = oracle.eikonal3d(u0, f, 0.5, tol, max_iter=mi)
+    elif kind == "m32":
+        r = oracle.eikonal3d_m32(u0, f, 0.5, tol, max_iter=mi)
+    else:
+        r = oracle.eikonal3d(u0, f, 0.5, tol, dtype=np.float32, max_iter=mi)
+    if not fixed:
+        assert gpu_it == r[1], e
+    return r[0]
 
 
 def test_forward_nan_and_negative_zero_inputs():
     import adtomo_hip as A
     ev = _odd_value_events()
-    refs = [oracle.eikonal3d(u0, f, 0.5, 1e-4) for u0, f in ev]
-    assert np.isnan(refs[1][0]).sum() > 100 and np.signbit(refs[0][0][refs[0][0] == 0]).sum() == 3
-    for (u0, f), (u_ref, it, err) in zip(ev, refs):
+    r0, r1 = oracle.eikonal3d(*ev[0], 0.5, 1e-4)[0], oracle.eikonal3d(*ev[1], 0.5, 1e-4)[0]
+    assert np.isnan(r1).sum() > 100 and np.signbit(r0[r0 == 0]).sum() == 3
+
+    def check(kind, e, u, it):
+        u_ref = _odd_ref(kind, *ev[e], it, e)
+        assert bits_equal(host(u), u_ref), (kind, e)
+        assert np.array_equal(np.signbit(host(u)), np.signbit(u_ref)), (kind, e)
+
+    for e, (u0, f) in enumerate(ev):
         u = A.eikonal3d(dev(u0), dev(f), 0.5, *u0.shape, 1e-4, False)
-        assert A.ops.last_info["iterations"] == it, u0.shape
-        assert bits_equal(host(u), u_ref), u0.shape
-        assert np.array_equal(np.signbit(host(u)), np.signbit(u_ref)), u0.shape
+        check("f64", e, u, A.ops.last_info["iterations"])
     res = A.eikonal3d_batched([dev(u0) for u0, _ in ev], [dev(f) for _, f in ev],
                               [0.5] * len(ev), [1e-4] * len(ev))
-    for e, (u_ref, it, _) in enumerate(refs):
-        assert res["info"][e]["iterations"] == it, e
-        assert bits_equal(host(res["u"][e]), u_ref), e
-        assert np.array_equal(np.signbit(host(res["u"][e])), np.signbit(u_ref)), e
+    for e in range(len(ev)):
+        check("f64", e, res["u"][e], res["info"][e]["iterations"])
     u32 = [dev(u0, np.float32) for u0, _ in ev]
     f32 = [dev(f, np.float32) for _, f in ev]
     # FP32 storage, FP64 solves
     res = A.eikonal3d_batched(u32, f32, [0.5] * len(ev), [1e-4] * len(ev), compute_f64=True)
-    for e, (u0, f) in enumerate(ev):
-        um, it, _ = oracle.eikonal3d_m32(u0, f, 0.5, 1e-4)
-        assert res["info"][e]["iterations"] == it, e
-        assert bits_equal(host(res["u"][e]), um), e
-        assert np.array_equal(np.signbit(host(res["u"][e])), np.signbit(um)), e
+    for e in range(len(ev)):
+        check("m32", e, res["u"][e], res["info"][e]["iterations"])
     # FP32
     res = A.eikonal3d_batched(u32, f32, [0.5] * len(ev), [1e-4] * len(ev))
-    for e, (u0, f) in enumerate(ev):
-        u_32, it, _ = oracle.eikonal3d(u0, f, 0.5, 1e-4, dtype=np.float32)
-        assert res["info"][e]["iterations"] == it, e
-        assert bits_equal(host(res["u"][e]), u_32), e
-        assert np.array_equal(np.signbit(host(res["u"][e])), np.signbit(u_32)), e
+    for e in range(len(ev)):
+        check("f32", e, res["u"][e], res["info"][e]["iterations"])
+
+
+@pytest.mark.xfail(strict=True, reason=(
+    "stop rule with NaN differences: the reference folds err = std::max(fabs(u - u_old), err) "
+    "in memory order (Eikonal3D.cpp:79-81), so a NaN difference restarts the maximum and err is "
+    "the maximum over the nodes after the last NaN difference; the step loops fold err with "
+    "NaN differences left out, a larger err that can take more iterations"))
+def test_forward_stop_rule_with_late_nan_differences():
+    import adtomo_hip as A
+    ev = _odd_value_events()
+    for e in _NAN_ORDER:
+        u0, f = ev[e]
+        _, it, _ = oracle.eikonal3d(u0, f, 0.5, 1e-4)
+        A.eikonal3d(dev(u0), dev(f), 0.5, *u0.shape, 1e-4, False)
+        assert A.ops.last_info["iterations"] == it, e
 
 
 @pytest.mark.parametrize("tol", [1e-3, 1e-6])

@@ -770,46 +770,53 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       bool unsafe = (int)odd | redo_prev | (int)Real<T>::odd_(xc_in) | (int)Real<T>::odd_(xd_in);
       auto nodes = [&](auto fast, auto issue) {
         constexpr bool FAST = decltype(fast)::value, ISSUE = decltype(issue)::value;
+        // the updates in two blocks of half a segment each (a lane's nodes
+        // share their column, so one branch covers a block, and the plane
+        // reads of a node can be issued during the updates before it); the
+        // next step's loads are issued before the first block (old values)
+        // and between the blocks (slowness, band edges)
+        auto half = [&](auto lo) {
+          constexpr int E0 = decltype(lo)::value;
+          if (cv.valid()) {
+#pragma unroll
+            for (int e = E0; e < E0 + kLW / 2; ++e) {
+              if (FULL || e < cnt) {
+                const int rw = e * P::RS;
+                // std::min(u(+1), u(-1)) per axis, one-sided at the edges
+                // (Eikonal3D.cpp:47-52)
+                const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
+                const T fe = pf[rw + p + 1];
+                T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
+                T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
+                if (e == 0 && c_first) uc = dc;
+                if ((FULL ? e == kLW - 1 : e == cnt - 1) && c_last) dc = uc;
+                const T ux = FAST ? Real<T>::min1_(xp, xm) : smin(xp, xm);
+                const T uy = FAST ? Real<T>::min1_(yp, ym) : smin(yp, ym);
+                // std::min(u(k+1), u(k-1)): canonical c+1 is k+1 unless the
+                // sweep runs k downwards
+                const T mc = FAST ? Real<T>::min1_(uc, dc) : FC ? smin(uc, dc) : smin(dc, uc);
+                T cand;
+                if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
+                  cand = (T)solution3d<double, FAST>(ux, uy, mc, fe, S.hd, unsafe);
+                else
+                  cand = solution3d<T, FAST>(ux, uy, mc, fe, S.h, unsafe);
+                // a NaN or -0 candidate (which the single-instruction minimum
+                // would order differently) sends the step to the exact pass
+                if constexpr (FAST) unsafe = (int)unsafe | (int)Real<T>::odd_(cand);
+                OWN[e] = FAST ? Real<T>::min1_(cand, OWN[e]) : smin(cand, OWN[e]);
+              }
+            }
+          }
+        };
+        if constexpr (ISSUE) seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
+        half(std::integral_constant<int, 0>{});
         if constexpr (ISSUE) {
-          seg_issue<T, FULL>(FD, S.src, seg_of(cv2), cnt, p);
           seg_issue<T, FULL>(FF, S.f, seg_of(cv1), cnt, p);
           issue_edge(cv1);
           XCM_n = load_cm(cv1);
           XCP_n = load_cp(cv1);
         }
-        // one branch around the whole segment (a lane's nodes share their
-        // column), so the updates form one block and the plane reads of a
-        // node can be issued during the updates before it
-        if (cv.valid()) {
-#pragma unroll
-          for (int e = 0; e < kLW; ++e) {
-            if (FULL || e < cnt) {
-              const int rw = e * P::RS;
-              // std::min(u(+1), u(-1)) per axis, one-sided at the edges
-              // (Eikonal3D.cpp:47-52)
-              const T xp = x_plus[rw], xm = x_minus[rw], yp = y_plus[rw], ym = y_minus[rw];
-              const T fe = pf[rw + p + 1];
-              T uc = (e == 0) ? xc_in : OWN[e == 0 ? 0 : e - 1];
-              T dc = (FULL ? e + 1 < kLW : e + 1 < cnt) ? OWN[e + 1 < kLW ? e + 1 : e] : xd_in;
-              if (e == 0 && c_first) uc = dc;
-              if ((FULL ? e == kLW - 1 : e == cnt - 1) && c_last) dc = uc;
-              const T ux = FAST ? Real<T>::min1_(xp, xm) : smin(xp, xm);
-              const T uy = FAST ? Real<T>::min1_(yp, ym) : smin(yp, ym);
-              // std::min(u(k+1), u(k-1)): canonical c+1 is k+1 unless the
-              // sweep runs k downwards
-              const T mc = FAST ? Real<T>::min1_(uc, dc) : FC ? smin(uc, dc) : smin(dc, uc);
-              T cand;
-              if constexpr (MIX)  // FP32 storage, FP64 local solve rounded once
-                cand = (T)solution3d<double, FAST>(ux, uy, mc, fe, S.hd, unsafe);
-              else
-                cand = solution3d<T, FAST>(ux, uy, mc, fe, S.h, unsafe);
-              // a NaN or -0 candidate (which the single-instruction minimum
-              // would order differently) sends the step to the exact pass
-              if constexpr (FAST) unsafe = (int)unsafe | (int)Real<T>::odd_(cand);
-              OWN[e] = FAST ? Real<T>::min1_(cand, OWN[e]) : smin(cand, OWN[e]);
-            }
-          }
-        }
+        half(std::integral_constant<int, kLW / 2>{});
 #pragma unroll
         for (int e = 0; e < kLW; ++e)
           if (FULL ? e == kLW - 1 : e == cnt - 1) xo = OWN[e];

@@ -67,6 +67,11 @@ template <> struct Real<double> {
   // NaN or -0 (class bits sNaN, qNaN, -0): the values on which min1_ /
   // max1_ may differ from the compare-and-select forms
   static __device__ __forceinline__ bool odd_(double x) { return __builtin_amdgcn_class(x, 0x23); }
+  // NaN, +-inf or |x| >= 2^240: an input (u0, f, h) with which the solve
+  // may produce NaN.  Below it (f h < 2^480, and an update never raises a
+  // value), travel times stay far below 2^511 on any grid that fits in
+  // memory, so every square in the solve stays finite.
+  static __device__ __forceinline__ bool wild_(double x) { return !(__builtin_fabs(x) < 0x1p240); }
 };
 template <> struct Real<float> {
   static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
@@ -85,6 +90,7 @@ template <> struct Real<float> {
     return r;
   }
   static __device__ __forceinline__ bool odd_(float x) { return __builtin_amdgcn_classf(x, 0x23); }
+  static __device__ __forceinline__ bool wild_(float x) { return !(__builtin_fabsf(x) < 0x1p20f); }
 };
 
 template <typename T> __device__ __forceinline__ T smin(T x, T y) { return (y < x) ? y : x; }
@@ -225,11 +231,11 @@ __device__ __forceinline__ uint64_t bits_of(float x) { return (uint32_t)__float_
 
 // Order-preserving key of a travel-time difference for max-reductions with
 // integer atomics.  Non-negative IEEE values order like their bit patterns;
-// NaN maps to 0, so NaN differences are left out of the maximum.  The
+// NaN maps to 0, so NaN differences are left out of the maximum: the
 // reference's fold `err = std::max(fabs(u - u_old), err)` (Eikonal3D.cpp:79-81,
-// in memory order) restarts at a NaN difference instead: its err is the
-// maximum over the nodes after the last NaN difference, which can be
-// smaller, so an event with NaN differences can take more iterations here.
+// in memory order) whenever no NaN difference occurs.  (With NaN differences
+// the reference's err is the maximum after the last one; the diagonal forward
+// then evaluates it in memory order instead.)
 __device__ __forceinline__ uint64_t maxkey(double x) {
   return (x == x) ? (uint64_t)__double_as_longlong(x) : 0ull;
 }

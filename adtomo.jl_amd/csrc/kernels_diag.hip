@@ -135,6 +135,7 @@ struct DSweep {
   int cs;       // canonical c of this pass's first node (k chunks)
   int Lfull;    // k extent of the event (g.L is the chunk's)
   int nseg;     // c segments (active waves), 1..kDW
+  bool first;   // the event's first sweep: its old values are u0
   rsrc_t u;     // travel times: new values written (and read back) here
   rsrc_t src;   // old values: u, or u0 in the very first sweep (u = u0,
                 // Eikonal3D.cpp:72, folded into that sweep)
@@ -457,6 +458,23 @@ __device__ __forceinline__ bool flight_odd(const Flight<T>& fl, bool full) {
   return odd;
 }
 
+// Does a transfer in flight (once arrived) hold a value with which the
+// solve may produce NaN (Real::wild_)?
+template <typename T>
+__device__ __forceinline__ bool flight_wild(const Flight<T>& fl, bool full) {
+  using P = Pieces<T>;
+  bool wild = false;
+#pragma unroll
+  for (int i = 0; i < P::LPC; ++i)
+#pragma unroll
+    for (int e = 0; e < P::EP; ++e) wild = (int)wild | (int)Real<T>::wild_(piece_get<T>(fl.pc[i], e));
+  if (!full) {
+#pragma unroll
+    for (int t = 0; t < P::EP - 1; ++t) wild = (int)wild | (int)Real<T>::wild_(fl.tail[t]);
+  }
+  return wild;
+}
+
 // Publishes canonical node 0 of every lane's segment from a transfer in
 // flight (once it has arrived) to dst[lane]: the old value the wave
 // upstream along k needs one past its own segment.
@@ -572,7 +590,7 @@ __device__ __forceinline__ void publish_progress(int* prog, int w, int done) {
 template <typename T, bool FULL, bool MIX, bool ERR, bool CH, bool FC>
 __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, int cnt,
                                                     lptr<T> xch, lptr<T> xold, lptr<T> planes,
-                                                    int* prog, int* abort_flag) {
+                                                    int* prog, int* flags) {
   using P = Pieces<T>;
   const DGeom& g = S.g;
   const int p = threadIdx.x & 63;
@@ -664,6 +682,8 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
   // one orders such values differently); so does the step after it, which
   // reads the values that step landed or produced
   int redo_prev = __builtin_amdgcn_ballot_w64(flight_odd(FD, FULL)) != 0;
+  if (S.first && __builtin_amdgcn_ballot_w64(flight_wild(FD, FULL)) != 0 && p == 0)
+    __hip_atomic_store(&flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   seg_issue<T, FULL>(FD, S.src, seg_of(cv1), cnt, p);
   seg_issue<T, FULL>(FF, S.f, seg_of(cv), cnt, p);
   issue_edge(cv);
@@ -699,6 +719,11 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     for (int j = 0; j < kLW; ++j)
       if (FULL || j < cnt) OWN[j] = px[j * P::RS + p + 1];
     bool odd = flight_odd(FD, FULL);
+    // the event's first sweep lands every node's u0 and f once: a value with
+    // which the solve may produce NaN sets flags[1]
+    if (S.first && __builtin_amdgcn_ballot_w64((int)flight_wild(FD, FULL) |
+                                               (int)flight_wild(FF, FULL)) != 0 && p == 0)
+      __hip_atomic_store(&flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     seg_land<T, FULL>(FD, px, cnt, fwd, p);
     seg_land<T, FULL>(FF, pf, cnt, fwd, p);
     XCM = XCM_n;
@@ -729,7 +754,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
     // v+2, slowness and band edges of v+1) are issued with the first
     // nodes, every step, so they have nearly a whole step to arrive.
     T xo = T(0);
-    wait_neighbours(prog, abort_flag, w, S.nseg, t);
+    wait_neighbours(prog, flags, w, S.nseg, t);
     {
       const T xc_in = w > 0 ? xch[((t - 1) & 1) * kDW * 64 + (w - 1) * 64 + p] : XCM;
       const T xd_in = has_x ? xold[((t - 1) & 1) * kDW * 64 + (w + 1) * 64 + p] : XCP;
@@ -826,6 +851,13 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
       if (__builtin_expect(redo_prev, 0)) {
         load_seg(OWN, S.src, cv, 0, cv.valid(), c0, cnt, L, S.tC);
         nodes(std::false_type{}, std::false_type{});
+        // a NaN result (a NaN candidate always comes here) sets flags[1]
+        bool nan_out = false;
+#pragma unroll
+        for (int e = 0; e < kLW; ++e)
+          if (FULL || e < cnt) nan_out = (int)nan_out | (int)(cv.valid() && OWN[e] != OWN[e]);
+        if (__builtin_amdgcn_ballot_w64(nan_out) != 0 && p == 0)
+          __hip_atomic_store(&flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
     // plane n: new values of v (all reads of v-1 are done)
@@ -866,7 +898,7 @@ __device__ __forceinline__ uint64_t diag_sweep_wave(const DSweep<T>& S, int w, i
 // key and writes the next snapshot.
 template <typename T, bool MIX, bool ERR>
 __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, lptr<T> xold,
-                                               lptr<T> planes, int* prog, int* abort_flag) {
+                                               lptr<T> planes, int* prog, int* flags) {
   // the wave index is uniform across the wave: keep it (and everything
   // derived from it) in scalar registers
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -882,7 +914,7 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
   cnt = cnt < 0 ? 0 : (cnt > S.g.Lw ? S.g.Lw : cnt);
   auto run = [&](auto full, auto ch, auto fc) {
     return diag_sweep_wave<T, decltype(full)::value, MIX, ERR, decltype(ch)::value,
-                           decltype(fc)::value>(S, w, cnt, xch, xold, planes, prog, abort_flag);
+                           decltype(fc)::value>(S, w, cnt, xch, xold, planes, prog, flags);
   };
   auto by_c = [&](auto full, auto ch) {
     return S.flipC ? run(full, ch, std::true_type{}) : run(full, ch, std::false_type{});
@@ -894,6 +926,47 @@ __device__ __forceinline__ uint64_t diag_sweep(const DSweep<T>& S, lptr<T> xch, 
 }
 
 // The whole forward of one event, run by one workgroup of kDW waves.
+// The stop rule's err as the reference folds it (Eikonal3D.cpp:79-81):
+// err = std::max(fabs(u[j] - u_old[j]), err) over j in memory order, where a
+// NaN difference makes err NaN and the next difference replaces it, so err
+// is the maximum over the nodes after the last NaN difference (NaN if that
+// is the last node).  Two passes of the workgroup: the last NaN difference,
+// then the maximum after it, writing the next snapshot dst = u (dst may be
+// snap: each node is read before it is written, by the same thread).
+template <typename T>
+__device__ T stop_err_in_order(const T* u, const T* snap, T* dst, int64_t N,
+                               unsigned long long* last_nan, uint64_t* red) {
+  if (threadIdx.x == 0) *last_nan = 0;
+  __syncthreads();
+  unsigned long long last = 0;  // 1 + index of the last NaN difference seen
+  for (int64_t j = threadIdx.x; j < N; j += blockDim.x) {
+    const T d = Real<T>::fabs_(u[j] - snap[j]);
+    if (d != d) last = (unsigned long long)j + 1;
+  }
+  if (last) atomicMax(last_nan, last);
+  __syncthreads();
+  const unsigned long long from = *last_nan;  // the first index that counts
+  uint64_t key = 0;
+  for (int64_t j = threadIdx.x; j < N; j += blockDim.x) {
+    const T uj = u[j];
+    if ((unsigned long long)j >= from) {
+      const uint64_t k = maxkey(Real<T>::fabs_(uj - snap[j]));
+      key = k > key ? k : key;
+    }
+    dst[j] = uj;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor(key, off);
+    key = o > key ? o : key;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
+  __syncthreads();
+  uint64_t r = 0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r = red[q] > r ? red[q] : r;
+  __syncthreads();
+  return from == (unsigned long long)N ? T(__builtin_nan("")) : key_to_real(r, T(0));
+}
+
 template <typename T, bool MIX>
 __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_iter) {
   __shared__ T xch[2 * kDW * 64];
@@ -901,12 +974,16 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   __shared__ T planes[kDW * 3 * Pieces<T>::PLANE];
   __shared__ uint64_t red[kDW];
   __shared__ int prog[kDW];
-  __shared__ int abort_flag;
+  // [0]: a bounded wait ran out (the kernel reports a fault); [1]: NaN may
+  // occur in u (an input Real::wild_ flags, or a NaN result): the stop rule
+  // is then evaluated in memory order, as the reference does
+  __shared__ int flags[2];
+  __shared__ unsigned long long last_nan;
   const int m = ev.m, n = ev.n, l = ev.l;
   if (!ev.fwd_diag) return;  // handled by the k-streaming kernel
   const int64_t N = (int64_t)m * n * l;
   const uint64_t bytes = (uint64_t)N * sizeof(T);
-  if (threadIdx.x == 0) abort_flag = 0;
+  if (threadIdx.x < 2) flags[threadIdx.x] = 0;
   __syncthreads();
   DSweep<T> S;
   S.g = make_dgeom(m, n, l);
@@ -925,6 +1002,8 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   asm("" : "+s"(hd));
   S.h = (T)hd;
   S.hd = hd;
+  // (read by the waves only after the barriers of the first sweep)
+  if (threadIdx.x == 0 && Real<T>::wild_(S.h)) flags[1] = 1;
   S.u = make_rsrc(ev.u, bytes);
   S.u0 = make_rsrc(ev.u0, bytes);
   S.f = make_rsrc(ev.f, bytes);
@@ -936,6 +1015,9 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   // may sweep sw+1 run chained after sweep sw (see "Chained sweeps")?
   auto chains = [&](int it_, int sw) -> bool {
     if (sw + 1 > 7 || (it_ == 0 && sw == 0) || nchunk != 1) return false;
+    // the stop-rule sweep runs alone when it must be evaluated in order
+    // (flags[1] is read after a barrier: the same value in every lane)
+    if (sw + 1 == 7 && __builtin_amdgcn_readfirstlane(flags[1])) return false;
     const int flip = kDir3[sw] ^ kDir3[sw + 1];
     if (flip & 4) return false;
     const bool flips_a = (flip & 1) ? S.g.a_is_i : !S.g.a_is_i;
@@ -943,6 +1025,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
   };
   for (it = 0; it < max_iter; ++it) {
     uint64_t key = 0;
+    bool in_order = false;  // err from stop_err_in_order
     for (int sw = 0; sw < 8;) {
       int nch = 1;
       while (nch < kChainMax && chains(it, sw + nch - 1)) ++nch;
@@ -955,6 +1038,8 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
       S.src = (it == 0 && sw == 0) ? S.u0 : S.u;
       const bool last = sw + nch - 1 == 7;  // the chain ends with the stop-rule sweep
       if (last) S.snap_src = make_rsrc(it == 0 ? ev.u0 : ev.uold, bytes);
+      if (last && __builtin_amdgcn_readfirstlane(flags[1])) in_order = true;
+      S.first = it == 0 && sw == 0;
       for (int ch = 0; ch < nchunk; ++ch) {
         S.cs = ch * kChunk;
         const int Lc = l - S.cs < kChunk ? l - S.cs : kChunk;
@@ -968,31 +1053,36 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
           S.cfl |= ((fa ? 1 : 0) | (fb ? 2 : 0)) << (2 * q);
         }
         S.coff = S.flipC ? l - 1 - S.cs : S.cs;
-        if (!last) {
+        if (!last || in_order) {
           diag_sweep<T, MIX, false>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes, prog,
-                                    &abort_flag);
+                                    flags);
         } else {
           // err = max |u - u_old| (Eikonal3D.cpp:79-81), taken as the last
           // sweep stores each node, which also writes u_old = u for the
           // next iteration (u_old of the first iteration is u0 itself)
           const uint64_t kc =
               diag_sweep<T, MIX, true>(S, (lptr<T>)xch, (lptr<T>)xold, (lptr<T>)planes, prog,
-                                       &abort_flag);
+                                       flags);
           key = kc > key ? kc : key;
         }
       }
       sw += nch;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      uint64_t o = __shfl_xor(key, off);
-      key = o > key ? o : key;
+    if (in_order) {
+      err = stop_err_in_order((const T*)ev.u, (const T*)(it == 0 ? ev.u0 : ev.uold),
+                              (T*)ev.uold, N, &last_nan, red);
+    } else {
+      for (int off = 32; off > 0; off >>= 1) {
+        uint64_t o = __shfl_xor(key, off);
+        key = o > key ? o : key;
+      }
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
+      __syncthreads();
+      uint64_t r = 0;
+      for (int q = 0; q < kDW; ++q) r = red[q] > r ? red[q] : r;
+      __syncthreads();
+      err = key_to_real(r, T(0));
     }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
-    __syncthreads();
-    uint64_t r = 0;
-    for (int q = 0; q < kDW; ++q) r = red[q] > r ? red[q] : r;
-    __syncthreads();
-    err = key_to_real(r, T(0));
     if (threadIdx.x == 0 && it < kMaxLoggedIters) ev.info->err_log[it] = (double)err;
     if (err < tol) {
       ++it;
@@ -1003,7 +1093,7 @@ __device__ __forceinline__ void fwd3_diag_event(const EventDesc& ev, int max_ite
     ev.info->iterations = it > max_iter ? max_iter : it;
     ev.info->converged = (err < tol) ? 1 : 0;
     ev.info->err = (double)err;
-    if (abort_flag) ev.info->fault = 1;
+    if (flags[0]) ev.info->fault = 1;
   }
 }
 

@@ -62,9 +62,9 @@ extern "C" {
 typedef struct adt_info {
   int32_t iterations;    /* forward iterations run (K) */
   int32_t converged;     /* 1 if the stop rule fired within the cap */
-  double err;            /* last stop-rule value (3D max-norm, 2D rel-L2; 3D leaves NaN
-                            differences out, where the reference's memory-order fold
-                            keeps the maximum after the last one) */
+  double err;            /* last stop-rule value (3D max-norm, 2D rel-L2); with NaN
+                            differences, the reference's memory-order value (the
+                            maximum after the last one) */
   int32_t ambiguous;     /* 2D: err within 1e-12 (rel) of the 1e-8 threshold */
   int32_t adj_sweeps;    /* adjoint Gauss-Seidel sweeps run */
   int32_t adj_converged; /* 1 if the adjoint reached its exact fixed point */

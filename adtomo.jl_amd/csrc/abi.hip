@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <limits>
 #include <list>
 #include <map>
 #include <mutex>
@@ -1308,17 +1309,22 @@ int fwd3_plain_impl(const T* u0, const T* f, T h, int64_t m, int64_t n, int64_t 
             u, f, h, (int)m, (int)n, (int)l, dirs[sw], sp, lo, cnt);
       }
     ADT_HIP(hipGetLastError());
-    ADT_HIP(hipMemsetAsync(dkey, 0, sizeof(unsigned long long), s));
-    adt::maxdiff_kernel<T><<<grid, 256, 0, s>>>(u, uold, N, dkey);
-    unsigned long long hk = 0;
-    ADT_HIP(hipMemcpyAsync(&hk, dkey, sizeof(hk), hipMemcpyDeviceToHost, s));
+    // err in the reference's memory order: the maximum after the last NaN
+    // difference (dkey[1]: 1 + its index; NaN if it is the last node)
+    ADT_HIP(hipMemsetAsync(dkey, 0, 2 * sizeof(unsigned long long), s));
+    adt::lastnan_kernel<T><<<grid, 256, 0, s>>>(u, uold, N, dkey + 1);
+    adt::maxdiff_kernel<T><<<grid, 256, 0, s>>>(u, uold, N, dkey + 1, dkey);
+    unsigned long long hk[2] = {0, 0};
+    ADT_HIP(hipMemcpyAsync(hk, dkey, sizeof(hk), hipMemcpyDeviceToHost, s));
     ADT_HIP(hipStreamSynchronize(s));
-    if (sizeof(T) == 8) {
+    if (hk[1] == (unsigned long long)N) {
+      err = std::numeric_limits<T>::quiet_NaN();
+    } else if (sizeof(T) == 8) {
       double e;
-      memcpy(&e, &hk, 8);
+      memcpy(&e, &hk[0], 8);
       err = (T)e;
     } else {
-      uint32_t k32 = (uint32_t)hk;
+      uint32_t k32 = (uint32_t)hk[0];
       float e;
       memcpy(&e, &k32, 4);
       err = (T)e;

@@ -129,7 +129,10 @@ __global__ void strided_copy_kernel(const T* src, T* dst, int64_t N, int64_t d1,
 template <typename T>
 __global__ void copy_kernel(T* dst, const T* src, int64_t N);
 template <typename T>
-__global__ void maxdiff_kernel(const T* a, const T* b, int64_t N, unsigned long long* out);
+__global__ void maxdiff_kernel(const T* a, const T* b, int64_t N, const unsigned long long* from,
+                               unsigned long long* out);
+template <typename T>
+__global__ void lastnan_kernel(const T* a, const T* b, int64_t N, unsigned long long* last);
 template <typename T>
 __global__ void adj3_prep_kernel(const T* u, const T* u0, const T* g, int m, int n, int l,
                                  uint8_t* code, T* lam, T* grad_u0);

@@ -89,11 +89,30 @@ __global__ void copy_kernel(T* __restrict__ dst, const T* __restrict__ src, int6
     dst[p] = src[p];
 }
 
+// The stop rule's err as the reference folds it (Eikonal3D.cpp:79-81):
+// err = std::max(fabs(a[p] - b[p]), err) over p in memory order, where a
+// NaN difference makes err NaN and the next difference replaces it, so err
+// is the maximum over p >= *from, *from = 1 + the index of the last NaN
+// difference (lastnan_kernel; err is NaN when that is N).
+template <typename T>
+__global__ void lastnan_kernel(const T* __restrict__ a, const T* __restrict__ b, int64_t N,
+                               unsigned long long* last) {
+  unsigned long long m = 0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const T d = Real<T>::fabs_(a[p] - b[p]);
+    if (d != d) m = (unsigned long long)p + 1;
+  }
+  if (m) atomicMax(last, m);
+}
+
 template <typename T>
 __global__ void maxdiff_kernel(const T* __restrict__ a, const T* __restrict__ b, int64_t N,
+                               const unsigned long long* __restrict__ from,
                                unsigned long long* out) {
+  const int64_t p0 = (int64_t)*from;
   uint64_t key = 0;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+  for (int64_t p = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
        p += (int64_t)gridDim.x * blockDim.x) {
     uint64_t q = maxkey(Real<T>::fabs_(a[p] - b[p]));
     key = q > key ? q : key;
@@ -346,7 +365,9 @@ __global__ void arith_kernel(int op, const T* __restrict__ a, const T* __restric
   template __global__ void fwd3_plane_kernel<T>(T*, const T*, T, int, int, int, int, int, int, \
                                                 int);                                           \
   template __global__ void copy_kernel<T>(T*, const T*, int64_t);                               \
-  template __global__ void maxdiff_kernel<T>(const T*, const T*, int64_t, unsigned long long*); \
+  template __global__ void lastnan_kernel<T>(const T*, const T*, int64_t, unsigned long long*); \
+  template __global__ void maxdiff_kernel<T>(const T*, const T*, int64_t,                       \
+                                             const unsigned long long*, unsigned long long*);   \
   template __global__ void adj3_prep_kernel<T>(const T*, const T*, const T*, int, int, int,     \
                                                uint8_t*, T*, T*);                               \
   template __global__ void adj3_plane_kernel<T>(const T*, const T*, const uint8_t*, T*, int,    \

@@ -338,6 +338,10 @@ def test_forward_stop_rule_in_memory_order():
         A.eikonal3d(dev(u0), dev(f), 0.5, *u0.shape, 1e-4, False)
         assert A.ops.last_info["iterations"] == it, e
         assert bits_equal(np.array([A.ops.last_info["err"]]), np.array([err])), e
+        # the plain per-hyperplane loop (lastnan_kernel + maxdiff_kernel)
+        A.eikonal3d_plain(dev(u0), dev(f), 0.5, 1e-4)
+        assert A.ops.last_info["iterations"] == it, e
+        assert bits_equal(np.array([A.ops.last_info["err"]]), np.array([err])), e
 
 
 @pytest.mark.parametrize("tol", [1e-3, 1e-6])

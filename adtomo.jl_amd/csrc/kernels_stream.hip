@@ -343,6 +343,7 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
   __shared__ T lds_new[2 * kW * 64];
   __shared__ T lds_old[2 * kW * 64];
   __shared__ uint64_t red[kW];
+  __shared__ unsigned long long last_nan;
   const EventDesc& ev = evs[blockIdx.x];
   const int m = ev.m, n = ev.n, l = ev.l;
   if (ev.fwd_diag) return;  // handled by the diagonal-streaming kernel
@@ -376,14 +377,34 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
       else
         stream_sweep<T, false, MIX>(S, (lptr<T>)lds_new, (lptr<T>)lds_old);
     }
-    // err = max |u - u_old| over the event (coalesced pass after the 8 sweeps)
+    // err = max |u - u_old| over the event (coalesced pass after the 8
+    // sweeps) as the reference folds it in memory order (Eikonal3D.cpp:79-81):
+    // a NaN difference restarts the maximum, so with NaN differences err is
+    // the maximum after the last one (NaN if that is the last node), taken
+    // by a second pass
     uint64_t key = 0;
+    unsigned long long from = 0;  // 1 + index of the last NaN difference
     {
       const gptr<const T> ua = (gptr<const T>)ev.u;
       const gptr<const T> sr = (gptr<const T>)((it == 0) ? ev.u0 : ev.uold);
+      if (threadIdx.x == 0) last_nan = 0;
+      __syncthreads();
+      unsigned long long last = 0;
       for (int64_t q = threadIdx.x; q < N; q += blockDim.x) {
-        const uint64_t kq = maxkey(Real<T>::fabs_(ua[q] - sr[q]));
+        const T d = Real<T>::fabs_(ua[q] - sr[q]);
+        if (d != d) last = (unsigned long long)q + 1;
+        const uint64_t kq = maxkey(d);
         key = kq > key ? kq : key;
+      }
+      if (last) atomicMax(&last_nan, last);
+      __syncthreads();
+      from = last_nan;
+      if (from) {
+        key = 0;
+        for (int64_t q = (int64_t)from + threadIdx.x; q < N; q += blockDim.x) {
+          const uint64_t kq = maxkey(Real<T>::fabs_(ua[q] - sr[q]));
+          key = kq > key ? kq : key;
+        }
       }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -395,7 +416,7 @@ __global__ void __launch_bounds__(kW * 64) fwd3_stream_kernel(const EventDesc* _
     uint64_t r = 0;
     for (int q = 0; q < kW; ++q) r = red[q] > r ? red[q] : r;
     __syncthreads();
-    err = key_to_real(r, T(0));
+    err = from == (unsigned long long)N ? T(__builtin_nan("")) : key_to_real(r, T(0));
     if (threadIdx.x == 0 && it < kMaxLoggedIters) ev.info->err_log[it] = (double)err;
     if (err < tol) {
       ++it;

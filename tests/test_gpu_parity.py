@@ -322,7 +322,7 @@ def test_forward_nan_and_negative_zero_inputs():
         check("f32", e, res["u"][e], res["info"][e]["iterations"])
 
 
-def test_forward_stop_rule_in_memory_order():
+def test_forward_stop_rule_in_memory_order(tmp_path):
     # events 2 and 3 have NaN differences late in memory order: the
     # reference folds err = std::max(fabs(u - u_old), err) in memory order
     # (Eikonal3D.cpp:79-81), so a NaN difference restarts the maximum and
@@ -342,6 +342,20 @@ def test_forward_stop_rule_in_memory_order():
         A.eikonal3d_plain(dev(u0), dev(f), 0.5, 1e-4)
         assert A.ops.last_info["iterations"] == it, e
         assert bits_equal(np.array([A.ops.last_info["err"]]), np.array([err])), e
+    # the k-streaming forward (launch setting diag_max_l = 0)
+    saved = str(tmp_path / "tune.json")
+    A.save_tune(saved)
+    try:
+        A.set_tune("float64", diag_max_l=0)
+        for e in (2, 3):
+            u0, f = ev[e]
+            u_ref, it, err = oracle.eikonal3d(u0, f, 0.5, 1e-4)
+            u = A.eikonal3d(dev(u0), dev(f), 0.5, *u0.shape, 1e-4, False)
+            assert A.ops.last_info["iterations"] == it, e
+            assert bits_equal(np.array([A.ops.last_info["err"]]), np.array([err])), e
+            assert bits_equal(host(u), u_ref), e
+    finally:
+        A.load_tune(saved)
 
 
 @pytest.mark.parametrize("tol", [1e-3, 1e-6])

@@ -49,7 +49,7 @@ FUSED_KERNEL = "fwd_adj3_diag_kernel<double, false>"
 # scaled by a copy kernel of known size and checked against the TCC EA
 # request counters; kernels not profiled at the current code report null
 TRAFFIC_SOURCE = "profiles/pmc_traffic_fwd3_diag_adj3_tile_256x128x128x64.md"
-TRAFFIC_BYTES_256 = {FWD_KERNEL: 119.01e9 + 54.44e9, ADJ_TILE_KERNEL: 37.90e9 + 7.98e9}
+TRAFFIC_BYTES_256 = {FWD_KERNEL: 122.87e9 + 53.99e9, ADJ_TILE_KERNEL: 37.91e9 + 7.98e9}
 
 
 def log(*a):
